@@ -1,0 +1,445 @@
+/*
+ * N-API addon: the thin boundary between the Node host (js/solver.js) and the C-ABI of libaht.so (include/aht.h).
+ * One JS function per aht_* entry point, typed arrays passed by pointer, no copies beyond what the C-ABI does.
+ * step() and benchmark() run aht_step + aht_sync on a libuv worker thread and return a Promise (the reference's
+ * step(n) resolves on queue completion, js/live/solver.js:401-406). Errors throw with aht_last_error's message.
+ *
+ * The solver handle is a JS external; the host serialises calls per solver (the C-ABI is one caller per solver).
+ */
+#define NAPI_VERSION 4
+#include <node_api.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "aht.h"
+
+#define CHECK_NAPI(call)                                                         \
+  do {                                                                           \
+    if ((call) != napi_ok) {                                                     \
+      napi_throw_error(env, NULL, "aht addon: N-API call failed: " #call);       \
+      return NULL;                                                               \
+    }                                                                            \
+  } while (0)
+
+typedef struct {
+  aht_solver *s;
+} handle_t;
+
+static void finalize_handle(napi_env env, void *data, void *hint) {
+  (void)env; (void)hint;
+  handle_t *h = (handle_t *)data;
+  if (h->s) aht_destroy(h->s);
+  free(h);
+}
+
+static napi_value throw_rc(napi_env env, aht_solver *s, int rc) {
+  char buf[600];
+  snprintf(buf, sizeof buf, "aht: %s (%s)", s ? aht_last_error(s) : "", aht_error_string(rc));
+  napi_throw_error(env, NULL, buf);
+  return NULL;
+}
+
+static napi_value undef(napi_env env) {
+  napi_value u;
+  napi_get_undefined(env, &u);
+  return u;
+}
+
+static int get_args(napi_env env, napi_callback_info info, size_t want, napi_value *argv) {
+  size_t argc = want;
+  if (napi_get_cb_info(env, info, &argc, argv, NULL, NULL) != napi_ok) return 0;
+  if (argc < want) {
+    napi_throw_type_error(env, NULL, "aht addon: too few arguments");
+    return 0;
+  }
+  return 1;
+}
+
+static handle_t *get_handle(napi_env env, napi_value v) {
+  void *p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p || !((handle_t *)p)->s) {
+    napi_throw_error(env, NULL, "aht addon: invalid or destroyed solver handle");
+    return NULL;
+  }
+  return (handle_t *)p;
+}
+
+/* pointer and byte length of a typed array; NULL for null/undefined */
+static int get_buf(napi_env env, napi_value v, void **ptr, size_t *bytes) {
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t == napi_null || t == napi_undefined) { *ptr = NULL; *bytes = 0; return 1; }
+  bool is_ta = false;
+  napi_is_typedarray(env, v, &is_ta);
+  if (!is_ta) { napi_throw_type_error(env, NULL, "aht addon: expected a typed array"); return 0; }
+  napi_typedarray_type ty;
+  size_t len, off;
+  napi_value ab;
+  void *data;
+  if (napi_get_typedarray_info(env, v, &ty, &len, &data, &ab, &off) != napi_ok) return 0;
+  size_t el = (ty == napi_int8_array || ty == napi_uint8_array || ty == napi_uint8_clamped_array) ? 1
+            : (ty == napi_int16_array || ty == napi_uint16_array) ? 2
+            : (ty == napi_float64_array || ty == napi_bigint64_array || ty == napi_biguint64_array) ? 8 : 4;
+  *ptr = data;
+  *bytes = len * el;
+  return 1;
+}
+
+static int get_i32(napi_env env, napi_value v, int32_t *out) {
+  if (napi_get_value_int32(env, v, out) != napi_ok) { napi_throw_type_error(env, NULL, "aht addon: expected a number"); return 0; }
+  return 1;
+}
+
+static int get_f64(napi_env env, napi_value v, double *out) {
+  if (napi_get_value_double(env, v, out) != napi_ok) { napi_throw_type_error(env, NULL, "aht addon: expected a number"); return 0; }
+  return 1;
+}
+
+static int get_prop_i32(napi_env env, napi_value o, const char *k, int32_t *out, int32_t dflt) {
+  bool has = false;
+  napi_has_named_property(env, o, k, &has);
+  if (!has) { *out = dflt; return 1; }
+  napi_value v;
+  napi_get_named_property(env, o, k, &v);
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t == napi_undefined || t == napi_null) { *out = dflt; return 1; }
+  return get_i32(env, v, out);
+}
+
+static int get_prop_f64(napi_env env, napi_value o, const char *k, double *out) {
+  napi_value v;
+  if (napi_get_named_property(env, o, k, &v) != napi_ok) return 0;
+  return get_f64(env, v, out);
+}
+
+static void set_num(napi_env env, napi_value o, const char *k, double x) {
+  napi_value v;
+  napi_create_double(env, x, &v);
+  napi_set_named_property(env, o, k, v);
+}
+
+/* create({nx, ny, nz, storage, device, block}) -> handle */
+static napi_value js_create(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  aht_dims d;
+  memset(&d, 0, sizeof d);
+  if (!get_prop_i32(env, argv[0], "nx", &d.nx, 0) || !get_prop_i32(env, argv[0], "ny", &d.ny, 0) ||
+      !get_prop_i32(env, argv[0], "nz", &d.nz, 0) || !get_prop_i32(env, argv[0], "storage", &d.storage, 0) ||
+      !get_prop_i32(env, argv[0], "device", &d.device, 0) || !get_prop_i32(env, argv[0], "block", &d.block, 0))
+    return NULL;
+  aht_solver *s = NULL;
+  int rc = aht_create(&d, &s);
+  if (rc != AHT_OK) {
+    napi_value r = throw_rc(env, s, rc);
+    if (s) aht_destroy(s);
+    return r;
+  }
+  handle_t *h = (handle_t *)calloc(1, sizeof *h);
+  h->s = s;
+  napi_value ext;
+  CHECK_NAPI(napi_create_external(env, h, finalize_handle, NULL, &ext));
+  return ext;
+}
+
+static napi_value js_destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  void *p = NULL;
+  napi_get_value_external(env, argv[0], &p);
+  handle_t *h = (handle_t *)p;
+  if (h && h->s) { aht_destroy(h->s); h->s = NULL; }
+  return undef(env);
+}
+
+/* setColumns(h, kt u32, ktop u32, cd f32, q f32) */
+static napi_value js_set_columns(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  if (!get_args(env, info, 5, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  void *p[4];
+  size_t b[4];
+  for (int k = 0; k < 4; k++) if (!get_buf(env, argv[1 + k], &p[k], &b[k])) return NULL;
+  int rc = aht_set_columns(h->s, (const uint32_t *)p[0], (const uint32_t *)p[1], (const float *)p[2], (const float *)p[3]);
+  return rc ? throw_rc(env, h->s, rc) : undef(env);
+}
+
+/* setParams(h, {tau0..sigmax, inflow_w.., nsp, nsw, oclip}, prof f32[nz*4]) */
+static napi_value js_set_params(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  aht_params P;
+  memset(&P, 0, sizeof P);
+  static const char *fk[8] = {"tau0", "cs2smag", "gbeta", "prt", "kfloor", "momfac", "srcscale", "sigmax"};
+  static const char *ik[7] = {"inflow_w", "inflow_e", "inflow_s", "inflow_n", "nsp", "nsw", "oclip"};
+  float *pf = &P.tau0;
+  int32_t *pi = &P.inflow_w;
+  for (int k = 0; k < 8; k++) { double x; if (!get_prop_f64(env, argv[1], fk[k], &x)) return NULL; pf[k] = (float)x; }
+  for (int k = 0; k < 7; k++) if (!get_prop_i32(env, argv[1], ik[k], &pi[k], 0)) return NULL;
+  void *pr;
+  size_t b;
+  if (!get_buf(env, argv[2], &pr, &b)) return NULL;
+  int rc = aht_set_params(h->s, &P, (const float *)pr);
+  return rc ? throw_rc(env, h->s, rc) : undef(env);
+}
+
+#define SIMPLE(name, call)                                               \
+  static napi_value name(napi_env env, napi_callback_info info) {        \
+    napi_value argv[1];                                                  \
+    if (!get_args(env, info, 1, argv)) return NULL;                      \
+    handle_t *h = get_handle(env, argv[0]);                              \
+    if (!h) return NULL;                                                 \
+    int rc = call(h->s);                                                 \
+    return rc ? throw_rc(env, h->s, rc) : undef(env);                    \
+  }
+SIMPLE(js_init_flow, aht_init_flow)
+SIMPLE(js_reset_heat, aht_reset_heat)
+SIMPLE(js_mean_reset, aht_mean_reset)
+SIMPLE(js_sync, aht_sync)
+
+static napi_value js_reinit_cells(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  void *p;
+  size_t b;
+  if (!get_buf(env, argv[1], &p, &b)) return NULL;
+  int rc = aht_reinit_cells(h->s, (const uint32_t *)p);
+  return rc ? throw_rc(env, h->s, rc) : undef(env);
+}
+
+/* ---- asynchronous stepping ------------------------------------------------------------------------------------ */
+typedef struct {
+  napi_async_work work;
+  napi_deferred deferred;
+  napi_ref keep;     /* keeps the handle's external alive while the work runs */
+  handle_t *h;
+  int32_t n;
+  uint32_t flags;
+  int bench;
+  double ms;
+  int rc;
+  char err[512];
+} step_job;
+
+static void step_execute(napi_env env, void *data) {
+  (void)env;
+  step_job *j = (step_job *)data;
+  if (j->bench) {
+    j->rc = aht_benchmark(j->h->s, j->n, &j->ms);
+  } else {
+    j->rc = aht_step(j->h->s, j->n, j->flags);
+    if (j->rc == AHT_OK) j->rc = aht_sync(j->h->s);
+  }
+  if (j->rc) snprintf(j->err, sizeof j->err, "aht: %s (%s)", aht_last_error(j->h->s), aht_error_string(j->rc));
+}
+
+static void step_complete(napi_env env, napi_status status, void *data) {
+  step_job *j = (step_job *)data;
+  if (status != napi_ok || j->rc) {
+    napi_value msg, e;
+    napi_create_string_utf8(env, j->rc ? j->err : "aht: step cancelled", NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, NULL, msg, &e);
+    napi_reject_deferred(env, j->deferred, e);
+  } else {
+    napi_value v;
+    if (j->bench) napi_create_double(env, j->ms, &v);
+    else napi_get_undefined(env, &v);
+    napi_resolve_deferred(env, j->deferred, v);
+  }
+  napi_delete_reference(env, j->keep);
+  napi_delete_async_work(env, j->work);
+  free(j);
+}
+
+static napi_value start_job(napi_env env, napi_value hv, handle_t *h, int32_t n, uint32_t flags, int bench) {
+  step_job *j = (step_job *)calloc(1, sizeof *j);
+  j->h = h; j->n = n; j->flags = flags; j->bench = bench;
+  napi_value promise, name;
+  CHECK_NAPI(napi_create_promise(env, &j->deferred, &promise));
+  CHECK_NAPI(napi_create_reference(env, hv, 1, &j->keep));
+  CHECK_NAPI(napi_create_string_utf8(env, bench ? "aht_benchmark" : "aht_step", NAPI_AUTO_LENGTH, &name));
+  CHECK_NAPI(napi_create_async_work(env, NULL, name, step_execute, step_complete, j, &j->work));
+  CHECK_NAPI(napi_queue_async_work(env, j->work));
+  return promise;
+}
+
+/* step(h, n, flags) -> Promise<void> */
+static napi_value js_step(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  int32_t n, fl;
+  if (!get_i32(env, argv[1], &n) || !get_i32(env, argv[2], &fl)) return NULL;
+  return start_job(env, argv[0], h, n, (uint32_t)fl, 0);
+}
+
+/* benchmark(h, n) -> Promise<ms> */
+static napi_value js_benchmark(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  int32_t n;
+  if (!get_i32(env, argv[1], &n)) return NULL;
+  return start_job(env, argv[0], h, n, 0, 1);
+}
+
+/* readSurface(h, conv, th f32, u f32|null, v f32|null) */
+static napi_value js_read_surface(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  if (!get_args(env, info, 5, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  double conv;
+  void *p[3];
+  size_t b[3];
+  if (!get_f64(env, argv[1], &conv)) return NULL;
+  for (int k = 0; k < 3; k++) if (!get_buf(env, argv[2 + k], &p[k], &b[k])) return NULL;
+  int rc = aht_read_surface(h->s, (float)conv, (float *)p[0], (float *)p[1], (float *)p[2]);
+  return rc ? throw_rc(env, h->s, rc) : undef(env);
+}
+
+/* readVolume(h, nzv, volmax, vol u8) -> thetaMax */
+static napi_value js_read_volume(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  int32_t nzv;
+  double vm;
+  void *p;
+  size_t b;
+  if (!get_i32(env, argv[1], &nzv) || !get_f64(env, argv[2], &vm) || !get_buf(env, argv[3], &p, &b)) return NULL;
+  float tmax = 0.0f;
+  int rc = aht_read_volume(h->s, nzv, (float)vm, (uint8_t *)p, &tmax);
+  if (rc) return throw_rc(env, h->s, rc);
+  napi_value r;
+  napi_create_double(env, tmax, &r);
+  return r;
+}
+
+/* stats(h) -> {sum_theta, ...} */
+static napi_value js_stats(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  aht_diag d;
+  int rc = aht_stats(h->s, &d);
+  if (rc) return throw_rc(env, h->s, rc);
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "sum_theta", d.sum_theta);
+  set_num(env, o, "sum_ke", d.sum_ke);
+  set_num(env, o, "sum_drho", d.sum_drho);
+  set_num(env, o, "n_fluid", d.n_fluid);
+  set_num(env, o, "theta_max", d.theta_max);
+  set_num(env, o, "umax", d.umax);
+  set_num(env, o, "non_finite", (double)d.non_finite);
+  return o;
+}
+
+/* readMean(h, surf3 f64[3*NXY] | null) -> budget object */
+static napi_value js_read_mean(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  void *p;
+  size_t b;
+  if (!get_buf(env, argv[1], &p, &b)) return NULL;
+  aht_budget bg;
+  int rc = aht_read_mean(h->s, (double *)p, &bg);
+  if (rc) return throw_rc(env, h->s, rc);
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "n_acc", (double)bg.n_acc);
+  set_num(env, o, "heat_in", bg.heat_in);
+  set_num(env, o, "heat_out", bg.heat_out);
+  set_num(env, o, "content", bg.content);
+  set_num(env, o, "q_w", bg.q_w);
+  set_num(env, o, "q_e", bg.q_e);
+  set_num(env, o, "q_s", bg.q_s);
+  set_num(env, o, "q_n", bg.q_n);
+  set_num(env, o, "mass_imbalance", bg.mass_imbalance);
+  return o;
+}
+
+/* readState(h, f f32|null, g f32|null), writeState(h, f|null, g|null) */
+static napi_value js_state(napi_env env, napi_callback_info info, int write) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  void *f, *g;
+  size_t bf, bg;
+  if (!get_buf(env, argv[1], &f, &bf) || !get_buf(env, argv[2], &g, &bg)) return NULL;
+  int rc = write ? aht_write_state(h->s, (const float *)f, (const float *)g) : aht_read_state(h->s, (float *)f, (float *)g);
+  return rc ? throw_rc(env, h->s, rc) : undef(env);
+}
+static napi_value js_read_state(napi_env env, napi_callback_info info) { return js_state(env, info, 0); }
+static napi_value js_write_state(napi_env env, napi_callback_info info) { return js_state(env, info, 1); }
+
+static napi_value js_read_out(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  void *p;
+  size_t b;
+  if (!get_buf(env, argv[1], &p, &b)) return NULL;
+  int rc = aht_read_out(h->s, (float *)p);
+  return rc ? throw_rc(env, h->s, rc) : undef(env);
+}
+
+/* footprint({nx, ny, nz, storage}) -> bytes */
+static napi_value js_footprint(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  aht_dims d;
+  memset(&d, 0, sizeof d);
+  if (!get_prop_i32(env, argv[0], "nx", &d.nx, 0) || !get_prop_i32(env, argv[0], "ny", &d.ny, 0) ||
+      !get_prop_i32(env, argv[0], "nz", &d.nz, 0) || !get_prop_i32(env, argv[0], "storage", &d.storage, 0))
+    return NULL;
+  napi_value r;
+  napi_create_double(env, (double)aht_footprint_bytes(&d), &r);
+  return r;
+}
+
+static napi_value js_steps_done(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  napi_value r;
+  napi_create_double(env, (double)aht_steps_done(h->s), &r);
+  return r;
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+  struct { const char *name; napi_callback cb; } fns[] = {
+    {"create", js_create}, {"destroy", js_destroy}, {"setColumns", js_set_columns}, {"setParams", js_set_params},
+    {"initFlow", js_init_flow}, {"resetHeat", js_reset_heat}, {"reinitCells", js_reinit_cells},
+    {"step", js_step}, {"sync", js_sync}, {"benchmark", js_benchmark}, {"readSurface", js_read_surface},
+    {"readVolume", js_read_volume}, {"stats", js_stats}, {"meanReset", js_mean_reset}, {"readMean", js_read_mean},
+    {"readState", js_read_state}, {"writeState", js_write_state}, {"readOut", js_read_out},
+    {"footprint", js_footprint}, {"stepsDone", js_steps_done},
+  };
+  for (size_t k = 0; k < sizeof fns / sizeof fns[0]; k++) {
+    napi_value f;
+    CHECK_NAPI(napi_create_function(env, fns[k].name, NAPI_AUTO_LENGTH, fns[k].cb, NULL, &f));
+    CHECK_NAPI(napi_set_named_property(env, exports, fns[k].name, f));
+  }
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

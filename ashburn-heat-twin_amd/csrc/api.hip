@@ -1,0 +1,450 @@
+// C-ABI of the coupled lattice-Boltzmann step (include/aht.h). Owns the device buffers of one solver: two f32
+// population lattices per distribution (A/B ping-pong, js/live/solver.js:149-150, :184), the packed columns, the
+// profile block and the FP64 accumulators. Every failure returns a negative code and leaves a message.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aht.h"
+#include "kernels.h"
+
+struct aht_solver {
+  aht_dims dims{};
+  int device = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  aht::Grid g{};
+  int block = aht::STEP_BLOCK;
+  float *f[2] = {nullptr, nullptr}, *gq[2] = {nullptr, nullptr};
+  uint4 *colv = nullptr;
+  uint32_t *ktop = nullptr, *oldk = nullptr;
+  float4 *prof = nullptr;
+  double *acc_surf = nullptr, *acc_face = nullptr, *acc_content = nullptr, *red_out = nullptr;
+  uint32_t content_cap = 0;
+  double *stat_part = nullptr;
+  float *out4 = nullptr, *surf3 = nullptr;
+  uint32_t *vol = nullptr, *tmax = nullptr;
+  int parity = 0;
+  bool has_cols = false, has_params = false, has_out = false;
+  int64_t steps = 0, n_acc = 0;
+  std::vector<float> h_q;
+  std::string err;
+};
+
+namespace {
+
+int fail(aht_solver *s, int code, const char *fmt, ...) {
+  if (s) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    s->err = buf;
+  }
+  return code;
+}
+
+#define HIP_TRY(s, call)                                                                                    \
+  do {                                                                                                      \
+    hipError_t e_ = (call);                                                                                 \
+    if (e_ != hipSuccess) return fail((s), e_ == hipErrorOutOfMemory ? AHT_ENOMEM : AHT_EDEVICE, "%s: %s", \
+                                      #call, hipGetErrorString(e_));                                        \
+  } while (0)
+
+size_t face_total(int nx, int ny, int nz) { return aht::face_offset(4, nx, ny, nz); }
+
+uint32_t max_content_slots(const aht_dims &d, int block) {
+  const uint32_t nxy = (uint32_t)d.nx * (uint32_t)d.ny;
+  return ((nxy + block - 1) / block) * (uint32_t)d.nz * (uint32_t)(block / 64);
+}
+
+int check_ready(aht_solver *s) {
+  if (!s) return AHT_EINVAL;
+  if (!s->has_cols || !s->has_params) return fail(s, AHT_ESTATE, "columns and parameters must be set first");
+  return AHT_OK;
+}
+
+int set_device(aht_solver *s) {
+  HIP_TRY(s, hipSetDevice(s->device));
+  return AHT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *aht_error_string(int code) {
+  switch (code) {
+    case AHT_OK: return "ok";
+    case AHT_EINVAL: return "invalid argument";
+    case AHT_ENOMEM: return "device out of memory";
+    case AHT_EDEVICE: return "HIP runtime error";
+    case AHT_ESTATE: return "call out of order";
+    default: return "unknown error";
+  }
+}
+
+const char *aht_last_error(const aht_solver *s) { return s ? s->err.c_str() : "null solver"; }
+
+int64_t aht_footprint_bytes(const aht_dims *d) {
+  if (!d || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return -1;
+  const int64_t N = (int64_t)d->nx * d->ny * d->nz, NXY = (int64_t)d->nx * d->ny;
+  const int64_t pop = 2 * 26 * N * 4;
+  const int64_t cols = NXY * (16 + 4 + 4) + (int64_t)d->nz * 16;
+  const int64_t acc = 3 * NXY * 8 + (int64_t)face_total(d->nx, d->ny, d->nz) * 8 + (int64_t)max_content_slots(*d, 256) * 8;
+  const int64_t misc = ((N + 255) / 256) * 64 + 3 * NXY * 4 + (N + 3) / 4 * 4 / 1 + 4096;
+  return pop + cols + acc + misc;
+}
+
+int aht_create(const aht_dims *dims, aht_solver **out) {
+  if (!dims || !out) return AHT_EINVAL;
+  *out = nullptr;
+  const aht_dims d = *dims;
+  if (d.nx < 2 || d.ny < 2 || d.nz < 2) return AHT_EINVAL;
+  const uint64_t N = (uint64_t)d.nx * d.ny * d.nz;
+  if (N >= (1ull << 31)) return AHT_EINVAL;
+  if (d.storage != AHT_STORE_F32) return AHT_EINVAL;
+  const int block = d.block ? d.block : aht::STEP_BLOCK;
+  if (block < 64 || block > 256 || block % 64) return AHT_EINVAL;
+  aht_solver *s = new aht_solver();
+  s->dims = d;
+  s->device = d.device;
+  s->block = block;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= d.device || d.device < 0) {
+    // keep the solver so the caller can read the message, then let it be destroyed
+    *out = s;
+    return fail(s, AHT_EDEVICE, "no HIP device %d (found %d)", d.device, ndev);
+  }
+  *out = s;
+  int rc = set_device(s);
+  if (rc) return rc;
+  HIP_TRY(s, hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+  HIP_TRY(s, hipEventCreate(&s->ev0));
+  HIP_TRY(s, hipEventCreate(&s->ev1));
+  const uint32_t NXY = (uint32_t)d.nx * d.ny;
+  aht::Grid &g = s->g;
+  g.nx = d.nx; g.ny = d.ny; g.nz = d.nz; g.kmin = d.nz; g.NXY = NXY; g.N = (uint32_t)N;
+  g.inv_nx = 1.0f / (float)d.nx;
+  for (int p = 0; p < 2; p++) {
+    HIP_TRY(s, hipMalloc(&s->f[p], 19 * N * sizeof(float)));
+    HIP_TRY(s, hipMalloc(&s->gq[p], 7 * N * sizeof(float)));
+    HIP_TRY(s, hipMemsetAsync(s->f[p], 0, 19 * N * sizeof(float), s->st));
+    HIP_TRY(s, hipMemsetAsync(s->gq[p], 0, 7 * N * sizeof(float), s->st));
+  }
+  HIP_TRY(s, hipMalloc(&s->colv, NXY * sizeof(uint4)));
+  HIP_TRY(s, hipMalloc(&s->ktop, NXY * sizeof(uint32_t)));
+  HIP_TRY(s, hipMalloc(&s->oldk, NXY * sizeof(uint32_t)));
+  HIP_TRY(s, hipMalloc(&s->prof, d.nz * sizeof(float4)));
+  HIP_TRY(s, hipMalloc(&s->acc_surf, 3 * (size_t)NXY * sizeof(double)));
+  HIP_TRY(s, hipMalloc(&s->acc_face, face_total(d.nx, d.ny, d.nz) * sizeof(double)));
+  s->content_cap = max_content_slots(d, block);
+  HIP_TRY(s, hipMalloc(&s->acc_content, s->content_cap * sizeof(double)));
+  HIP_TRY(s, hipMalloc(&s->red_out, 8 * sizeof(double)));
+  HIP_TRY(s, hipMalloc(&s->stat_part, 8 * (size_t)((N + 255) / 256) * sizeof(double)));
+  HIP_TRY(s, hipMalloc(&s->surf3, 3 * (size_t)NXY * sizeof(float)));
+  HIP_TRY(s, hipMalloc(&s->vol, ((N + 3) / 4) * sizeof(uint32_t)));
+  HIP_TRY(s, hipMalloc(&s->tmax, sizeof(uint32_t)));
+  s->g.colv = s->colv; s->g.ktop = s->ktop; s->g.prof = s->prof;
+  rc = aht_mean_reset(s);
+  if (rc) return rc;
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  return AHT_OK;
+}
+
+void aht_destroy(aht_solver *s) {
+  if (!s) return;
+  if (s->st) {
+    (void)hipSetDevice(s->device);
+    (void)hipStreamSynchronize(s->st);
+  }
+  void *bufs[] = {s->f[0], s->f[1], s->gq[0], s->gq[1], s->colv, s->ktop, s->oldk, s->prof, s->acc_surf, s->acc_face,
+                  s->acc_content, s->red_out, s->stat_part, s->out4, s->surf3, s->vol, s->tmax};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  delete s;
+}
+
+int aht_set_columns(aht_solver *s, const uint32_t *kt, const uint32_t *ktop, const float *cd, const float *q) {
+  if (!s || !kt || !ktop || !cd || !q) return s ? fail(s, AHT_EINVAL, "null column array") : AHT_EINVAL;
+  const uint32_t NXY = s->g.NXY;
+  const int nz = s->g.nz;
+  std::vector<uint4> cv(NXY);
+  int kmin = nz;
+  for (uint32_t c = 0; c < NXY; c++) {
+    if (kt[c] < 1 || kt[c] > ktop[c] || (int)ktop[c] > nz)
+      return fail(s, AHT_EINVAL, "column %u: need 1 <= kt (%u) <= ktop (%u) <= nz (%d)", c, kt[c], ktop[c], nz);
+    if (!std::isfinite(cd[c]) || !std::isfinite(q[c])) return fail(s, AHT_EINVAL, "column %u: non-finite cd or q", c);
+    uint32_t cdb, qb;
+    memcpy(&cdb, &cd[c], 4);
+    memcpy(&qb, &q[c], 4);
+    cv[c] = make_uint4(kt[c], ktop[c], cdb, qb);
+    kmin = std::min(kmin, (int)ktop[c]);
+  }
+  int rc = set_device(s);
+  if (rc) return rc;
+  HIP_TRY(s, hipMemcpyAsync(s->colv, cv.data(), NXY * sizeof(uint4), hipMemcpyHostToDevice, s->st));
+  HIP_TRY(s, hipMemcpyAsync(s->ktop, ktop, NXY * sizeof(uint32_t), hipMemcpyHostToDevice, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  s->g.kmin = kmin;
+  s->h_q.assign(q, q + NXY);
+  s->has_cols = true;
+  return AHT_OK;
+}
+
+int aht_set_params(aht_solver *s, const aht_params *p, const float *prof) {
+  if (!s || !p || !prof) return s ? fail(s, AHT_EINVAL, "null params") : AHT_EINVAL;
+  if (p->nsp < 1 || p->nsw < 1) return fail(s, AHT_EINVAL, "nsp and nsw must be >= 1");
+  const float *pf = &p->tau0;
+  for (int k = 0; k < 8; k++)
+    if (!std::isfinite(pf[k])) return fail(s, AHT_EINVAL, "non-finite parameter %d", k);
+  static_assert(sizeof(aht::Params) == sizeof(aht_params), "parameter block layout");
+  memcpy(&s->g.P, p, sizeof(aht_params));
+  int rc = set_device(s);
+  if (rc) return rc;
+  HIP_TRY(s, hipMemcpyAsync(s->prof, prof, s->g.nz * sizeof(float4), hipMemcpyHostToDevice, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  s->has_params = true;
+  return AHT_OK;
+}
+
+int aht_init_flow(aht_solver *s) {
+  int rc = check_ready(s);
+  if (rc) return rc;
+  if ((rc = set_device(s))) return rc;
+  HIP_TRY(s, aht::launch_init(s->g, s->f[0], s->f[1], s->st));
+  return AHT_OK;
+}
+
+int aht_reset_heat(aht_solver *s) {
+  if (!s) return AHT_EINVAL;
+  int rc = set_device(s);
+  if (rc) return rc;
+  const size_t b = 7 * (size_t)s->g.N * sizeof(float);
+  HIP_TRY(s, hipMemsetAsync(s->gq[0], 0, b, s->st));
+  HIP_TRY(s, hipMemsetAsync(s->gq[1], 0, b, s->st));
+  return AHT_OK;
+}
+
+int aht_reinit_cells(aht_solver *s, const uint32_t *old_ktop) {
+  int rc = check_ready(s);
+  if (rc) return rc;
+  if (!old_ktop) return fail(s, AHT_EINVAL, "null old_ktop");
+  if ((rc = set_device(s))) return rc;
+  HIP_TRY(s, hipMemcpyAsync(s->oldk, old_ktop, s->g.NXY * sizeof(uint32_t), hipMemcpyHostToDevice, s->st));
+  HIP_TRY(s, aht::launch_reinit(s->g, s->oldk, s->f[s->parity], s->gq[s->parity], s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  return AHT_OK;
+}
+
+int aht_step(aht_solver *s, int32_t n, uint32_t flags) {
+  int rc = check_ready(s);
+  if (rc) return rc;
+  if (n < 0) return fail(s, AHT_EINVAL, "negative step count");
+  if ((rc = set_device(s))) return rc;
+  const bool acc = flags & AHT_STEP_ACC, out = flags & AHT_STEP_OUT;
+  if (out && !s->out4) HIP_TRY(s, hipMalloc(&s->out4, 4 * (size_t)s->g.N * sizeof(float)));
+  aht::Acc a{s->acc_surf, s->acc_face, s->acc_content};
+  for (int32_t it = 0; it < n; it++) {
+    const int p = s->parity;
+    float *o4 = (out && it == n - 1) ? s->out4 : nullptr;
+    if (o4) HIP_TRY(s, hipMemsetAsync(o4, 0, 4 * (size_t)s->g.N * sizeof(float), s->st));
+    HIP_TRY(s, aht::launch_step(s->g, s->f[p], s->f[1 - p], s->gq[p], s->gq[1 - p], acc ? &a : nullptr, o4, s->block,
+                                s->st));
+    s->parity ^= 1;
+    s->steps++;
+    if (acc) s->n_acc++;
+    if (o4) s->has_out = true;
+  }
+  return AHT_OK;
+}
+
+int aht_sync(aht_solver *s) {
+  if (!s) return AHT_EINVAL;
+  int rc = set_device(s);
+  if (rc) return rc;
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  return AHT_OK;
+}
+
+int64_t aht_steps_done(const aht_solver *s) { return s ? s->steps : -1; }
+
+int aht_benchmark(aht_solver *s, int32_t n, double *ms) {
+  int rc = check_ready(s);
+  if (rc) return rc;
+  if (!ms || n < 1) return fail(s, AHT_EINVAL, "benchmark needs n >= 1 and an output");
+  if ((rc = set_device(s))) return rc;
+  HIP_TRY(s, hipEventRecord(s->ev0, s->st));
+  if ((rc = aht_step(s, n, 0))) return rc;
+  HIP_TRY(s, hipEventRecord(s->ev1, s->st));
+  HIP_TRY(s, hipEventSynchronize(s->ev1));
+  float t = 0.0f;
+  HIP_TRY(s, hipEventElapsedTime(&t, s->ev0, s->ev1));
+  *ms = t;
+  return AHT_OK;
+}
+
+int aht_read_surface(aht_solver *s, float conv, float *theta, float *u, float *v) {
+  int rc = check_ready(s);
+  if (rc) return rc;
+  if (!theta) return fail(s, AHT_EINVAL, "null theta");
+  if ((rc = set_device(s))) return rc;
+  const int p = s->parity;
+  const size_t NXY = s->g.NXY;
+  HIP_TRY(s, aht::launch_surface(s->g, s->f[p], s->gq[p], conv, s->surf3, s->st));
+  HIP_TRY(s, hipMemcpyAsync(theta, s->surf3, NXY * sizeof(float), hipMemcpyDeviceToHost, s->st));
+  if (u) HIP_TRY(s, hipMemcpyAsync(u, s->surf3 + NXY, NXY * sizeof(float), hipMemcpyDeviceToHost, s->st));
+  if (v) HIP_TRY(s, hipMemcpyAsync(v, s->surf3 + 2 * NXY, NXY * sizeof(float), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  return AHT_OK;
+}
+
+int aht_read_volume(aht_solver *s, int32_t nzv, float volmax, uint8_t *vol, float *theta_max) {
+  int rc = check_ready(s);
+  if (rc) return rc;
+  if (nzv < 1 || nzv > s->g.nz) return fail(s, AHT_EINVAL, "nzv must be within 1..nz");
+  if ((rc = set_device(s))) return rc;
+  const float vm = volmax > 0.0f ? volmax : 3.0f;
+  HIP_TRY(s, hipMemsetAsync(s->tmax, 0, sizeof(uint32_t), s->st));
+  HIP_TRY(s, aht::launch_volume(s->g, s->gq[s->parity], nzv, 255.0f / vm, s->vol, s->tmax, s->st));
+  if (vol) HIP_TRY(s, hipMemcpyAsync(vol, s->vol, (size_t)nzv * s->g.NXY, hipMemcpyDeviceToHost, s->st));
+  uint32_t tb = 0;
+  HIP_TRY(s, hipMemcpyAsync(&tb, s->tmax, sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  if (theta_max) memcpy(theta_max, &tb, 4);
+  return AHT_OK;
+}
+
+int aht_stats(aht_solver *s, aht_diag *out) {
+  int rc = check_ready(s);
+  if (rc) return rc;
+  if (!out) return fail(s, AHT_EINVAL, "null stats");
+  if ((rc = set_device(s))) return rc;
+  const int p = s->parity;
+  HIP_TRY(s, aht::launch_stats(s->g, s->f[p], s->gq[p], s->stat_part, s->red_out, s->st));
+  double r[8];
+  HIP_TRY(s, hipMemcpyAsync(r, s->red_out, 7 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  out->sum_theta = r[0]; out->sum_ke = r[1]; out->sum_drho = r[2]; out->n_fluid = r[3];
+  out->theta_max = r[4]; out->umax = r[5]; out->non_finite = (int64_t)r[6];
+  return AHT_OK;
+}
+
+int aht_read_out(aht_solver *s, float *out4) {
+  if (!s || !out4) return AHT_EINVAL;
+  if (!s->has_out) return fail(s, AHT_ESTATE, "no AHT_STEP_OUT step has run");
+  int rc = set_device(s);
+  if (rc) return rc;
+  HIP_TRY(s, hipMemcpyAsync(out4, s->out4, 4 * (size_t)s->g.N * sizeof(float), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  return AHT_OK;
+}
+
+int aht_mean_reset(aht_solver *s) {
+  if (!s) return AHT_EINVAL;
+  int rc = set_device(s);
+  if (rc) return rc;
+  const aht::Grid &g = s->g;
+  HIP_TRY(s, hipMemsetAsync(s->acc_surf, 0, 3 * (size_t)g.NXY * sizeof(double), s->st));
+  HIP_TRY(s, hipMemsetAsync(s->acc_face, 0, face_total(g.nx, g.ny, g.nz) * sizeof(double), s->st));
+  HIP_TRY(s, hipMemsetAsync(s->acc_content, 0, s->content_cap * sizeof(double), s->st));
+  s->n_acc = 0;
+  return AHT_OK;
+}
+
+int aht_read_face_mean(aht_solver *s, int32_t face, double *out) {
+  if (!s || !out || face < 0 || face > 3) return s ? fail(s, AHT_EINVAL, "face must be 0..3") : AHT_EINVAL;
+  int rc = set_device(s);
+  if (rc) return rc;
+  const aht::Grid &g = s->g;
+  const size_t len = aht::face_len(face, g.nx, g.ny, g.nz), off = aht::face_offset(face, g.nx, g.ny, g.nz);
+  HIP_TRY(s, hipMemcpyAsync(out, s->acc_face + off, len * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  const double inv = s->n_acc > 0 ? 1.0 / (double)s->n_acc : 0.0;
+  for (size_t k = 0; k < len; k++) out[k] *= inv;
+  return AHT_OK;
+}
+
+int aht_read_mean(aht_solver *s, double *surf3, aht_budget *b) {
+  if (!s) return AHT_EINVAL;
+  int rc = set_device(s);
+  if (rc) return rc;
+  const aht::Grid &g = s->g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t NXY = g.NXY;
+  const double inv = s->n_acc > 0 ? 1.0 / (double)s->n_acc : 0.0;
+  if (surf3) {
+    std::vector<double> h(3 * NXY);
+    HIP_TRY(s, hipMemcpyAsync(h.data(), s->acc_surf, 3 * NXY * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(s, hipStreamSynchronize(s->st));
+    for (int comp = 0; comp < 3; comp++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++)
+          surf3[comp * NXY + (size_t)(ny - 1 - j) * nx + i] = h[comp * NXY + (size_t)j * nx + i] * inv;
+  }
+  if (b) {
+    memset(b, 0, sizeof *b);
+    b->n_acc = s->n_acc;
+    double hin = 0.0;
+    for (float x : s->h_q) hin += (double)x;
+    b->heat_in = hin;
+    HIP_TRY(s, aht::launch_sum(s->acc_content, s->content_cap, s->red_out, s->st));
+    double content = 0.0;
+    HIP_TRY(s, hipMemcpyAsync(&content, s->red_out, sizeof(double), hipMemcpyDeviceToHost, s->st));
+    std::vector<double> fc(face_total(nx, ny, nz));
+    HIP_TRY(s, hipMemcpyAsync(fc.data(), s->acc_face, fc.size() * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(s, hipStreamSynchronize(s->st));
+    b->content = content * inv;
+    // heat_diag on the mean face fields: out = sum_E th*u - sum_W th*u + sum_N th*v - sum_S th*v
+    double hf[4] = {0, 0, 0, 0}, qf[4] = {0, 0, 0, 0};
+    for (int f = 0; f < 4; f++) {
+      const size_t len = aht::face_len(f, nx, ny, nz) / 2, off = aht::face_offset(f, nx, ny, nz);
+      for (size_t k = 0; k < len; k++) {
+        const double th = fc[off + k] * inv, un = fc[off + len + k] * inv;
+        hf[f] += th * un;
+        qf[f] += un;
+      }
+    }
+    b->heat_out = hf[1] - hf[0] + hf[3] - hf[2];
+    b->q_w = qf[0]; b->q_e = qf[1]; b->q_s = qf[2]; b->q_n = qf[3];
+    const double thru = std::fabs(qf[0]) + std::fabs(qf[1]) + std::fabs(qf[2]) + std::fabs(qf[3]);
+    b->mass_imbalance = std::fabs(qf[0] - qf[1] + qf[2] - qf[3]) / std::max(thru, 1e-300);
+  }
+  return AHT_OK;
+}
+
+int aht_read_state(aht_solver *s, float *f, float *g) {
+  if (!s) return AHT_EINVAL;
+  int rc = set_device(s);
+  if (rc) return rc;
+  const size_t N = s->g.N;
+  const int p = s->parity;
+  if (f) HIP_TRY(s, hipMemcpyAsync(f, s->f[p], 19 * N * sizeof(float), hipMemcpyDeviceToHost, s->st));
+  if (g) HIP_TRY(s, hipMemcpyAsync(g, s->gq[p], 7 * N * sizeof(float), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  return AHT_OK;
+}
+
+int aht_write_state(aht_solver *s, const float *f, const float *g) {
+  if (!s) return AHT_EINVAL;
+  int rc = set_device(s);
+  if (rc) return rc;
+  const size_t N = s->g.N;
+  const int p = s->parity;
+  if (f) HIP_TRY(s, hipMemcpyAsync(s->f[p], f, 19 * N * sizeof(float), hipMemcpyHostToDevice, s->st));
+  if (g) HIP_TRY(s, hipMemcpyAsync(s->gq[p], g, 7 * N * sizeof(float), hipMemcpyHostToDevice, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  return AHT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+/*
+ * aht.h — C-ABI of the MI355X-native coupled lattice-Boltzmann step (D3Q19 regularized-BGK flow + D3Q7
+ * potential-temperature transport) of the Ashburn heat twin. Implemented in HIP for gfx950 (libaht.so).
+ *
+ * The library owns every device buffer. Callers pass plain host pointers and sizes; there is no torch, no CPU path
+ * and no fallback: every entry point fails with a negative AHT_E* code when the GPU or the kernels are unavailable.
+ *
+ * Layout (as the reference, js/live/kernel.js:17): populations [q][cell], cell = (k*ny + j)*nx + i, j = 0 at the south
+ * edge; per-column arrays [j*nx + i] in that solver orientation; surface outputs north-up [row*nx + col], row 0 = north.
+ *
+ * Threading: one host thread per solver at a time. Work is enqueued on the solver's own HIP stream; aht_step returns
+ * once enqueued, every read returns after the data is on the host.
+ */
+#ifndef AHT_H
+#define AHT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AHT_VERSION 1
+
+/* return codes */
+#define AHT_OK 0
+#define AHT_EINVAL (-1)   /* bad argument (null pointer, size, geometry rule)          */
+#define AHT_ENOMEM (-2)   /* device allocation failed                                  */
+#define AHT_EDEVICE (-3)  /* HIP runtime error or no GPU                               */
+#define AHT_ESTATE (-4)   /* call out of order (no columns / params yet, destroyed)    */
+
+/* population storage formats */
+#define AHT_STORE_F32 0   /* f32 populations (208 B per fluid-cell update)                            */
+#define AHT_STORE_F16 1   /* f16 deviations f - w*rho0, theta_q (104 B per update), f32 arithmetic    */
+#define AHT_STORE_BF16 2  /* bf16 deviations, same layout as F16                                      */
+
+/* aht_step flags */
+#define AHT_STEP_ACC 1u   /* accumulate the FP64 time means (surface theta/u/v, face fluxes, heat content)  */
+#define AHT_STEP_OUT 2u   /* the last step of the call writes (theta, ux, uy, uz) of every fluid cell        */
+
+typedef struct aht_solver aht_solver;
+
+typedef struct {
+  int32_t nx, ny, nz;
+  int32_t storage;       /* AHT_STORE_*                                          */
+  int32_t device;        /* HIP device ordinal                                   */
+  int32_t block;         /* threads per block of the step kernel (0 = default)   */
+  int32_t reserved[2];
+} aht_dims;
+
+/* the scalar block of the reference's uniform Params (js/live/kernel.js:44-50; values js/live/solver.js:225-238) */
+typedef struct {
+  float tau0, cs2smag, gbeta, prt, kfloor, momfac, srcscale, sigmax;
+  int32_t inflow_w, inflow_e, inflow_s, inflow_n, nsp, nsw, oclip, pad;
+} aht_params;
+
+/* diagnostics of the stored state over fluid cells (js/live/kernel.js:580-633 + js/live/solver.js:559-576) */
+typedef struct {
+  double sum_theta;   /* sum of theta (K * cells)                      */
+  double sum_ke;      /* sum of 0.5 |j|^2 / rho (lattice units)         */
+  double sum_drho;    /* sum of rho - 1                                 */
+  double n_fluid;     /* finite fluid cells                             */
+  double theta_max;   /* max(theta, 0)                                  */
+  double umax;        /* max |j| / rho (lattice units)                  */
+  int64_t non_finite; /* cells with non-finite rho or theta             */
+} aht_diag;
+
+/* time-mean budgets, lattice units (the host scales by rho*cp, dx, dt, conv); semantics of heat_diag,
+ * tools/live/validate_live.py:226-240, evaluated from the FP64 time means of the face cells */
+typedef struct {
+  int64_t n_acc;           /* accumulated steps                                                   */
+  double heat_in;          /* sum over columns of q (K per step per cell, i.e. cells*K/step)       */
+  double heat_out;         /* sum_E th*u - sum_W th*u + sum_N th*v - sum_S th*v of the mean fields */
+  double content;          /* sum over fluid cells of the mean theta (K * cells)                  */
+  double q_w, q_e, q_s, q_n; /* mean volume flux through each face (sum of mean u or v)           */
+  double mass_imbalance;   /* |q_w - q_e + q_s - q_n| / (|q_w| + |q_e| + |q_s| + |q_n|)            */
+} aht_budget;
+
+/* lifetime (replaces createLiveSolver's device and buffer set-up, js/live/solver.js:44-101, :137-158) */
+int aht_create(const aht_dims *dims, aht_solver **out);
+void aht_destroy(aht_solver *s);                                 /* destroy(), js/live/solver.js:485-493 */
+const char *aht_last_error(const aht_solver *s);                 /* message of the last failure          */
+const char *aht_error_string(int code);
+
+/* geometry and sources per column: kt >= 1, kt <= ktop <= nz (replaces writeBuffer of ktop and the packed colv,
+ * js/live/solver.js:338, :359-376; numba ABI kt/ktop/cd/q, tools/live/validate_live.py:110-112) */
+int aht_set_columns(aht_solver *s, const uint32_t *kt, const uint32_t *ktop, const float *cd, const float *q);
+/* parameter block and the per-level profile prof[nz][4] = (u_in, v_in, kb, 0), lattice units
+ * (replaces _writeParams, js/live/solver.js:225-242) */
+int aht_set_params(aht_solver *s, const aht_params *p, const float *prof);
+
+/* state (replaces resetFlow/_reinitFlow's init kernel, js/live/solver.js:288-293, :389-398; genInitWGSL) */
+int aht_init_flow(aht_solver *s);                 /* f := equilibrium of the inflow profile, both lattices   */
+int aht_reset_heat(aht_solver *s);                /* g := 0, both lattices (resetHeat, solver.js:380-386)     */
+int aht_reinit_cells(aht_solver *s, const uint32_t *old_ktop); /* solid->fluid cells at rest (genReinitWGSL) */
+
+/* stepping (replaces step(n) / _encodeSteps, js/live/solver.js:401-427; numba lbm_step loop,
+ * tools/live/validate_live.py:105-117) */
+int aht_step(aht_solver *s, int32_t n, uint32_t flags);
+int aht_sync(aht_solver *s);
+int64_t aht_steps_done(const aht_solver *s);
+/* n steps between two HIP events recorded on the solver's stream (replaces benchmark(), solver.js:474-483);
+ * *ms = event-measured device time of the n steps */
+int aht_benchmark(aht_solver *s, int32_t n, double *ms);
+
+/* read-back (replaces read()'s surf kernel, js/live/solver.js:439-471, genSurfWGSL kernel.js:435-471):
+ * pre-collision theta and force-corrected u*conv, v*conv at k = ktop, north-up; u or v may be NULL */
+int aht_read_surface(aht_solver *s, float conv, float *theta, float *u, float *v);
+/* the u8 theta volume of the lowest nzv levels, [k][row][col] north-up, round(theta*255/volmax) clamped,
+ * plus the max theta over those fluid cells (genVolWGSL, kernel.js:475-520) */
+int aht_read_volume(aht_solver *s, int32_t nzv, float volmax, uint8_t *vol, float *theta_max);
+/* diagnostics (replaces _dbg.stats, js/live/solver.js:559-576) */
+int aht_stats(aht_solver *s, aht_diag *out);
+/* the (theta, ux, uy, uz) [4][N] written by the last AHT_STEP_OUT step (stepOut/readDbg, solver.js:548-556) */
+int aht_read_out(aht_solver *s, float *out4);
+
+/* FP64 time means accumulated by AHT_STEP_ACC steps (replaces stepAcc + the offline means of
+ * tools/live/validate_live.py:258-290): surf3 = [3][nx*ny] mean (theta, ux, uy), north-up, lattice units
+ * (may be NULL); budget may be NULL */
+int aht_mean_reset(aht_solver *s);
+int aht_read_mean(aht_solver *s, double *surf3, aht_budget *budget);
+/* face means [2][nz][ny or nx] (theta, normal velocity) of face 0..3 = W, E, S, N, lattice units */
+int aht_read_face_mean(aht_solver *s, int32_t face, double *out);
+
+/* raw state at the current parity, f32 [19][N] and [7][N] (replaces _dbg.readState/writeState,
+ * js/live/solver.js:540-547); 16-bit storages convert on the device; f or g may be NULL */
+int aht_read_state(aht_solver *s, float *f, float *g);
+int aht_write_state(aht_solver *s, const float *f, const float *g);
+
+/* device bytes held by the solver (admission by footprint in the serving loop) */
+int64_t aht_footprint_bytes(const aht_dims *dims);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,71 @@
+"""The C-ABI library loads and exports every symbol include/aht.h declares (no compute calls: no GPU here), and the
+ctypes structures match the header's layout."""
+import ctypes
+import os
+import re
+import subprocess
+
+from conftest import PKG, ROOT, load_pkg
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, 'include', 'aht.h')).read()
+    txt = re.sub(r'/\*.*?\*/', '', txt, flags=re.S)
+    return sorted(set(re.findall(r'\b(aht_[a-z0-9_]+)\s*\(', txt)))
+
+
+def test_every_declared_symbol_is_exported():
+    m = load_pkg()
+    L = m.lib()
+    names = header_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the binding covers them all
+    assert set(names) == set(m.SIGNATURES), set(names) ^ set(m.SIGNATURES)
+
+
+def test_no_oracle_symbols_in_product_library():
+    m = load_pkg()
+    out = subprocess.run(['nm', '-D', '--defined-only', m.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    assert 'oracle_' not in out
+
+
+def test_struct_layouts():
+    m = load_pkg()
+    assert ctypes.sizeof(m.Dims) == 32
+    assert ctypes.sizeof(m.Params) == 64
+    assert ctypes.sizeof(m.Diag) == 56
+    assert ctypes.sizeof(m.Budget) == 72
+
+
+def test_footprint_is_positive_and_scales():
+    m = load_pkg()
+    L = m.lib()
+    small = L.aht_footprint_bytes(ctypes.byref(m.Dims(100, 100, 32, 0, 0, 0)))
+    big = L.aht_footprint_bytes(ctypes.byref(m.Dims(600, 600, 64, 0, 0, 0)))
+    assert small > 208 * 100 * 100 * 32 and big > 208 * 600 * 600 * 64
+    assert big > 50 * small
+
+
+def test_create_without_gpu_fails_cleanly():
+    """with no device the library reports an error instead of running anything on the CPU"""
+    m = load_pkg()
+    try:
+        import torch  # noqa: F401  (only to know whether a GPU is present)
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        return
+    try:
+        m.Solver(8, 8, 8)
+    except m.AhtError as e:
+        assert e.code in (m.AHT_EDEVICE, m.AHT_ENOMEM)
+    else:
+        raise AssertionError('solver created without a GPU')
+
+
+def test_library_links_no_torch():
+    out = subprocess.run(['ldd', os.path.join(PKG, 'lib', 'libaht.so')], capture_output=True, text=True).stdout
+    assert 'torch' not in out and 'amdhip64' in out

@@ -1,0 +1,133 @@
+"""HIP step vs the CPU oracle on the same seeded inputs, through the C-ABI (libaht.so).
+
+Tiny synthetic domains exercise every boundary branch of js/live/kernel.js: inflow and outflow faces on each side,
+backflow-clipped and prototype outlets, the free-slip lid, bounce-back on terrain and buildings, the log-law drag,
+both sponges and the rooftop sources. Bar: the f32 kernel and the f32 oracle evaluate the same expressions in the
+same order with FMA contraction off, so the states must agree to the last bit or within 1e-6 relative L2.
+"""
+import numpy as np
+import pytest
+
+from conftest import dump_domain, load_summary, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [  # (wind from, outlet clip, nsw): inflow faces w+s, e+n, e only, n only, w only
+    (210, 0, 5), (210, 1, 5), (30, 1, 3), (90, 1, 5), (0, 0, 16), (270, 1, 5),
+]
+
+
+def tiny(tmp_path, wdir, oclip, nsw, shape='24,20,12,7'):
+    return dump_domain(tmp_path, '--tiny', shape, '--dx', 30, '--dir', wdir, '--u10', 6, '--oclip', oclip, '--nsw', nsw,
+                       '--flux', 4000)
+
+
+def pair(pkg, O, dom):
+    o = O.Oracle(dom, 'f32')
+    o.init_flow()
+    s = pkg.Solver.from_dump(dom)
+    return o, s
+
+
+def assert_state_close(s, o, tol=1e-6):
+    f, g = s.read_state()
+    fl = o.fluid_mask()
+    assert rel_l2(f[:, fl], o.f[:, fl]) <= tol
+    assert rel_l2(g[:, fl], o.g[:, fl]) <= tol or np.abs(o.g[:, fl]).max() == 0
+    return float((f[:, fl].view(np.uint32) == o.f[:, fl].view(np.uint32)).mean())
+
+
+@pytest.mark.parametrize('wdir,oclip,nsw', VARIANTS)
+def test_tiny_domain_steps_match_oracle(tmp_path, pkg, O, wdir, oclip, nsw):
+    dom = tiny(tmp_path, wdir, oclip, nsw)
+    o, s = pair(pkg, O, dom)
+    with s:
+        f0, g0 = s.read_state()
+        assert np.array_equal(f0.view(np.uint32), o.f.view(np.uint32)), 'init state differs'
+        ro = o.step(1, out=True)
+        s.step(1, out=True)
+        go = s.read_out()
+        fl = o.fluid_mask()
+        for c in range(4):
+            assert np.abs(go[c, fl] - ro[c, fl]).max() <= 1e-6 * max(1e-30, np.abs(ro[c, fl]).max())
+        bits1 = assert_state_close(s, o)
+        o.step(49)
+        s.step(49)
+        bits50 = assert_state_close(s, o)
+        print(f'dir {wdir} oclip {oclip}: bit-identical f after 1 step {bits1:.6f}, after 50 {bits50:.6f}')
+        th_o, u_o, v_o = o.surface(dom['conv'])
+        th_g, u_g, v_g = s.read_surface(dom['conv'])
+        assert rel_l2(th_g, th_o) <= 1e-6 and rel_l2(u_g, u_o) <= 1e-6 and rel_l2(v_g, v_o) <= 1e-6
+
+
+def test_single_step_theta_from_developed_state(tmp_path, pkg, O):
+    """north_star bar: one step from an identical developed state, relative L2 of theta <= 1e-5"""
+    dom = tiny(tmp_path, 210, 1, 5)
+    o, s = pair(pkg, O, dom)
+    with s:
+        o.step(200)
+        s.write_state(o.f, o.g)
+        ro = o.step(1, out=True)
+        s.step(1, out=True)
+        go = s.read_out()
+        fl = o.fluid_mask()
+        assert rel_l2(go[0, fl], ro[0, fl]) <= 1e-5
+        assert rel_l2(go[1, fl], ro[1, fl]) <= 1e-5 and rel_l2(go[2, fl], ro[2, fl]) <= 1e-5
+
+
+def test_stats_volume_and_reinit_match_oracle(tmp_path, pkg, O):
+    dom = tiny(tmp_path, 210, 1, 5)
+    o, s = pair(pkg, O, dom)
+    with s:
+        o.step(30)
+        s.step(30)
+        so, sg = o.stats(), s.stats()
+        assert sg['non_finite'] == 0 and sg['n_fluid'] == so['n_fluid']
+        for k in ('sum_theta', 'sum_ke', 'sum_drho'):
+            assert abs(sg[k] - so[k]) <= 1e-6 * max(1.0, abs(so[k])), k
+        for k in ('theta_max', 'umax'):
+            assert abs(sg[k] - so[k]) <= 1e-6 * abs(so[k]), k
+        # u8 volume: theta of the next step's pre-collision moments, quantised round-half-even (genVolWGSL)
+        nzv = 8
+        vol, tmax = s.read_volume(nzv, 3.0)
+        nx, ny, nz = o.nx, o.ny, o.nz
+        save_f, save_g = o.f.copy(), o.g.copy()
+        th = o.step(1, out=True)[0].reshape(nz, ny, nx)
+        o.f, o.g = save_f, save_g
+        fl = o.fluid_mask().reshape(nz, ny, nx)
+        thv = np.where(fl, th, np.float32(0))[:nzv]
+        qv = np.clip(np.rint(thv * np.float32(255.0 / 3.0)), 0, 255).astype(np.uint8)[:, ::-1, :]
+        assert np.array_equal(vol.reshape(nzv, ny, nx), qv)
+        assert abs(tmax - max(float(thv.max()), 0.0)) <= 1e-7 * max(1.0, tmax)
+        # geometry change: lower ktop of some columns, the uncovered cells restart at rest
+        s.write_state(o.f, o.g)
+        old = dom['ktop'].copy()
+        new = np.maximum(dom['kt'], old - 1).astype(np.uint32)
+        s.set_columns(dom['kt'], new, dom['cd'], dom['q'])
+        s.reinit_cells(old)
+        o.set_columns(dom['kt'], new, dom['cd'], dom['q'])
+        o.reinit(old)
+        f, g = s.read_state()
+        assert np.array_equal(f, o.f) and np.array_equal(g, o.g)
+
+
+def test_p30a_reference_run_on_gpu(pkg, O, ashburn30):
+    """the reference's p30a case: 300 steps from rest on the Ashburn 30 m grid; read() surface statistics against
+    the recorded WebGPU and numba values, and the state against the f32 oracle"""
+    K = load_summary()['short']['p30a']
+    dom = ashburn30
+    with pkg.Solver.from_dump(dom) as s:
+        s.step(K['steps'])
+        th, u, v = s.read_surface(dom['conv'])
+        wind = np.hypot(u.astype(np.float64), v.astype(np.float64)).mean()
+        assert abs(th.max() - K['webgpu']['thetaMax']) <= 3e-6 * K['webgpu']['thetaMax']
+        assert abs(th.max() - K['numba']['theta_max']) <= 1e-5 * K['numba']['theta_max']
+        assert abs(th.astype(np.float64).mean() - K['webgpu']['surfMean']) <= 1e-5 * K['webgpu']['surfMean']
+        assert abs(wind - K['webgpu']['windMean']) <= 1e-5 * K['webgpu']['windMean']
+        o = O.Oracle(dom, 'f32')
+        o.init_flow()
+        o.step(K['steps'])
+        f, g = s.read_state()
+        fl = o.fluid_mask()
+        assert rel_l2(g[:, fl], o.g[:, fl]) <= 1e-6
+        assert rel_l2(f[:, fl], o.f[:, fl]) <= 1e-6
