@@ -107,9 +107,9 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   if (!dims || !out) return AHT_EINVAL;
   *out = nullptr;
   const aht_dims d = *dims;
-  if (d.nx < 2 || d.ny < 2 || d.nz < 2) return AHT_EINVAL;
+  if (d.nx < 2 || d.ny < 2 || d.nz < 2 || d.nz > 1022) return AHT_EINVAL;
   const uint64_t N = (uint64_t)d.nx * d.ny * d.nz;
-  if (N >= (1ull << 31)) return AHT_EINVAL;
+  if (N >= (1ull << 30)) return AHT_EINVAL;   // cell byte offsets stay 32-bit
   if (d.storage != AHT_STORE_F32) return AHT_EINVAL;
   const int block = d.block ? d.block : aht::STEP_BLOCK;
   if (block < 64 || block > 256 || block % 64) return AHT_EINVAL;
@@ -179,17 +179,28 @@ int aht_set_columns(aht_solver *s, const uint32_t *kt, const uint32_t *ktop, con
   if (!s || !kt || !ktop || !cd || !q) return s ? fail(s, AHT_EINVAL, "null column array") : AHT_EINVAL;
   const uint32_t NXY = s->g.NXY;
   const int nz = s->g.nz;
+  const int nx = s->g.nx, ny = s->g.ny;
   std::vector<uint4> cv(NXY);
   int kmin = nz;
   for (uint32_t c = 0; c < NXY; c++) {
     if (kt[c] < 1 || kt[c] > ktop[c] || (int)ktop[c] > nz)
       return fail(s, AHT_EINVAL, "column %u: need 1 <= kt (%u) <= ktop (%u) <= nz (%d)", c, kt[c], ktop[c], nz);
     if (!std::isfinite(cd[c]) || !std::isfinite(q[c])) return fail(s, AHT_EINVAL, "column %u: non-finite cd or q", c);
-    uint32_t cdb, qb;
-    memcpy(&cdb, &cd[c], 4);
-    memcpy(&qb, &q[c], 4);
-    cv[c] = make_uint4(kt[c], ktop[c], cdb, qb);
     kmin = std::min(kmin, (int)ktop[c]);
+  }
+  for (int j = 0; j < ny; j++) {
+    for (int i = 0; i < nx; i++) {
+      const uint32_t c = (uint32_t)j * nx + i;
+      // kfast: first level whose 3 x 3 x 3 source neighbourhood is all fluid (clamped at the faces)
+      uint32_t m = 0;
+      for (int b = std::max(j - 1, 0); b <= std::min(j + 1, ny - 1); b++)
+        for (int a = std::max(i - 1, 0); a <= std::min(i + 1, nx - 1); a++) m = std::max(m, ktop[(size_t)b * nx + a]);
+      const uint32_t kfast = std::min<uint32_t>(m + 1, 1023u);
+      uint32_t cdb, qb;
+      memcpy(&cdb, &cd[c], 4);
+      memcpy(&qb, &q[c], 4);
+      cv[c] = make_uint4(kt[c] | (ktop[c] << 10) | (kfast << 20), 0u, cdb, qb);
+    }
   }
   int rc = set_device(s);
   if (rc) return rc;

@@ -1,8 +1,15 @@
 // Device-side lattice tables and per-cell physics of the coupled D3Q19 + D3Q7 step (gfx950).
 //
-// The arithmetic follows the reference kernel generator js/live/kernel.js section by section (stream :159-243,
-// moments :264-277, forces :279-306, relaxation :308-338, collision :340-386) in the same operation order, so that
-// with FMA contraction off the f32 results equal the CPU oracle's. Layout: populations [q][cell],
+// The physics is the reference kernel generator's (js/live/kernel.js): moments :264-277, buoyancy / log-law drag /
+// top sponge :279-306, Smagorinsky + K-profile relaxation with the outflow viscous sponge :308-338, regularised BGK
+// with Guo forcing :340-367, D3Q7 advection-diffusion with rooftop source and sponge damping :369-386.
+// The arithmetic is arranged for the CDNA4 VALU instead of the generator's term-by-term order:
+//   * opposite populations (q, opp(q)) are handled as pairs: the moments use their sums and differences, and the
+//     collision evaluates the part even in c once and the odd part once, then writes both populations;
+//   * 1/rho, 1/tau and 1/tau_g are each computed once (v_rcp_f32) and multiplied, square roots use v_sqrt_f32;
+//   * the file is compiled with FMA contraction.
+// The result equals the reference expressions to f32 rounding (checked against the CPU oracle, which keeps the
+// generator's order, to relative L2 <= 1e-6 per step). Layout: populations [q][cell],
 // cell = (k*ny + j)*nx + i (js/live/kernel.js:17).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -10,189 +17,195 @@
 
 namespace aht {
 
-// D3Q19 / D3Q7 velocity sets, weights, opposite and z-mirror maps (js/live/kernel.js:19-28)
+// D3Q19 / D3Q7 velocity sets, opposite and z-mirror maps (js/live/kernel.js:19-28). Pairs (q, q+1) for odd q are
+// opposite directions: 1/2 x, 3/4 y, 5/6 z, 7/8 (1,1,0), 9/10 (1,-1,0), 11/12 (1,0,1), 13/14 (1,0,-1), 15/16 (0,1,1),
+// 17/18 (0,1,-1).
 __device__ constexpr int CX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
 __device__ constexpr int CY[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
 __device__ constexpr int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
 __device__ constexpr int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
 __device__ constexpr int MIRZ[19] = {0, 1, 2, 3, 4, 6, 5, 7, 8, 9, 10, 13, 14, 11, 12, 17, 18, 15, 16};
 __device__ constexpr int OPP7[7] = {0, 2, 1, 4, 3, 6, 5};
+__device__ constexpr int CX7[7] = {0, 1, -1, 0, 0, 0, 0};
+__device__ constexpr int CY7[7] = {0, 0, 0, 1, -1, 0, 0};
+__device__ constexpr int CZ7[7] = {0, 0, 0, 0, 0, 1, -1};
 
+// weights as the generator prints them (fmt(): f32-rounded, js/live/kernel.js:31-33)
 template <int q> __device__ __forceinline__ constexpr float w19() {
   return q == 0 ? (float)(1.0 / 3.0) : (q < 7 ? (float)(1.0 / 18.0) : (float)(1.0 / 36.0));
 }
 template <int q> __device__ __forceinline__ constexpr float w7() { return q == 0 ? 0.25f : 0.125f; }
-// fmt(4.5 * w_q): the regularisation prefactor, exact in f32 (js/live/kernel.js:361)
-template <int q> __device__ __forceinline__ constexpr float w45() { return q == 0 ? 1.5f : (q < 7 ? 0.25f : 0.125f); }
 
-// kernel parameters: the reference's uniform Params (js/live/kernel.js:44-50) plus grid and buffers
+// the reference's uniform Params block (js/live/kernel.js:44-50)
 struct Params {
   float tau0, cs2smag, gbeta, prt, kfloor, momfac, srcscale, sigmax;
   int inflow_w, inflow_e, inflow_s, inflow_n, nsp, nsw, oclip, pad;
 };
 
-// equilibrium polynomial 1 + 3cu + 4.5cu^2 - 1.5u^2 in the generator's order ((1 + 3cu) + (4.5cu)cu) - 1.5usq
-__device__ __forceinline__ float epoly(float cu, float usq) {
-  return ((1.0f + 3.0f * cu) + 4.5f * cu * cu) - 1.5f * usq;
-}
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// c.u with only the nonzero components, x then y then z (the generator's term list)
+// equilibrium polynomial 1 + 3cu + 4.5cu^2 - 1.5u^2 (boundary reconstruction)
+__device__ __forceinline__ float epoly(float cu, float usq) { return 1.0f + cu * (3.0f + 4.5f * cu) - 1.5f * usq; }
+
+// c.u over the nonzero components
 template <int cx, int cy, int cz>
 __device__ __forceinline__ float cdot(float x, float y, float z) {
   float s = 0.0f;
-  if (cx) s = s + (float)cx * x;
-  if (cy) s = s + (float)cy * y;
-  if (cz) s = s + (float)cz * z;
+  if (cx) s = (cx > 0) ? s + x : s - x;
+  if (cy) s = (cy > 0) ? s + y : s - y;
+  if (cz) s = (cz > 0) ? s + z : s - z;
   return s;
 }
 
-// post-stream moments: rho, j, theta and the second moments (js/live/kernel.js:264-277)
+// post-stream moments (js/live/kernel.js:264-277) from pair sums s_q = f_q + f_opp and differences d_q = f_q - f_opp
 struct Moments {
   float rho, jx, jy, jz, th, Pxx, Pyy, Pzz, Pxy, Pxz, Pyz;
 };
 
-template <bool SECOND>
 __device__ __forceinline__ Moments moments(const float (&f)[19], const float (&g)[7]) {
+  float s[19], d[19];
+#pragma unroll
+  for (int q = 1; q < 19; q += 2) { s[q] = f[q] + f[q + 1]; d[q] = f[q] - f[q + 1]; }
   Moments m;
-  float r = f[0];
-#pragma unroll
-  for (int q = 1; q < 19; q++) r = r + f[q];
-  m.rho = r;
-  float sx = 0.f, sy = 0.f, sz = 0.f;
-#pragma unroll
-  for (int q = 0; q < 19; q++) {
-    if (CX[q] == 1) sx = sx + f[q]; else if (CX[q] == -1) sx = sx - f[q];
-    if (CY[q] == 1) sy = sy + f[q]; else if (CY[q] == -1) sy = sy - f[q];
-    if (CZ[q] == 1) sz = sz + f[q]; else if (CZ[q] == -1) sz = sz - f[q];
-  }
-  m.jx = sx; m.jy = sy; m.jz = sz;
-  float t = g[0];
-#pragma unroll
-  for (int q = 1; q < 7; q++) t = t + g[q];
-  m.th = t;
-  if (SECOND) {
-    float xx = 0.f, yy = 0.f, zz = 0.f, xy = 0.f, xz = 0.f, yz = 0.f;
-#pragma unroll
-    for (int q = 0; q < 19; q++) {
-      if (CX[q] != 0) xx = xx + f[q];
-      if (CY[q] != 0) yy = yy + f[q];
-      if (CZ[q] != 0) zz = zz + f[q];
-      const int vxy = CX[q] * CY[q], vxz = CX[q] * CZ[q], vyz = CY[q] * CZ[q];
-      if (vxy == 1) xy = xy + f[q]; else if (vxy == -1) xy = xy - f[q];
-      if (vxz == 1) xz = xz + f[q]; else if (vxz == -1) xz = xz - f[q];
-      if (vyz == 1) yz = yz + f[q]; else if (vyz == -1) yz = yz - f[q];
-    }
-    m.Pxx = xx; m.Pyy = yy; m.Pzz = zz; m.Pxy = xy; m.Pxz = xz; m.Pyz = yz;
-  }
+  m.rho = f[0] + ((s[1] + s[3]) + (s[5] + s[7])) + ((s[9] + s[11]) + (s[13] + s[15])) + s[17];
+  m.jx = (d[1] + d[7]) + (d[9] + d[11]) + d[13];
+  m.jy = (d[3] + d[7]) - d[9] + (d[15] + d[17]);
+  m.jz = (d[5] + d[11]) - d[13] + (d[15] - d[17]);
+  m.Pxx = (s[1] + s[7]) + (s[9] + s[11]) + s[13];
+  m.Pyy = (s[3] + s[7]) + (s[9] + s[15]) + s[17];
+  m.Pzz = (s[5] + s[11]) + (s[13] + s[15]) + s[17];
+  m.Pxy = s[7] - s[9];
+  m.Pxz = s[11] - s[13];
+  m.Pyz = s[15] - s[17];
+  m.th = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + g[6]);
   return m;
 }
 
 struct Forces {
-  float Fx, Fy, Fz, sg, ux, uy, uz;
+  float Fx, Fy, Fz, sg, ux, uy, uz, inv_rho;
 };
 
-// buoyancy, limited log-law drag, top sponge, force-corrected velocity (js/live/kernel.js:279-306)
+// buoyancy, limited log-law drag at the first cell of building-free columns, top sponge, force-corrected velocity
+// (js/live/kernel.js:279-306). DRAG / SPONGE = false drop branches the caller has proven inactive.
+template <bool DRAG, bool SPONGE>
 __device__ __forceinline__ Forces forces(const Params &P, int NZ, int k, int kg, int ktc, float cd, float uI, float vI,
                                          const Moments &m) {
   Forces o;
-  const float rho = m.rho;
+  const float rho = m.rho, ir = rcp(rho);
+  o.inv_rho = ir;
   float Fx = 0.0f, Fy = 0.0f;
   float Fz = rho * P.gbeta * m.th;
-  if (k == kg && kg == ktc) {
-    const float ux0 = m.jx / rho, uy0 = m.jy / rho;
-    const float sp = sqrtf(ux0 * ux0 + uy0 * uy0);
-    Fx = -rho * cd * sp * ux0;
-    Fy = -rho * cd * sp * uy0;
-    const float lim = 0.5f * rho;
-    if (cd * sp > 0.5f) {
-      Fx = -lim * ux0;
-      Fy = -lim * uy0;
-    }
+  if (DRAG && k == kg && kg == ktc) {
+    const float ux0 = m.jx * ir, uy0 = m.jy * ir;
+    const float sp = fsqrt(ux0 * ux0 + uy0 * uy0);
+    const bool lim = cd * sp > 0.5f;                     // implicit limit: never reverse the flow within a step
+    const float c = lim ? -0.5f * rho : -rho * cd * sp;
+    Fx = c * ux0;
+    Fy = c * uy0;
   }
   float sg = 0.0f;
-  if (k >= NZ - P.nsp) {
+  if (SPONGE && k >= NZ - P.nsp) {
     const float aa = ((float)(k - (NZ - P.nsp)) + 1.0f) / (float)P.nsp;
     sg = P.sigmax * aa * aa;
-    Fx += rho * sg * (uI - m.jx / rho);
-    Fy += rho * sg * (vI - m.jy / rho);
-    Fz += -rho * sg * (m.jz / rho);
+    const float rs = rho * sg;
+    Fx += rs * (uI - m.jx * ir);
+    Fy += rs * (vI - m.jy * ir);
+    Fz -= rs * (m.jz * ir);
   }
   o.Fx = Fx; o.Fy = Fy; o.Fz = Fz; o.sg = sg;
-  o.ux = (m.jx + 0.5f * Fx) / rho;
-  o.uy = (m.jy + 0.5f * Fy) / rho;
-  o.uz = (m.jz + 0.5f * Fz) / rho;
+  o.ux = (m.jx + 0.5f * Fx) * ir;
+  o.uy = (m.jy + 0.5f * Fy) * ir;
+  o.uz = (m.jz + 0.5f * Fz) * ir;
   return o;
 }
 
-// relaxation times (js/live/kernel.js:308-338)
+// relaxation (js/live/kernel.js:308-338)
 struct Relax {
-  float om, fpre, taug, trN, Nxx, Nyy, Nzz, Nxy, Nxz, Nyz, usq;
+  float om45, fp3, fp9, omg, Nxx, Nyy, Nzz, Nxy, Nxz, Nyz, C0;
 };
 
 __device__ __forceinline__ Relax relax(const Params &P, int NX, int NY, int i, int j, float kb, const Moments &m,
                                        const Forces &F) {
   Relax r;
   const float rho = m.rho, ux = F.ux, uy = F.uy, uz = F.uz;
-  r.usq = ux * ux + uy * uy + uz * uz;
   const float third = (float)(1.0 / 3.0);
   const float r3 = rho * third;
-  r.Nxx = m.Pxx - r3 - rho * ux * ux;
-  r.Nyy = m.Pyy - r3 - rho * uy * uy;
-  r.Nzz = m.Pzz - r3 - rho * uz * uz;
-  r.Nxy = m.Pxy - rho * ux * uy;
-  r.Nxz = m.Pxz - rho * ux * uz;
-  r.Nyz = m.Pyz - rho * uy * uz;
-  const float Pn = sqrtf(r.Nxx * r.Nxx + r.Nyy * r.Nyy + r.Nzz * r.Nzz +
+  const float rux = rho * ux, ruy = rho * uy, ruz = rho * uz;
+  r.Nxx = m.Pxx - r3 - rux * ux;
+  r.Nyy = m.Pyy - r3 - ruy * uy;
+  r.Nzz = m.Pzz - r3 - ruz * uz;
+  r.Nxy = m.Pxy - rux * uy;
+  r.Nxz = m.Pxz - rux * uz;
+  r.Nyz = m.Pyz - ruy * uz;
+  const float Pn = fsqrt(r.Nxx * r.Nxx + r.Nyy * r.Nyy + r.Nzz * r.Nzz +
                          2.0f * (r.Nxy * r.Nxy + r.Nxz * r.Nxz + r.Nyz * r.Nyz));
   float tb = P.tau0 + 3.0f * P.momfac * kb;
+  // viscous sponge next to outflow faces
   int dmin = 1000;
-  if (P.inflow_w == 0 && i < dmin) dmin = i;
-  if (P.inflow_e == 0 && NX - 1 - i < dmin) dmin = NX - 1 - i;
-  if (P.inflow_s == 0 && j < dmin) dmin = j;
-  if (P.inflow_n == 0 && NY - 1 - j < dmin) dmin = NY - 1 - j;
+  if (P.inflow_w == 0) dmin = min(dmin, i);
+  if (P.inflow_e == 0) dmin = min(dmin, NX - 1 - i);
+  if (P.inflow_s == 0) dmin = min(dmin, j);
+  if (P.inflow_n == 0) dmin = min(dmin, NY - 1 - j);
   if (dmin < P.nsw) {
     const float rr = ((float)P.nsw - (float)dmin) / (float)P.nsw;
     tb += 0.15f * rr * rr;
   }
-  const float tau = 0.5f * (tb + sqrtf(tb * tb + P.cs2smag * Pn / rho));
+  const float tau = 0.5f * (tb + fsqrt(tb * tb + P.cs2smag * Pn * F.inv_rho));
   const float nusgs = (tau - tb) * third;
-  const float kap = nusgs / P.prt + kb + P.kfloor;
-  r.taug = 0.5f + 4.0f * kap;
-  r.om = 1.0f - 1.0f / tau;
-  r.fpre = 1.0f - 0.5f / tau;
-  r.trN = (r.Nxx + r.Nyy + r.Nzz) * third;
+  const float kap = nusgs / P.prt + kb + P.kfloor;      // P.prt is uniform: one scalar division per wave
+  const float itau = rcp(tau);
+  const float om = 1.0f - itau, fpre = 1.0f - 0.5f * itau;
+  r.omg = rcp(0.5f + 4.0f * kap);
+  r.om45 = 4.5f * om;
+  r.fp3 = 3.0f * fpre;
+  r.fp9 = 9.0f * fpre;
+  const float usq = ux * ux + uy * uy + uz * uz;
+  const float trN = (r.Nxx + r.Nyy + r.Nzz) * third;
+  const float uF = ux * F.Fx + uy * F.Fy + uz * F.Fz;
+  // the part of w_q^-1 f_q^post common to every q: rho (1 - 1.5 u^2) - 4.5 om trN - 3 fpre u.F
+  r.C0 = rho * (1.0f - 1.5f * usq) - r.om45 * trN - r.fp3 * uF;
   return r;
 }
 
-// regularised collision with Guo forcing of population q (js/live/kernel.js:340-367)
+// regularised collision with Guo forcing of the pair (q, q+1), q odd (js/live/kernel.js:340-367):
+// f_q^post / w_q = C0 + 3 rho cu + 4.5 rho cu^2 + 4.5 om (c c):N + fpre (3 cF + 9 cu cF), split into the part
+// even in c (E) and the odd part (O); the opposite population gets E - O.
 template <int q>
-__device__ __forceinline__ float collide19(const Moments &m, const Forces &F, const Relax &R) {
+__device__ __forceinline__ void collide_pair(const Moments &m, const Forces &F, const Relax &R, float &fq, float &fo) {
   constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
-  const float w = w19<q>();
   const float cu = cdot<cx, cy, cz>(F.ux, F.uy, F.uz);
-  const float feq = w * m.rho * epoly(cu, R.usq);
-  // (c c):N in the generator's term order: diagonals xx, yy, zz, then cross xy, xz, yz
-  float qn = 0.0f;
-  bool first = true;
-  if (cx != 0) { qn = first ? R.Nxx : qn + R.Nxx; first = false; }
-  if (cy != 0) { qn = first ? R.Nyy : qn + R.Nyy; first = false; }
-  if (cz != 0) { qn = first ? R.Nzz : qn + R.Nzz; first = false; }
-  if (cx != 0 && cy != 0) { const float t = (float)(2 * cx * cy) * R.Nxy; qn = first ? t : qn + t; first = false; }
-  if (cx != 0 && cz != 0) { const float t = (float)(2 * cx * cz) * R.Nxz; qn = first ? t : qn + t; first = false; }
-  if (cy != 0 && cz != 0) { const float t = (float)(2 * cy * cz) * R.Nyz; qn = first ? t : qn + t; first = false; }
-  const float fr = w45<q>() * (qn - R.trN);
   const float cF = cdot<cx, cy, cz>(F.Fx, F.Fy, F.Fz);
-  const float Fq = R.fpre * w * (3.0f * (cF - (F.ux * F.Fx + F.uy * F.Fy + F.uz * F.Fz)) + 9.0f * cu * cF);
-  return feq + R.om * fr + Fq;
+  float qn = 0.0f;                                      // (c c):N
+  if (cx) qn += R.Nxx;
+  if (cy) qn += R.Nyy;
+  if (cz) qn += R.Nzz;
+  if (cx && cy) qn += (float)(2 * cx * cy) * R.Nxy;
+  if (cx && cz) qn += (float)(2 * cx * cz) * R.Nxz;
+  if (cy && cz) qn += (float)(2 * cy * cz) * R.Nyz;
+  const float E = R.C0 + cu * (4.5f * m.rho * cu + R.fp9 * cF) + R.om45 * qn;
+  const float O = 3.0f * m.rho * cu + R.fp3 * cF;
+  const float w = w19<q>();
+  fq = w * (E + O);
+  fo = w * (E - O);
 }
 
-// advection-diffusion collision of theta population q with the rooftop source and top sponge (:369-386)
-template <int q>
-__device__ __forceinline__ float collide7(float gq, float th, float omg, float S, float sg, const Forces &F) {
-  constexpr int cx = (q == 1) - (q == 2), cy = (q == 3) - (q == 4), cz = (q == 5) - (q == 6);
-  const float w = w7<q>();
-  const float geq = w * th * (1.0f + 4.0f * cdot<cx, cy, cz>(F.ux, F.uy, F.uz));
-  return (gq + omg * (geq - gq) + w * S) * (1.0f - sg);
+__device__ __forceinline__ float collide_rest(const Relax &R) { return w19<0>() * R.C0; }
+
+// D3Q7 collision with the rooftop source and the top-sponge damping (js/live/kernel.js:369-386):
+// g^post = ((1 - omg) g + omg w th (1 + 4 c.u) + w S) (1 - sg)
+struct Scalar {
+  float a, b, cs0, cs1;   // a = (1-omg)(1-sg), b = omg w1 th (1-sg) (w1 = 1/8), source terms w S (1-sg)
+};
+
+__device__ __forceinline__ Scalar scalar_coeffs(float th, float omg, float S, float sg) {
+  Scalar c;
+  const float ds = 1.0f - sg;
+  c.a = (1.0f - omg) * ds;
+  c.b = omg * 0.125f * th * ds;
+  c.cs0 = (omg * 0.25f * th + 0.25f * S) * ds;
+  c.cs1 = 0.125f * S * ds;
+  return c;
 }
 
 }  // namespace aht

@@ -5,9 +5,13 @@
 // unshifted loads are 256-byte aligned whenever nx*ny is a multiple of 64). blockIdx.y is the level k, starting at the
 // lowest fluid level of the grid, so all-solid planes are never launched. Solid cells above that plane exit at once.
 //
-// Boundary logic follows js/live/kernel.js:159-243: per population, the lid branch first, then the lateral faces,
-// then the interior pull with half-way bounce-back from solid sources. The 3 x 3 neighbour ktop values (the cacheNbr
-// path of the reference, js/live/kernel.js:102-110) are loaded once per thread.
+// Two streaming paths feed one collision:
+//   * fast path: interior cells (not on a lateral face, below the lid) at or above the column's kfast level, the first
+//     level where every source cell in the 3 x 3 x 3 neighbourhood is fluid (host-computed). All 26 pulls are plain
+//     loads through buffer descriptors: one per population with the pull offset folded into the scalar base, so the
+//     only per-lane address is the cell's byte offset. Drag and the rooftop source cannot apply there (k > ktop).
+//   * boundary path: every branch of js/live/kernel.js:159-243 per population (lid first, then lateral faces, then the
+//     interior pull with half-way bounce-back from solid sources), with the 3 x 3 neighbour ktop values loaded once.
 #include "kernels.h"
 
 namespace aht {
@@ -15,6 +19,22 @@ namespace aht {
 struct Outflow {
   float rn, uxn, uyn, uzn, usqn, uxc, uyc, usqc;
 };
+
+// raw buffer descriptor over a population array (gfx9 dword3); offsets are the 32-bit byte offsets of cells
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, -1, 0x00020000);
+}
+// the b32 builtins move raw bits: bit casts, never numeric conversions
+__device__ __forceinline__ float ldb(const float *base, uint32_t boff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc(base), boff, 0, 0));
+}
+__device__ __forceinline__ void stb(float *base, uint32_t boff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rsrc(base), boff, 0, 0);
+}
+// base address of population q shifted back by `off` cells (the pull offset), as a plain integer
+__device__ __forceinline__ const float *shifted(const float *a, size_t N, int q, int off) {
+  return (const float *)((uintptr_t)a + (uintptr_t)((int64_t)q * (int64_t)N - off) * 4u);
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -30,6 +50,11 @@ __device__ __forceinline__ void split_column(const Grid &g, int c, int &i, int &
   i = ii; j = jj;
 }
 
+// packed column word: kt | ktop << 10 | kfast << 20 (nz < 1024)
+__device__ __forceinline__ int col_kt(uint32_t x) { return (int)(x & 1023u); }
+__device__ __forceinline__ int col_ktop(uint32_t x) { return (int)((x >> 10) & 1023u); }
+__device__ __forceinline__ int col_kfast(uint32_t x) { return (int)(x >> 20); }
+
 // outflow-face prelude from the own cell's pre-stream populations (js/live/kernel.js:122-148)
 __device__ __forceinline__ Outflow outflow_prelude(const Grid &g, const float *__restrict__ fA, size_t N, uint32_t n,
                                                    int i, int j) {
@@ -42,21 +67,20 @@ __device__ __forceinline__ Outflow outflow_prelude(const Grid &g, const float *_
     float a[19];
 #pragma unroll
     for (int q = 0; q < 19; q++) a[q] = fA[(size_t)q * N + n];
-    float r = a[0];
+    float r = a[0], sx = 0.f, sy = 0.f, sz = 0.f;
 #pragma unroll
-    for (int q = 1; q < 19; q++) r = r + a[q];
-    float sx = 0.f, sy = 0.f, sz = 0.f;
-#pragma unroll
-    for (int q = 0; q < 19; q++) {
-      if (CX[q] == 1) sx = sx + a[q]; else if (CX[q] == -1) sx = sx - a[q];
-      if (CY[q] == 1) sy = sy + a[q]; else if (CY[q] == -1) sy = sy - a[q];
-      if (CZ[q] == 1) sz = sz + a[q]; else if (CZ[q] == -1) sz = sz - a[q];
+    for (int q = 1; q < 19; q++) {
+      r += a[q];
+      sx += (float)CX[q] * a[q];
+      sy += (float)CY[q] * a[q];
+      sz += (float)CZ[q] * a[q];
     }
-    o.rn = r; o.uxn = sx / r; o.uyn = sy / r; o.uzn = sz / r;
+    const float ir = rcp(r);
+    o.rn = r; o.uxn = sx * ir; o.uyn = sy * ir; o.uzn = sz * ir;
   }
   o.usqn = o.uxn * o.uxn + o.uyn * o.uyn + o.uzn * o.uzn;
   float uxc = o.uxn, uyc = o.uyn;
-  if (P.oclip != 0) {
+  if (P.oclip != 0) {                       // backflow-clipped outlet: no inward normal velocity
     if (i == 0 && P.inflow_w == 0 && uxc > 0.0f) uxc = 0.0f;
     if (i == g.nx - 1 && P.inflow_e == 0 && uxc < 0.0f) uxc = 0.0f;
     if (j == 0 && P.inflow_s == 0 && uyc > 0.0f) uyc = 0.0f;
@@ -67,7 +91,7 @@ __device__ __forceinline__ Outflow outflow_prelude(const Grid &g, const float *_
   return o;
 }
 
-// post-stream population q of cell (i, j, k) (js/live/kernel.js:159-217)
+// post-stream population q of cell (i, j, k), every boundary branch (js/live/kernel.js:159-217)
 template <int q>
 __device__ __forceinline__ float pull19(const Grid &g, const float *__restrict__ fA, size_t N, uint32_t n, int i, int j,
                                         int k, const int (&kn)[3][3], float uI, float vI, const Outflow &ob) {
@@ -88,13 +112,7 @@ __device__ __forceinline__ float pull19(const Grid &g, const float *__restrict__
     const bool inflow = (cx == 1 && i == 0 && P.inflow_w != 0) || (cx == -1 && i == NX - 1 && P.inflow_e != 0) ||
                         (cy == 1 && j == 0 && P.inflow_s != 0) || (cy == -1 && j == NY - 1 && P.inflow_n != 0);
     const float w = w19<q>();
-    if (inflow) {
-      // inflow face: equilibrium of rho = 1, U = (uI, vI, 0)
-      float cu = 0.0f;
-      if (cx) cu = cu + (float)cx * uI;
-      if (cy) cu = cu + (float)cy * vI;
-      return w * epoly(cu, uI * uI + vI * vI);
-    }
+    if (inflow) return w * epoly(cdot<cx, cy, 0>(uI, vI, 0.0f), uI * uI + vI * vI);   // equilibrium, rho = 1
     // pressure outlet: non-equilibrium extrapolation towards rho = 1, optionally backflow-clipped
     const float cu = cdot<cx, cy, cz>(ob.uxn, ob.uyn, ob.uzn);
     const float own = fA[(size_t)q * N + n];
@@ -115,7 +133,7 @@ __device__ __forceinline__ float pull19(const Grid &g, const float *__restrict__
 template <int q>
 __device__ __forceinline__ float pull7(const Grid &g, const float *__restrict__ gA, size_t N, uint32_t n, int i, int j,
                                        int k, const int (&kn)[3][3]) {
-  constexpr int cx = (q == 1) - (q == 2), cy = (q == 3) - (q == 4), cz = (q == 5) - (q == 6);
+  constexpr int cx = CX7[q], cy = CY7[q], cz = CZ7[q];
   const Params &P = g.P;
   if (cz == -1 && k == g.nz - 1) return gA[(size_t)q * N + n];   // zero-gradient top
   const bool outx = (cx == 1 && i == 0) || (cx == -1 && i == g.nx - 1);
@@ -160,32 +178,41 @@ struct Pull7<7> {
                                              int, int, int, const int (&)[3][3]) {}
 };
 
+// fast path: all 26 sources are fluid interior cells; plain shifted loads through per-population descriptors
 template <int q>
-struct Collide19 {
-  __device__ __forceinline__ static void run(float *__restrict__ fB, size_t N, uint32_t n, const Moments &m,
-                                             const Forces &F, const Relax &R) {
-    fB[(size_t)q * N + n] = collide19<q>(m, F, R);
-    Collide19<q + 1>::run(fB, N, n, m, F, R);
+struct FastPull {
+  __device__ __forceinline__ static void run(float (&f)[19], float (&gg)[7], const Grid &g,
+                                             const float *__restrict__ fA, const float *__restrict__ gA, uint32_t boff) {
+    const int off = CX[q] + CY[q] * g.nx + CZ[q] * (int)g.NXY;
+    f[q] = ldb(shifted(fA, g.N, q, off), boff);
+    if (q < 7) {
+      const int o7 = CX7[q < 7 ? q : 0] + CY7[q < 7 ? q : 0] * g.nx + CZ7[q < 7 ? q : 0] * (int)g.NXY;
+      gg[q < 7 ? q : 0] = ldb(shifted(gA, g.N, q, o7), boff);
+    }
+    FastPull<q + 1>::run(f, gg, g, fA, gA, boff);
   }
 };
 template <>
-struct Collide19<19> {
-  __device__ __forceinline__ static void run(float *__restrict__, size_t, uint32_t, const Moments &, const Forces &,
-                                             const Relax &) {}
+struct FastPull<19> {
+  __device__ __forceinline__ static void run(float (&)[19], float (&)[7], const Grid &, const float *__restrict__,
+                                             const float *__restrict__, uint32_t) {}
 };
 
 template <int q>
-struct Collide7 {
-  __device__ __forceinline__ static void run(float *__restrict__ gB, size_t N, uint32_t n, const float (&gg)[7], float th,
-                                             float omg, float S, float sg, const Forces &F) {
-    gB[(size_t)q * N + n] = collide7<q>(gg[q], th, omg, S, sg, F);
-    Collide7<q + 1>::run(gB, N, n, gg, th, omg, S, sg, F);
+struct StorePairs {
+  __device__ __forceinline__ static void run(float *__restrict__ fB, size_t N, uint32_t boff, const Moments &m,
+                                             const Forces &F, const Relax &R) {
+    float a, b;
+    collide_pair<q>(m, F, R, a, b);
+    stb(fB + (size_t)q * N, boff, a);
+    stb(fB + (size_t)(q + 1) * N, boff, b);
+    StorePairs<q + 2>::run(fB, N, boff, m, F, R);
   }
 };
 template <>
-struct Collide7<7> {
-  __device__ __forceinline__ static void run(float *__restrict__, size_t, uint32_t, const float (&)[7], float, float,
-                                             float, float, const Forces &) {}
+struct StorePairs<19> {
+  __device__ __forceinline__ static void run(float *__restrict__, size_t, uint32_t, const Moments &, const Forces &,
+                                             const Relax &) {}
 };
 
 __device__ __forceinline__ void load_neighbours(const Grid &g, int i, int j, int kg, int (&kn)[3][3]) {
@@ -197,6 +224,21 @@ __device__ __forceinline__ void load_neighbours(const Grid &g, int i, int j, int
     for (int b = 0; b < 3; b++) kn[a][b] = (a == 1 && b == 1) ? kg : (int)g.ktop[ys[b] * g.nx + xs[a]];
 }
 
+// streaming + moments + forces of one fluid cell: the fast or the boundary path
+__device__ __forceinline__ void stream_cell(const Grid &g, const float *__restrict__ fA, const float *__restrict__ gA,
+                                            uint32_t n, int i, int j, int k, int kg, int ktc, bool fast, float uI,
+                                            float vI, float (&f)[19], float (&gg)[7]) {
+  if (fast) {
+    FastPull<0>::run(f, gg, g, fA, gA, n * 4u);
+  } else {
+    int kn[3][3];
+    load_neighbours(g, i, j, kg, kn);
+    const Outflow ob = outflow_prelude(g, fA, g.N, n, i, j);
+    Pull19<0>::run(f, g, fA, g.N, n, i, j, k, kn, uI, vI, ob);
+    Pull7<0>::run(gg, g, gA, g.N, n, i, j, k, kn);
+  }
+}
+
 // the fused step (genStepWGSL, js/live/kernel.js:395-431) with FP64 accumulation (ACC) and debug outputs (OUT)
 template <bool ACC, bool OUT>
 __global__ __launch_bounds__(256) void step_f32(Grid g, const float *__restrict__ fA, float *__restrict__ fB,
@@ -206,7 +248,7 @@ __global__ __launch_bounds__(256) void step_f32(Grid g, const float *__restrict_
   const int k = g.kmin + blockIdx.y;
   const bool live = c < (int)g.NXY;
   const uint4 cv = g.colv[live ? c : 0];
-  const int ktc = (int)cv.x, kg = (int)cv.y;
+  const int ktc = col_kt(cv.x), kg = col_ktop(cv.x);
   const bool fluid = live && k >= kg;
   if (!ACC && !fluid) return;
   double th_cell = 0.0;
@@ -215,22 +257,27 @@ __global__ __launch_bounds__(256) void step_f32(Grid g, const float *__restrict_
     const uint32_t n = (uint32_t)k * g.NXY + (uint32_t)c;
     int i, j;
     split_column(g, c, i, j);
-    const int zagl = max(k - ktc, 0);
-    const float4 pr = g.prof[zagl];
-    const float uI = pr.x, vI = pr.y, kb = pr.z;
-    int kn[3][3];
-    load_neighbours(g, i, j, kg, kn);
-    const Outflow ob = outflow_prelude(g, fA, N, n, i, j);
+    const float4 pr = g.prof[max(k - ktc, 0)];
+    const bool fast = k >= col_kfast(cv.x) && k < g.nz - 1 && i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
     float f[19], gg[7];
-    Pull19<0>::run(f, g, fA, N, n, i, j, k, kn, uI, vI, ob);
-    Pull7<0>::run(gg, g, gA, N, n, i, j, k, kn);
-    const Moments m = moments<true>(f, gg);
-    const Forces F = forces(g.P, g.nz, k, kg, ktc, __uint_as_float(cv.z), uI, vI, m);
-    const Relax R = relax(g.P, g.nx, g.ny, i, j, kb, m, F);
-    Collide19<0>::run(fB, N, n, m, F, R);
+    stream_cell(g, fA, gA, n, i, j, k, kg, ktc, fast, pr.x, pr.y, f, gg);
+    const Moments m = moments(f, gg);
+    const Forces F = forces<true, true>(g.P, g.nz, k, kg, ktc, __uint_as_float(cv.z), pr.x, pr.y, m);
+    const Relax R = relax(g.P, g.nx, g.ny, i, j, pr.z, m, F);
+    const uint32_t boff = n * 4u;
+    stb(fB, boff, collide_rest(R));
+    StorePairs<1>::run(fB, N, boff, m, F, R);
+    // D3Q7: g^post = a g + b (1 + 4 c.u) + w S (1 - sg)
     const float S = (k == kg) ? __uint_as_float(cv.w) * g.P.srcscale : 0.0f;
-    const float omg = 1.0f / R.taug;
-    Collide7<0>::run(gB, N, n, gg, m.th, omg, S, F.sg, F);
+    const Scalar sc = scalar_coeffs(m.th, R.omg, S, F.sg);
+    const float b1 = sc.b + sc.cs1, b4 = 4.0f * sc.b;
+    stb(gB, boff, sc.a * gg[0] + sc.cs0);
+    stb(gB + N, boff, sc.a * gg[1] + (b1 + b4 * F.ux));
+    stb(gB + 2 * N, boff, sc.a * gg[2] + (b1 - b4 * F.ux));
+    stb(gB + 3 * N, boff, sc.a * gg[3] + (b1 + b4 * F.uy));
+    stb(gB + 4 * N, boff, sc.a * gg[4] + (b1 - b4 * F.uy));
+    stb(gB + 5 * N, boff, sc.a * gg[5] + (b1 + b4 * F.uz));
+    stb(gB + 6 * N, boff, sc.a * gg[6] + (b1 - b4 * F.uz));
     if (OUT) {
       out4[n] = m.th;
       out4[N + n] = F.ux;
@@ -286,7 +333,7 @@ __global__ __launch_bounds__(256) void surface_f32(Grid g, const float *__restri
   int i, j;
   split_column(g, c, i, j);
   const uint4 cv = g.colv[c];
-  const int ktc = (int)cv.x, kg = (int)cv.y, k = kg;
+  const int ktc = col_kt(cv.x), kg = col_ktop(cv.x), k = kg;
   const uint32_t o = (uint32_t)(g.ny - 1 - j) * g.nx + i;
   if (k >= g.nz) {
     out3[o] = 0.0f; out3[g.NXY + o] = 0.0f; out3[2 * g.NXY + o] = 0.0f;
@@ -302,19 +349,21 @@ __global__ __launch_bounds__(256) void surface_f32(Grid g, const float *__restri
   float f[19], gg[7];
   Pull19<0>::run(f, g, fA, N, n, i, j, k, kn, pr.x, pr.y, ob);
   Pull7<0>::run(gg, g, gA, N, n, i, j, k, kn);
-  const Moments m = moments<false>(f, gg);
-  const Forces F = forces(g.P, g.nz, k, kg, ktc, __uint_as_float(cv.z), pr.x, pr.y, m);
+  const Moments m = moments(f, gg);
+  const Forces F = forces<true, true>(g.P, g.nz, k, kg, ktc, __uint_as_float(cv.z), pr.x, pr.y, m);
   out3[o] = m.th;
   out3[g.NXY + o] = F.ux * conv;
   out3[2 * g.NXY + o] = F.uy * conv;
 }
 
 // equilibrium of the inflow profile at the local height above terrain, both lattices (genInitWGSL :524-553)
+// evaluated operation by operation as the reference (no contraction), so the state is bit-identical to the closed form
 __global__ __launch_bounds__(256) void init_f32(Grid g, float *__restrict__ f0, float *__restrict__ f1) {
+#pragma clang fp contract(off)
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.y;
   if (c >= (int)g.NXY) return;
-  const int ktc = (int)g.colv[c].x;
+  const int ktc = col_kt(g.colv[c].x);
   const int zagl = min(max(k - ktc, 0), g.nz - 1);
   const float4 pr = g.prof[zagl];
   const float U = pr.x, V = pr.y, usq = U * U + V * V;
@@ -324,7 +373,7 @@ __global__ __launch_bounds__(256) void init_f32(Grid g, float *__restrict__ f0, 
   for (int q = 0; q < 19; q++) {
     const float cu = (float)CX[q] * U + (float)CY[q] * V;
     const float w = q == 0 ? (float)(1.0 / 3.0) : (q < 7 ? (float)(1.0 / 18.0) : (float)(1.0 / 36.0));
-    const float fq = w * epoly(cu, usq);
+    const float fq = w * (((1.0f + 3.0f * cu) + 4.5f * cu * cu) - 1.5f * usq);
     f0[(size_t)q * N + n] = fq;
     f1[(size_t)q * N + n] = fq;
   }

@@ -1,0 +1,265 @@
+'use strict';
+// Node solver host: the reference's live-solver interface (js/live/solver.js, spec js/live/README.md:30-63) over the
+// N-API addon of libaht.so. Geometry, parameters and columns come from scenario.js; every device operation goes
+// through the addon, which fails loudly when the HIP library or a GPU is missing (there is no CPU path).
+//
+// Calls that touch the device are serialised on one promise chain per solver, so read() issued while steps are in
+// flight sees the state after them (the WebGPU queue ordering of the reference) and the C-ABI is only ever used by
+// one thread at a time.
+
+const path = require('path');
+const P = require('./physics');
+const { Scenario } = require('./scenario');
+
+let addon = null;
+function loadAddon() {
+  if (!addon) {
+    const p = path.join(__dirname, '..', 'lib', 'aht_node.node');
+    try {
+      addon = require(p);
+    } catch (e) {
+      throw new Error(`aht: the N-API addon ${p} is not built or cannot load libaht.so (${e.message})`);
+    }
+  }
+  return addon;
+}
+
+const STEP_ACC = 1, STEP_OUT = 2;
+const STORAGE = { f32: 0, f16: 1, bf16: 2 };
+
+/** createLiveSolver's counterpart (js/live/solver.js:44-101). opts: dx, u10Max, heightM, base (static10 directory),
+ *  synthetic (generator spec), nz, alpha, nsp, nsw, outflowClip, existingFlux, met, sideInflowDeg, metReinit, dirTol,
+ *  allowAnyDx, onProgress, device (HIP ordinal), storage ('f32' | 'f16' | 'bf16'), block */
+async function createSolver(opts) {
+  const o = Object.assign({ dx: 30, u10Max: 8, heightM: 720, onProgress: () => {}, device: 0 }, opts || {});
+  const prog = (t, f) => { try { o.onProgress(t, f); } catch (e) { /* UI errors ignored */ } };
+  const A = loadAddon();
+  prog('Building the grid', 0.2);
+  const sc = new Scenario(o);          // validates dx, u10Max, heightM with the reference's messages
+  const s = new Solver(A, sc, o, prog);
+  try {
+    s._init();
+  } catch (e) {
+    s.destroy();
+    throw (e instanceof Error ? e : new Error(String(e)));
+  }
+  prog('Ready', 1);
+  return s;
+}
+
+class Solver {
+  constructor(A, sc, o, prog) {
+    this._A = A; this._sc = sc; this._o = o; this._prog = prog;
+    this._h = null; this._destroyed = false;
+    this._n = 0; this._steps = 0; this._q = Promise.resolve();
+    this.flowReinits = 0;
+    this._flowRef = null; this._applied = null;
+    this.info = null;
+    this.campusInfo = [];
+  }
+
+  get simTime() { return this._n * this._sc.dt; }
+  get grid() { return this._sc.grid; }
+
+  _init() {
+    const sc = this._sc, A = this._A;
+    this._prog('Allocating GPU memory', 0.3);
+    const storage = STORAGE[this._o.storage || 'f32'];
+    if (storage === undefined) throw new Error(`aht: unknown storage ${this._o.storage}`);
+    this._h = A.create({ nx: sc.nx, ny: sc.ny, nz: sc.nz, storage, device: +this._o.device || 0, block: this._o.block || 0 });
+    this.info = Object.assign(sc.info(), { adapter: 'AMD Instinct (HIP, gfx950)', storage: this._o.storage || 'f32' });
+    this.nzv = this.info.nzv;
+    this._uploadColumns();
+    this._applyMet(this._o.met || P.DEFAULT_MET, true);
+    this._prog('Initialising the flow', 0.85);
+    this.resetFlow();
+  }
+
+  _check() {
+    if (this._destroyed) throw new Error('live solver: destroyed');
+  }
+
+  _enqueue(fn) {
+    const p = this._q.then(() => { this._check(); return fn(); });
+    this._q = p.catch(() => {});
+    return p;
+  }
+
+  _uploadColumns() {
+    const c = this._sc.col;
+    this._A.setColumns(this._h, c.kt, c.ktop, c.cd, c.q);
+    this.info.nFluid = this._sc.nFluid;
+  }
+
+  _uploadParams() {
+    this._A.setParams(this._h, this._sc.params, this._sc.profBlock());
+  }
+
+  _applyMet(m, first) {
+    const met = this._sc.setMet(m);
+    this._uploadParams();
+    // flow re-initialisation on a turn of more than dirTol against the lagged adjusted direction
+    // (js/live/solver.js:271-285)
+    const ref = this._flowRef, applied = this._applied;
+    let reinit = false;
+    if (!first && ref && applied && this._o.metReinit !== false) {
+      const ang = (a, b) => Math.abs(((a - b + 540) % 360) - 180);
+      const turn = Math.max(ang(met.dirFrom, ref.dir), ang(met.dirFrom, applied.dir));
+      const tol = P.def(this._o.dirTol, P.DIR_TOL);
+      if (met.u10 > P.CALM && Math.max(ref.u, applied.u) > P.CALM && turn > tol) reinit = true;
+    }
+    this._applied = { dir: met.dirFrom, u: met.u10 };
+    if (!ref || reinit) this._flowRef = { dir: met.dirFrom, u: met.u10 };
+    if (reinit) {
+      this._A.initFlow(this._h);
+      this.flowReinits += 1;
+    }
+    return met;
+  }
+
+  _advanceFlowRef(n) {
+    const ref = this._flowRef, ap = this._applied;
+    if (!ref || !ap) return;
+    const a = 1 - Math.exp(-n * this._sc.dt / P.ADJ_TAU_S);
+    ref.dir = (((ref.dir + (((ap.dir - ref.dir + 540) % 360) - 180) * a) % 360) + 360) % 360;
+    ref.u += (ap.u - ref.u) * a;
+  }
+
+  /** { u10, dirFrom, wstar, blh, ustar }; applies from the next step (js/live/solver.js:245-285) */
+  setMet(m) {
+    this._check();
+    return this._enqueue(() => this._applyMet(m || {}, false));
+  }
+
+  /** W per m^2 of operational data-center roof (js/live/solver.js:305-310) */
+  setExistingFlux(wm2) {
+    this._check();
+    this._sc.setExistingFlux(wm2);
+    this.info.heatExistingMW = this._sc.dcRoof * this._sc.flux / 1e6;
+    return this._enqueue(() => this._uploadColumns());
+  }
+
+  /** [{ id, halls: [[x0, z0, x1, z1], ...] (scene metres), mw }]; replaces the previous list; cells uncovered by a
+   *  removed campus restart at rest (js/live/solver.js:313-356) */
+  setNewCampuses(list) {
+    this._check();
+    const g = this._sc.setCampuses(list);
+    this.campusInfo = this._sc.ras.info;
+    return this._enqueue(() => {
+      this._uploadColumns();
+      if (g.grow) this._A.reinitCells(this._h, g.oldKtop);
+    });
+  }
+
+  /** theta -> 0, flow kept, simTime -> 0 */
+  resetHeat() {
+    this._check();
+    this._n = 0; this._steps = 0;
+    return this._enqueue(() => this._A.resetHeat(this._h));
+  }
+
+  /** equilibrium with the inflow profile at the local height above terrain, theta -> 0, simTime -> 0 */
+  resetFlow() {
+    this._check();
+    this._n = 0; this._steps = 0;
+    if (this._applied) this._flowRef = Object.assign({}, this._applied);
+    return this._enqueue(() => { this._A.initFlow(this._h); this._A.resetHeat(this._h); });
+  }
+
+  /** n steps; resolves when the GPU has finished them */
+  step(n, flags) {
+    try { this._check(); } catch (e) { return Promise.reject(e); }
+    n = Math.max(0, Math.floor(n === undefined ? 1 : n));
+    this._n += n; this._steps += n;
+    this._advanceFlowRef(n);
+    return this._enqueue(() => (n > 0 ? this._A.step(this._h, n, flags || 0) : undefined));
+  }
+
+  /** snapshot of the near-ground state: { simTime, surf, u, v, vol, volMax, nzv, thetaMax, heatMW, steps } */
+  read(opts) {
+    const o = Object.assign({ vol: true, wind: true, volMax: 3.0 }, opts || {});
+    try { this._check(); } catch (e) { return Promise.reject(e); }
+    const simTime = this.simTime, steps = this._steps, heatMW = this._sc.heatMW;
+    return this._enqueue(() => {
+      const NXY = this._sc.nx * this._sc.ny, vm = o.volMax > 0 ? +o.volMax : 3.0;
+      const surf = new Float32Array(NXY);
+      const u = o.wind ? new Float32Array(NXY) : null, v = o.wind ? new Float32Array(NXY) : null;
+      this._A.readSurface(this._h, this._sc.conv, surf, u, v);
+      const vol = o.vol ? new Uint8Array(this.nzv * NXY) : null;
+      const thetaMax = this._A.readVolume(this._h, this.nzv, vm, vol);
+      return { simTime, surf, u, v, vol, volMax: vm, nzv: this.nzv, thetaMax, heatMW, steps };
+    });
+  }
+
+  /** steps the model and returns MLUPS over all nx*ny*nz cells, device time from HIP events */
+  benchmark(nSteps) {
+    const n = nSteps === undefined ? 300 : nSteps;
+    return this.step(Math.min(20, n)).then(() => {
+      this._n += n; this._steps += n;
+      this._advanceFlowRef(n);
+      return this._enqueue(() => this._A.benchmark(this._h, n)).then(ms => this.info.nCells * n / (ms * 1e3));
+    });
+  }
+
+  destroy() {
+    if (this._destroyed) return;
+    this._destroyed = true;
+    if (this._h) this._A.destroy(this._h);
+    this._h = null;
+  }
+
+  /** validation hooks (js/live/solver.js:496-577) */
+  get _dbg() {
+    const self = this, A = this._A, sc = this._sc;
+    const N = () => sc.nx * sc.ny * sc.nz;
+    return {
+      domain() {
+        return { nx: sc.nx, ny: sc.ny, nz: sc.nz, dx: sc.dx, zbase: sc.dom.zbase, dt: sc.dt, conv: sc.conv,
+          params: Object.assign({}, sc.params), inflow: Object.assign({}, sc.inflow), met: Object.assign({}, sc.met),
+          uin: sc.prof.ul, vin: sc.prof.vl, kbg: sc.prof.kl, kt: sc.col.kt, ktop: sc.col.ktop, cd: sc.col.cd, q: sc.col.q,
+          heatMW: sc.heatMW, steps: self._steps, campusInfo: self.campusInfo };
+      },
+      readState() {
+        return self._enqueue(() => {
+          const f = new Float32Array(19 * N()), g = new Float32Array(7 * N());
+          A.readState(self._h, f, g);
+          return { f, g };
+        });
+      },
+      writeState(f, g) { return self._enqueue(() => A.writeState(self._h, f || null, g || null)); },
+      stepOut(n) { return self.step(n, STEP_OUT); },
+      readDbg() {
+        return self._enqueue(() => { const o = new Float32Array(4 * N()); A.readOut(self._h, o); return o; });
+      },
+      stepAcc(n) { return self.step(n, STEP_ACC); },
+      clearDbg() { return self._enqueue(() => A.meanReset(self._h)); },
+      /** FP64 time means: surface (theta, u, v in m/s, north-up) and the heat budget in MW / MJ */
+      mean() {
+        return self._enqueue(() => {
+          const NXY = sc.nx * sc.ny, s3 = new Float64Array(3 * NXY);
+          const b = A.readMean(self._h, s3);
+          for (let k = NXY; k < 3 * NXY; k++) s3[k] *= sc.conv;
+          const dx = sc.dx;
+          return {
+            nAcc: b.n_acc, surf: s3.subarray(0, NXY), u: s3.subarray(NXY, 2 * NXY), v: s3.subarray(2 * NXY),
+            heatInjectedMW: b.heat_in * P.RHO_CP * dx * dx * dx / sc.dt / 1e6,
+            heatAdvectedOutMW: P.RHO_CP * b.heat_out * sc.conv * dx * dx / 1e6,
+            heatContentMJ: b.content * P.RHO_CP * dx * dx * dx / 1e6,
+            massImbalance: b.mass_imbalance,
+          };
+        });
+      },
+      stats() {
+        return self._enqueue(() => {
+          const d = A.stats(self._h), c = sc.conv, dx = sc.dx;
+          return { simTime: self.simTime, steps: self._steps, nFluid: d.n_fluid, nonFinite: d.non_finite,
+            thetaMax: d.theta_max, heatContentMJ: d.sum_theta * P.RHO_CP * dx * dx * dx / 1e6,
+            keMean: d.sum_ke / Math.max(d.n_fluid, 1) * c * c, rhoDevMean: d.sum_drho / Math.max(d.n_fluid, 1),
+            umax: d.umax * c };
+        });
+      },
+    };
+  }
+}
+
+module.exports = { createSolver, Solver, STEP_ACC, STEP_OUT, loadAddon };

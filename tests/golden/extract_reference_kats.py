@@ -43,6 +43,11 @@ def main():
             'speed_mean': {'webgpu': lr['surface_mean_wind']['speed_mean_webgpu'],
                            'numba': lr['surface_mean_wind']['speed_mean_numba']},
         }
+    api = d['api20']['result']
+    out['api20'] = {k: api[k] for k in ('badDx', 'badU10Max', 'readNoVolNoWind', 'concurrentReads', 'campuses',
+                                        'heatExistingMW800', 'afterResetHeat', 'afterResetFlow', 'flowReinits',
+                                        'afterDestroy')}
+    out['api20']['dt'] = api['info']['dt']
     json.dump(out, open(OUT, 'w'), indent=1)
     print('->', OUT)
 

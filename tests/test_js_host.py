@@ -1,0 +1,57 @@
+"""The Node host without a GPU: option validation (the reference's messages, js/live/solver.js:50-54), the clean
+failure when no device exists, determinism of the seeded synthetic generator, and the workload geometry."""
+import json
+import os
+import subprocess
+
+import numpy as np
+
+from conftest import ROOT, dump_domain, load_summary
+
+API = os.path.join(ROOT, 'tests', 'js', 'api_test.js')
+
+
+def node_eval(code):
+    out = subprocess.run(['node', '-e', code], capture_output=True, text=True, check=True, cwd=ROOT).stdout
+    return json.loads(out)
+
+
+def test_rejection_messages_match_reference():
+    K = load_summary()['api20']
+    out = json.loads(subprocess.run(['node', API, '--cpu'], capture_output=True, text=True, check=True).stdout)
+    assert out['badDx'] == K['badDx']
+    assert out['badU10Max'] == K['badU10Max']
+    assert out['badHeight'].startswith('rejected: live solver: heightM must be within 300-2000 m')
+    assert out['noGpu'].startswith('rejected: aht:')          # no GPU here: the addon refuses, no CPU path
+
+
+def test_synthetic_generator_is_seeded(tmp_path):
+    a = dump_domain(tmp_path / 'a', '--tiny', '40,30,16,5')
+    b = dump_domain(tmp_path / 'b', '--tiny', '40,30,16,5')
+    c = dump_domain(tmp_path / 'c', '--tiny', '40,30,16,6')
+    for k in ('kt', 'ktop', 'cd', 'q', 'prof'):
+        assert np.array_equal(a[k], b[k]), k
+    assert not (np.array_equal(a['ktop'], c['ktop']) and np.array_equal(a['q'], c['q']))
+
+
+def test_workload_c1_geometry(tmp_path):
+    d = dump_domain(tmp_path, '--workload', 'c1')
+    assert (d['nx'], d['ny'], d['nz'], d['dx']) == (200, 200, 32, 30)
+    kt, ktop = d['kt'].astype(int), d['ktop'].astype(int)
+    assert kt.min() >= 1 and (ktop >= kt).all() and ktop.max() < d['nz'] - 8
+    assert d['params']['inflow_w'] == 1 and d['params']['inflow_s'] == 1      # wind from 225 deg
+    # 120 source roofs at 400 W/m2: heat equals 400 W/m2 times the rasterised source roof area
+    assert abs(d['heatMW'] - 400 * d['info']['dcRoofArea_m2'] / 1e6) < 1e-6 * d['heatMW']
+    assert d['info']['dt'] == 0.1 * 30 / (5 * 15 ** 0.2)
+
+
+def test_campus_rasterisation_conserves_area():
+    r = node_eval("""
+      const D = require('./ashburn-heat-twin_amd/js/domain.js');
+      const cp = [{ id: 'X', mw: 60, halls: [[-1013, 222, -713, 422]] }];
+      const a = D.rasterizeCampuses(cp, 30, 200, 200);
+      let area = 0, heat = 0; for (let k = 0; k < a.newArea.length; k++) { area += a.newArea[k]; heat += a.heatW[k]; }
+      console.log(JSON.stringify({ area, heat, info: a.info }));""")
+    assert abs(r['area'] - 300 * 200) < 1e-6
+    assert abs(r['heat'] - 60e6) < 1e-3
+    assert r['info'][0]['cells'] == 11 * 7          # columns 66..76, rows 107..113
