@@ -20,7 +20,8 @@ LIB_PATH = os.path.join(LIB_DIR, 'libaht.so')
 ROOT = os.path.dirname(HERE)
 
 AHT_OK, AHT_EINVAL, AHT_ENOMEM, AHT_EDEVICE, AHT_ESTATE = 0, -1, -2, -3, -4
-STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
+STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
+STORAGE_NAMES = {'f32': 0, 'f16': 1, 'bf16': 2, 'f64': 3}
 STEP_ACC, STEP_OUT = 1, 2
 FACE_W, FACE_E, FACE_S, FACE_N = 0, 1, 2, 3
 RHO_CP = 1.16 * 1005.0
@@ -82,6 +83,8 @@ SIGNATURES = {
     'aht_read_face_mean': (ctypes.c_int, [_vp, _i32, _vp]),
     'aht_read_state': (ctypes.c_int, [_vp, _vp, _vp]),
     'aht_write_state': (ctypes.c_int, [_vp, _vp, _vp]),
+    'aht_read_state_f64': (ctypes.c_int, [_vp, _vp, _vp]),
+    'aht_write_state_f64': (ctypes.c_int, [_vp, _vp, _vp]),
     'aht_footprint_bytes': (_i64, [ctypes.POINTER(Dims)]),
 }
 
@@ -124,6 +127,8 @@ class Solver:
 
     def __init__(self, nx, ny, nz, storage=STORE_F32, device=0, block=0):
         L = lib()
+        storage = STORAGE_NAMES.get(storage, storage)
+        self.storage = storage
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.N = self.nx * self.ny * self.nz
         self.NXY = self.nx * self.ny
@@ -258,6 +263,17 @@ class Solver:
         fa = None if f is None else np.ascontiguousarray(f, np.float32)
         ga = None if g is None else np.ascontiguousarray(g, np.float32)
         self._check(lib().aht_write_state(self.h, _ptr(fa), _ptr(ga)))
+
+    def read_state_f64(self):
+        f = np.empty((19, self.N), np.float64)
+        g = np.empty((7, self.N), np.float64)
+        self._check(lib().aht_read_state_f64(self.h, _ptr(f), _ptr(g)))
+        return f, g
+
+    def write_state_f64(self, f=None, g=None):
+        fa = None if f is None else np.ascontiguousarray(f, np.float64)
+        ga = None if g is None else np.ascontiguousarray(g, np.float64)
+        self._check(lib().aht_write_state_f64(self.h, _ptr(fa), _ptr(ga)))
 
 
 def budget_physical(b, dx, dt, conv):

@@ -21,7 +21,10 @@ struct aht_solver {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   aht::Grid g{};
   int block = aht::STEP_BLOCK;
-  float *f[2] = {nullptr, nullptr}, *gq[2] = {nullptr, nullptr};
+  int storage = AHT_STORE_F32;
+  int eb = 4;                 // bytes per stored population element
+  void *f[2] = {nullptr, nullptr}, *gq[2] = {nullptr, nullptr};
+  void *stage = nullptr;      // one population plane in f64, for state conversion
   uint4 *colv = nullptr;
   uint32_t *ktop = nullptr, *oldk = nullptr;
   float4 *prof = nullptr;
@@ -96,7 +99,8 @@ const char *aht_last_error(const aht_solver *s) { return s ? s->err.c_str() : "n
 int64_t aht_footprint_bytes(const aht_dims *d) {
   if (!d || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return -1;
   const int64_t N = (int64_t)d->nx * d->ny * d->nz, NXY = (int64_t)d->nx * d->ny;
-  const int64_t pop = 2 * 26 * N * 4;
+  const int eb = d->storage == AHT_STORE_F32 ? 4 : (d->storage == AHT_STORE_F64 ? 8 : 2);
+  const int64_t pop = 2 * 26 * N * eb;
   const int64_t cols = NXY * (16 + 4 + 4) + (int64_t)d->nz * 16;
   const int64_t acc = 3 * NXY * 8 + (int64_t)face_total(d->nx, d->ny, d->nz) * 8 + (int64_t)max_content_slots(*d, 256) * 8;
   const int64_t misc = ((N + 255) / 256) * 64 + 3 * NXY * 4 + (N + 3) / 4 * 4 / 1 + 4096;
@@ -109,12 +113,14 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   const aht_dims d = *dims;
   if (d.nx < 2 || d.ny < 2 || d.nz < 2 || d.nz > 1022) return AHT_EINVAL;
   const uint64_t N = (uint64_t)d.nx * d.ny * d.nz;
-  if (N >= (1ull << 30)) return AHT_EINVAL;   // cell byte offsets stay 32-bit
-  if (d.storage != AHT_STORE_F32) return AHT_EINVAL;
+  if (d.storage < AHT_STORE_F32 || d.storage > AHT_STORE_F64) return AHT_EINVAL;
+  if (N * (uint64_t)aht::storage_bytes(d.storage) >= (1ull << 32)) return AHT_EINVAL;   // byte offsets stay 32-bit
   const int block = d.block ? d.block : aht::STEP_BLOCK;
   if (block < 64 || block > 256 || block % 64) return AHT_EINVAL;
   aht_solver *s = new aht_solver();
   s->dims = d;
+  s->storage = d.storage;
+  s->eb = aht::storage_bytes(d.storage);
   s->device = d.device;
   s->block = block;
   int ndev = 0;
@@ -134,10 +140,10 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   g.nx = d.nx; g.ny = d.ny; g.nz = d.nz; g.kmin = d.nz; g.NXY = NXY; g.N = (uint32_t)N;
   g.inv_nx = 1.0f / (float)d.nx;
   for (int p = 0; p < 2; p++) {
-    HIP_TRY(s, hipMalloc(&s->f[p], 19 * N * sizeof(float)));
-    HIP_TRY(s, hipMalloc(&s->gq[p], 7 * N * sizeof(float)));
-    HIP_TRY(s, hipMemsetAsync(s->f[p], 0, 19 * N * sizeof(float), s->st));
-    HIP_TRY(s, hipMemsetAsync(s->gq[p], 0, 7 * N * sizeof(float), s->st));
+    HIP_TRY(s, hipMalloc(&s->f[p], 19 * N * s->eb));
+    HIP_TRY(s, hipMalloc(&s->gq[p], 7 * N * s->eb));
+    HIP_TRY(s, hipMemsetAsync(s->f[p], 0, 19 * N * s->eb, s->st));
+    HIP_TRY(s, hipMemsetAsync(s->gq[p], 0, 7 * N * s->eb, s->st));
   }
   HIP_TRY(s, hipMalloc(&s->colv, NXY * sizeof(uint4)));
   HIP_TRY(s, hipMalloc(&s->ktop, NXY * sizeof(uint32_t)));
@@ -165,7 +171,7 @@ void aht_destroy(aht_solver *s) {
     (void)hipSetDevice(s->device);
     (void)hipStreamSynchronize(s->st);
   }
-  void *bufs[] = {s->f[0], s->f[1], s->gq[0], s->gq[1], s->colv, s->ktop, s->oldk, s->prof, s->acc_surf, s->acc_face,
+  void *bufs[] = {s->f[0], s->f[1], s->gq[0], s->gq[1], s->stage, s->colv, s->ktop, s->oldk, s->prof, s->acc_surf, s->acc_face,
                   s->acc_content, s->red_out, s->stat_part, s->out4, s->surf3, s->vol, s->tmax};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -233,7 +239,7 @@ int aht_init_flow(aht_solver *s) {
   int rc = check_ready(s);
   if (rc) return rc;
   if ((rc = set_device(s))) return rc;
-  HIP_TRY(s, aht::launch_init(s->g, s->f[0], s->f[1], s->st));
+  HIP_TRY(s, aht::launch_init(s->storage, s->g, s->f[0], s->f[1], s->st));
   return AHT_OK;
 }
 
@@ -241,7 +247,7 @@ int aht_reset_heat(aht_solver *s) {
   if (!s) return AHT_EINVAL;
   int rc = set_device(s);
   if (rc) return rc;
-  const size_t b = 7 * (size_t)s->g.N * sizeof(float);
+  const size_t b = 7 * (size_t)s->g.N * s->eb;      // zero bits are 0.0 in every storage format
   HIP_TRY(s, hipMemsetAsync(s->gq[0], 0, b, s->st));
   HIP_TRY(s, hipMemsetAsync(s->gq[1], 0, b, s->st));
   return AHT_OK;
@@ -253,7 +259,7 @@ int aht_reinit_cells(aht_solver *s, const uint32_t *old_ktop) {
   if (!old_ktop) return fail(s, AHT_EINVAL, "null old_ktop");
   if ((rc = set_device(s))) return rc;
   HIP_TRY(s, hipMemcpyAsync(s->oldk, old_ktop, s->g.NXY * sizeof(uint32_t), hipMemcpyHostToDevice, s->st));
-  HIP_TRY(s, aht::launch_reinit(s->g, s->oldk, s->f[s->parity], s->gq[s->parity], s->st));
+  HIP_TRY(s, aht::launch_reinit(s->storage, s->g, s->oldk, s->f[s->parity], s->gq[s->parity], s->st));
   HIP_TRY(s, hipStreamSynchronize(s->st));
   return AHT_OK;
 }
@@ -270,8 +276,8 @@ int aht_step(aht_solver *s, int32_t n, uint32_t flags) {
     const int p = s->parity;
     float *o4 = (out && it == n - 1) ? s->out4 : nullptr;
     if (o4) HIP_TRY(s, hipMemsetAsync(o4, 0, 4 * (size_t)s->g.N * sizeof(float), s->st));
-    HIP_TRY(s, aht::launch_step(s->g, s->f[p], s->f[1 - p], s->gq[p], s->gq[1 - p], acc ? &a : nullptr, o4, s->block,
-                                s->st));
+    HIP_TRY(s, aht::launch_step(s->storage, s->g, s->f[p], s->f[1 - p], s->gq[p], s->gq[1 - p], acc ? &a : nullptr, o4,
+                                s->block, s->st));
     s->parity ^= 1;
     s->steps++;
     if (acc) s->n_acc++;
@@ -312,7 +318,7 @@ int aht_read_surface(aht_solver *s, float conv, float *theta, float *u, float *v
   if ((rc = set_device(s))) return rc;
   const int p = s->parity;
   const size_t NXY = s->g.NXY;
-  HIP_TRY(s, aht::launch_surface(s->g, s->f[p], s->gq[p], conv, s->surf3, s->st));
+  HIP_TRY(s, aht::launch_surface(s->storage, s->g, s->f[p], s->gq[p], conv, s->surf3, s->st));
   HIP_TRY(s, hipMemcpyAsync(theta, s->surf3, NXY * sizeof(float), hipMemcpyDeviceToHost, s->st));
   if (u) HIP_TRY(s, hipMemcpyAsync(u, s->surf3 + NXY, NXY * sizeof(float), hipMemcpyDeviceToHost, s->st));
   if (v) HIP_TRY(s, hipMemcpyAsync(v, s->surf3 + 2 * NXY, NXY * sizeof(float), hipMemcpyDeviceToHost, s->st));
@@ -327,7 +333,7 @@ int aht_read_volume(aht_solver *s, int32_t nzv, float volmax, uint8_t *vol, floa
   if ((rc = set_device(s))) return rc;
   const float vm = volmax > 0.0f ? volmax : 3.0f;
   HIP_TRY(s, hipMemsetAsync(s->tmax, 0, sizeof(uint32_t), s->st));
-  HIP_TRY(s, aht::launch_volume(s->g, s->gq[s->parity], nzv, 255.0f / vm, s->vol, s->tmax, s->st));
+  HIP_TRY(s, aht::launch_volume(s->storage, s->g, s->gq[s->parity], nzv, 255.0f / vm, s->vol, s->tmax, s->st));
   if (vol) HIP_TRY(s, hipMemcpyAsync(vol, s->vol, (size_t)nzv * s->g.NXY, hipMemcpyDeviceToHost, s->st));
   uint32_t tb = 0;
   HIP_TRY(s, hipMemcpyAsync(&tb, s->tmax, sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
@@ -342,7 +348,7 @@ int aht_stats(aht_solver *s, aht_diag *out) {
   if (!out) return fail(s, AHT_EINVAL, "null stats");
   if ((rc = set_device(s))) return rc;
   const int p = s->parity;
-  HIP_TRY(s, aht::launch_stats(s->g, s->f[p], s->gq[p], s->stat_part, s->red_out, s->st));
+  HIP_TRY(s, aht::launch_stats(s->storage, s->g, s->f[p], s->gq[p], s->stat_part, s->red_out, s->st));
   double r[8];
   HIP_TRY(s, hipMemcpyAsync(r, s->red_out, 7 * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(s, hipStreamSynchronize(s->st));
@@ -434,28 +440,57 @@ int aht_read_mean(aht_solver *s, double *surf3, aht_budget *b) {
   return AHT_OK;
 }
 
-int aht_read_state(aht_solver *s, float *f, float *g) {
+// state exchange through one staging plane: storage <-> f32 / f64, one population at a time
+static int state_io(aht_solver *s, void *f, void *g, int f64, int write) {
   if (!s) return AHT_EINVAL;
   int rc = set_device(s);
   if (rc) return rc;
   const size_t N = s->g.N;
   const int p = s->parity;
-  if (f) HIP_TRY(s, hipMemcpyAsync(f, s->f[p], 19 * N * sizeof(float), hipMemcpyDeviceToHost, s->st));
-  if (g) HIP_TRY(s, hipMemcpyAsync(g, s->gq[p], 7 * N * sizeof(float), hipMemcpyDeviceToHost, s->st));
+  const size_t hb = f64 ? 8 : 4;
+  // f32 storage and f32 host format: plain copies
+  if (s->storage == AHT_STORE_F32 && !f64) {
+    if (write) {
+      if (f) HIP_TRY(s, hipMemcpyAsync(s->f[p], f, 19 * N * 4, hipMemcpyHostToDevice, s->st));
+      if (g) HIP_TRY(s, hipMemcpyAsync(s->gq[p], g, 7 * N * 4, hipMemcpyHostToDevice, s->st));
+    } else {
+      if (f) HIP_TRY(s, hipMemcpyAsync(f, s->f[p], 19 * N * 4, hipMemcpyDeviceToHost, s->st));
+      if (g) HIP_TRY(s, hipMemcpyAsync(g, s->gq[p], 7 * N * 4, hipMemcpyDeviceToHost, s->st));
+    }
+    HIP_TRY(s, hipStreamSynchronize(s->st));
+    return AHT_OK;
+  }
+  if (!s->stage) HIP_TRY(s, hipMalloc(&s->stage, N * 8));
+  const float w0 = (float)(1.0 / 3.0), w1 = (float)(1.0 / 18.0), w2 = (float)(1.0 / 36.0);
+  const float W19[19] = {w0, w1, w1, w1, w1, w1, w1, w2, w2, w2, w2, w2, w2, w2, w2, w2, w2, w2, w2};
+  for (int is_f = 1; is_f >= 0; is_f--) {
+    char *host = (char *)(is_f ? f : g);
+    if (!host) continue;
+    char *dev = (char *)(is_f ? s->f[p] : s->gq[p]);
+    const int nq = is_f ? 19 : 7;
+    for (int q = 0; q < nq; q++) {
+      const float w = is_f ? W19[q] : 0.0f;
+      char *plane = dev + (size_t)q * N * s->eb;
+      if (write) {
+        HIP_TRY(s, hipMemcpyAsync(s->stage, host + (size_t)q * N * hb, N * hb, hipMemcpyHostToDevice, s->st));
+        HIP_TRY(s, aht::launch_encode(s->storage, s->stage, f64, plane, (uint32_t)N, w, is_f, s->st));
+      } else {
+        HIP_TRY(s, aht::launch_decode(s->storage, plane, s->stage, f64, (uint32_t)N, w, is_f, s->st));
+        HIP_TRY(s, hipMemcpyAsync(host + (size_t)q * N * hb, s->stage, N * hb, hipMemcpyDeviceToHost, s->st));
+      }
+    }
+  }
   HIP_TRY(s, hipStreamSynchronize(s->st));
   return AHT_OK;
 }
 
+int aht_read_state(aht_solver *s, float *f, float *g) { return state_io(s, f, g, 0, 0); }
 int aht_write_state(aht_solver *s, const float *f, const float *g) {
-  if (!s) return AHT_EINVAL;
-  int rc = set_device(s);
-  if (rc) return rc;
-  const size_t N = s->g.N;
-  const int p = s->parity;
-  if (f) HIP_TRY(s, hipMemcpyAsync(s->f[p], f, 19 * N * sizeof(float), hipMemcpyHostToDevice, s->st));
-  if (g) HIP_TRY(s, hipMemcpyAsync(s->gq[p], g, 7 * N * sizeof(float), hipMemcpyHostToDevice, s->st));
-  HIP_TRY(s, hipStreamSynchronize(s->st));
-  return AHT_OK;
+  return state_io(s, (void *)f, (void *)g, 0, 1);
+}
+int aht_read_state_f64(aht_solver *s, double *f, double *g) { return state_io(s, f, g, 1, 0); }
+int aht_write_state_f64(aht_solver *s, const double *f, const double *g) {
+  return state_io(s, (void *)f, (void *)g, 1, 1);
 }
 
 }  // extern "C"

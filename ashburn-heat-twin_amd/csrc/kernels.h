@@ -34,17 +34,27 @@ size_t face_offset(int face, int nx, int ny, int nz);   // offset of face 0..3 i
 size_t face_len(int face, int nx, int ny, int nz);      // doubles per face (2 * nz * ny or 2 * nz * nx)
 uint32_t content_slots(const Grid &g, int block);       // wavefront slots of the step grid
 
-hipError_t launch_step(const Grid &g, const float *fA, float *fB, const float *gA, float *gB, const Acc *acc,
+int storage_bytes(int storage);                         // bytes per population element of an AHT_STORE_* format
+
+// kernels are templated on the storage format; the launchers dispatch on the AHT_STORE_* id
+hipError_t launch_step(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB, const Acc *acc,
                        float *out4, int block, hipStream_t st);
-hipError_t launch_init(const Grid &g, float *f0, float *f1, hipStream_t st);
-hipError_t launch_reinit(const Grid &g, const uint32_t *old_ktop, float *f, float *gq, hipStream_t st);
-hipError_t launch_surface(const Grid &g, const float *fA, const float *gA, float conv, float *out3, hipStream_t st);
-hipError_t launch_volume(const Grid &g, const float *gA, int nzv, float volscale, uint32_t *vout, uint32_t *tmax_bits,
-                         hipStream_t st);
+hipError_t launch_init(int storage, const Grid &g, void *f0, void *f1, hipStream_t st);
+hipError_t launch_reinit(int storage, const Grid &g, const uint32_t *old_ktop, void *f, void *gq, hipStream_t st);
+hipError_t launch_surface(int storage, const Grid &g, const void *fA, const void *gA, float conv, float *out3,
+                          hipStream_t st);
+hipError_t launch_volume(int storage, const Grid &g, const void *gA, int nzv, float volscale, uint32_t *vout,
+                         uint32_t *tmax_bits, hipStream_t st);
 // two-pass deterministic reduction: per-block partials into part (>= stats_partials(g) entries), then one block
-hipError_t launch_stats(const Grid &g, const float *fA, const float *gA, double *part, double *out8, hipStream_t st);
+hipError_t launch_stats(int storage, const Grid &g, const void *fA, const void *gA, double *part, double *out8,
+                        hipStream_t st);
 uint32_t stats_partials(const Grid &g);
-// sums content slots, surface and face accumulators into out (deterministic order)
+// fixed-order sum of n doubles into *out (one block)
 hipError_t launch_sum(const double *x, uint32_t n, double *out, hipStream_t st);
+// one population plane of n cells between storage and f32 (or f64) host-format buffers; w = rest weight for f planes
+hipError_t launch_decode(int storage, const void *src, void *dst, int dst_f64, uint32_t n, float w, int is_f,
+                         hipStream_t st);
+hipError_t launch_encode(int storage, const void *src, int src_f64, void *dst, uint32_t n, float w, int is_f,
+                         hipStream_t st);
 
 }  // namespace aht

@@ -8,7 +8,9 @@
 //     collision evaluates the part even in c once and the odd part once, then writes both populations;
 //   * 1/rho, 1/tau and 1/tau_g are each computed once (v_rcp_f32) and multiplied, square roots use v_sqrt_f32;
 //   * the file is compiled with FMA contraction.
-// The result equals the reference expressions to f32 rounding (checked against the CPU oracle, which keeps the
+// Everything is templated on the arithmetic type R: float for the production kernels (f32, f16 and bf16 storage),
+// double for the FP64 twin that serves as the reference of the reduced-precision error statements.
+// The f32 result equals the reference expressions to f32 rounding (checked against the CPU oracle, which keeps the
 // generator's order, to relative L2 <= 1e-6 per step). Layout: populations [q][cell],
 // cell = (k*ny + j)*nx + i (js/live/kernel.js:17).
 #pragma once
@@ -44,14 +46,17 @@ struct Params {
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
 
 // equilibrium polynomial 1 + 3cu + 4.5cu^2 - 1.5u^2 (boundary reconstruction)
-__device__ __forceinline__ float epoly(float cu, float usq) { return 1.0f + cu * (3.0f + 4.5f * cu) - 1.5f * usq; }
+template <typename R>
+__device__ __forceinline__ R epoly(R cu, R usq) { return R(1) + cu * (R(3) + R(4.5) * cu) - R(1.5) * usq; }
 
 // c.u over the nonzero components
-template <int cx, int cy, int cz>
-__device__ __forceinline__ float cdot(float x, float y, float z) {
-  float s = 0.0f;
+template <int cx, int cy, int cz, typename R>
+__device__ __forceinline__ R cdot(R x, R y, R z) {
+  R s = R(0);
   if (cx) s = (cx > 0) ? s + x : s - x;
   if (cy) s = (cy > 0) ? s + y : s - y;
   if (cz) s = (cz > 0) ? s + z : s - z;
@@ -59,15 +64,17 @@ __device__ __forceinline__ float cdot(float x, float y, float z) {
 }
 
 // post-stream moments (js/live/kernel.js:264-277) from pair sums s_q = f_q + f_opp and differences d_q = f_q - f_opp
+template <typename R>
 struct Moments {
-  float rho, jx, jy, jz, th, Pxx, Pyy, Pzz, Pxy, Pxz, Pyz;
+  R rho, jx, jy, jz, th, Pxx, Pyy, Pzz, Pxy, Pxz, Pyz;
 };
 
-__device__ __forceinline__ Moments moments(const float (&f)[19], const float (&g)[7]) {
-  float s[19], d[19];
+template <typename R>
+__device__ __forceinline__ Moments<R> moments(const R (&f)[19], const R (&g)[7]) {
+  R s[19], d[19];
 #pragma unroll
   for (int q = 1; q < 19; q += 2) { s[q] = f[q] + f[q + 1]; d[q] = f[q] - f[q + 1]; }
-  Moments m;
+  Moments<R> m;
   m.rho = f[0] + ((s[1] + s[3]) + (s[5] + s[7])) + ((s[9] + s[11]) + (s[13] + s[15])) + s[17];
   m.jx = (d[1] + d[7]) + (d[9] + d[11]) + d[13];
   m.jy = (d[3] + d[7]) - d[9] + (d[15] + d[17]);
@@ -82,65 +89,68 @@ __device__ __forceinline__ Moments moments(const float (&f)[19], const float (&g
   return m;
 }
 
+template <typename R>
 struct Forces {
-  float Fx, Fy, Fz, sg, ux, uy, uz, inv_rho;
+  R Fx, Fy, Fz, sg, ux, uy, uz, inv_rho;
 };
 
 // buoyancy, limited log-law drag at the first cell of building-free columns, top sponge, force-corrected velocity
 // (js/live/kernel.js:279-306). DRAG / SPONGE = false drop branches the caller has proven inactive.
-template <bool DRAG, bool SPONGE>
-__device__ __forceinline__ Forces forces(const Params &P, int NZ, int k, int kg, int ktc, float cd, float uI, float vI,
-                                         const Moments &m) {
-  Forces o;
-  const float rho = m.rho, ir = rcp(rho);
+template <bool DRAG, bool SPONGE, typename R>
+__device__ __forceinline__ Forces<R> forces(const Params &P, int NZ, int k, int kg, int ktc, R cd, R uI, R vI,
+                                            const Moments<R> &m) {
+  Forces<R> o;
+  const R rho = m.rho, ir = rcp(rho);
   o.inv_rho = ir;
-  float Fx = 0.0f, Fy = 0.0f;
-  float Fz = rho * P.gbeta * m.th;
+  R Fx = R(0), Fy = R(0);
+  R Fz = rho * R(P.gbeta) * m.th;
   if (DRAG && k == kg && kg == ktc) {
-    const float ux0 = m.jx * ir, uy0 = m.jy * ir;
-    const float sp = fsqrt(ux0 * ux0 + uy0 * uy0);
-    const bool lim = cd * sp > 0.5f;                     // implicit limit: never reverse the flow within a step
-    const float c = lim ? -0.5f * rho : -rho * cd * sp;
+    const R ux0 = m.jx * ir, uy0 = m.jy * ir;
+    const R sp = fsqrt(ux0 * ux0 + uy0 * uy0);
+    const bool lim = cd * sp > R(0.5);                   // implicit limit: never reverse the flow within a step
+    const R c = lim ? R(-0.5) * rho : -rho * cd * sp;
     Fx = c * ux0;
     Fy = c * uy0;
   }
-  float sg = 0.0f;
+  R sg = R(0);
   if (SPONGE && k >= NZ - P.nsp) {
-    const float aa = ((float)(k - (NZ - P.nsp)) + 1.0f) / (float)P.nsp;
-    sg = P.sigmax * aa * aa;
-    const float rs = rho * sg;
+    const R aa = (R(k - (NZ - P.nsp)) + R(1)) / R(P.nsp);
+    sg = R(P.sigmax) * aa * aa;
+    const R rs = rho * sg;
     Fx += rs * (uI - m.jx * ir);
     Fy += rs * (vI - m.jy * ir);
     Fz -= rs * (m.jz * ir);
   }
   o.Fx = Fx; o.Fy = Fy; o.Fz = Fz; o.sg = sg;
-  o.ux = (m.jx + 0.5f * Fx) * ir;
-  o.uy = (m.jy + 0.5f * Fy) * ir;
-  o.uz = (m.jz + 0.5f * Fz) * ir;
+  o.ux = (m.jx + R(0.5) * Fx) * ir;
+  o.uy = (m.jy + R(0.5) * Fy) * ir;
+  o.uz = (m.jz + R(0.5) * Fz) * ir;
   return o;
 }
 
 // relaxation (js/live/kernel.js:308-338)
+template <typename R>
 struct Relax {
-  float om45, fp3, fp9, omg, Nxx, Nyy, Nzz, Nxy, Nxz, Nyz, C0;
+  R om45, fp3, fp9, omg, Nxx, Nyy, Nzz, Nxy, Nxz, Nyz, C0;
 };
 
-__device__ __forceinline__ Relax relax(const Params &P, int NX, int NY, int i, int j, float kb, const Moments &m,
-                                       const Forces &F) {
-  Relax r;
-  const float rho = m.rho, ux = F.ux, uy = F.uy, uz = F.uz;
-  const float third = (float)(1.0 / 3.0);
-  const float r3 = rho * third;
-  const float rux = rho * ux, ruy = rho * uy, ruz = rho * uz;
+template <typename R>
+__device__ __forceinline__ Relax<R> relax(const Params &P, int NX, int NY, int i, int j, R kb, const Moments<R> &m,
+                                          const Forces<R> &F) {
+  Relax<R> r;
+  const R rho = m.rho, ux = F.ux, uy = F.uy, uz = F.uz;
+  const R third = R(1.0 / 3.0);
+  const R r3 = rho * third;
+  const R rux = rho * ux, ruy = rho * uy, ruz = rho * uz;
   r.Nxx = m.Pxx - r3 - rux * ux;
   r.Nyy = m.Pyy - r3 - ruy * uy;
   r.Nzz = m.Pzz - r3 - ruz * uz;
   r.Nxy = m.Pxy - rux * uy;
   r.Nxz = m.Pxz - rux * uz;
   r.Nyz = m.Pyz - ruy * uz;
-  const float Pn = fsqrt(r.Nxx * r.Nxx + r.Nyy * r.Nyy + r.Nzz * r.Nzz +
-                         2.0f * (r.Nxy * r.Nxy + r.Nxz * r.Nxz + r.Nyz * r.Nyz));
-  float tb = P.tau0 + 3.0f * P.momfac * kb;
+  const R Pn = fsqrt(r.Nxx * r.Nxx + r.Nyy * r.Nyy + r.Nzz * r.Nzz +
+                     R(2) * (r.Nxy * r.Nxy + r.Nxz * r.Nxz + r.Nyz * r.Nyz));
+  R tb = R(P.tau0) + R(3) * R(P.momfac) * kb;
   // viscous sponge next to outflow faces
   int dmin = 1000;
   if (P.inflow_w == 0) dmin = min(dmin, i);
@@ -148,63 +158,66 @@ __device__ __forceinline__ Relax relax(const Params &P, int NX, int NY, int i, i
   if (P.inflow_s == 0) dmin = min(dmin, j);
   if (P.inflow_n == 0) dmin = min(dmin, NY - 1 - j);
   if (dmin < P.nsw) {
-    const float rr = ((float)P.nsw - (float)dmin) / (float)P.nsw;
-    tb += 0.15f * rr * rr;
+    const R rr = (R(P.nsw) - R(dmin)) / R(P.nsw);
+    tb += R(0.15f) * rr * rr;
   }
-  const float tau = 0.5f * (tb + fsqrt(tb * tb + P.cs2smag * Pn * F.inv_rho));
-  const float nusgs = (tau - tb) * third;
-  const float kap = nusgs / P.prt + kb + P.kfloor;      // P.prt is uniform: one scalar division per wave
-  const float itau = rcp(tau);
-  const float om = 1.0f - itau, fpre = 1.0f - 0.5f * itau;
-  r.omg = rcp(0.5f + 4.0f * kap);
-  r.om45 = 4.5f * om;
-  r.fp3 = 3.0f * fpre;
-  r.fp9 = 9.0f * fpre;
-  const float usq = ux * ux + uy * uy + uz * uz;
-  const float trN = (r.Nxx + r.Nyy + r.Nzz) * third;
-  const float uF = ux * F.Fx + uy * F.Fy + uz * F.Fz;
+  const R tau = R(0.5) * (tb + fsqrt(tb * tb + R(P.cs2smag) * Pn * F.inv_rho));
+  const R nusgs = (tau - tb) * third;
+  const R kap = nusgs / R(P.prt) + kb + R(P.kfloor);   // P.prt is uniform: one division per wave
+  const R itau = rcp(tau);
+  const R om = R(1) - itau, fpre = R(1) - R(0.5) * itau;
+  r.omg = rcp(R(0.5) + R(4) * kap);
+  r.om45 = R(4.5) * om;
+  r.fp3 = R(3) * fpre;
+  r.fp9 = R(9) * fpre;
+  const R usq = ux * ux + uy * uy + uz * uz;
+  const R trN = (r.Nxx + r.Nyy + r.Nzz) * third;
+  const R uF = ux * F.Fx + uy * F.Fy + uz * F.Fz;
   // the part of w_q^-1 f_q^post common to every q: rho (1 - 1.5 u^2) - 4.5 om trN - 3 fpre u.F
-  r.C0 = rho * (1.0f - 1.5f * usq) - r.om45 * trN - r.fp3 * uF;
+  r.C0 = rho * (R(1) - R(1.5) * usq) - r.om45 * trN - r.fp3 * uF;
   return r;
 }
 
 // regularised collision with Guo forcing of the pair (q, q+1), q odd (js/live/kernel.js:340-367):
 // f_q^post / w_q = C0 + 3 rho cu + 4.5 rho cu^2 + 4.5 om (c c):N + fpre (3 cF + 9 cu cF), split into the part
 // even in c (E) and the odd part (O); the opposite population gets E - O.
-template <int q>
-__device__ __forceinline__ void collide_pair(const Moments &m, const Forces &F, const Relax &R, float &fq, float &fo) {
+template <int q, typename T>
+__device__ __forceinline__ void collide_pair(const Moments<T> &m, const Forces<T> &F, const Relax<T> &R, T &fq, T &fo) {
   constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
-  const float cu = cdot<cx, cy, cz>(F.ux, F.uy, F.uz);
-  const float cF = cdot<cx, cy, cz>(F.Fx, F.Fy, F.Fz);
-  float qn = 0.0f;                                      // (c c):N
+  const T cu = cdot<cx, cy, cz>(F.ux, F.uy, F.uz);
+  const T cF = cdot<cx, cy, cz>(F.Fx, F.Fy, F.Fz);
+  T qn = T(0);                                          // (c c):N
   if (cx) qn += R.Nxx;
   if (cy) qn += R.Nyy;
   if (cz) qn += R.Nzz;
-  if (cx && cy) qn += (float)(2 * cx * cy) * R.Nxy;
-  if (cx && cz) qn += (float)(2 * cx * cz) * R.Nxz;
-  if (cy && cz) qn += (float)(2 * cy * cz) * R.Nyz;
-  const float E = R.C0 + cu * (4.5f * m.rho * cu + R.fp9 * cF) + R.om45 * qn;
-  const float O = 3.0f * m.rho * cu + R.fp3 * cF;
-  const float w = w19<q>();
+  if (cx && cy) qn += T(2 * cx * cy) * R.Nxy;
+  if (cx && cz) qn += T(2 * cx * cz) * R.Nxz;
+  if (cy && cz) qn += T(2 * cy * cz) * R.Nyz;
+  const T E = R.C0 + cu * (T(4.5) * m.rho * cu + R.fp9 * cF) + R.om45 * qn;
+  const T O = T(3) * m.rho * cu + R.fp3 * cF;
+  const T w = T(w19<q>());
   fq = w * (E + O);
   fo = w * (E - O);
 }
 
-__device__ __forceinline__ float collide_rest(const Relax &R) { return w19<0>() * R.C0; }
+template <typename T>
+__device__ __forceinline__ T collide_rest(const Relax<T> &R) { return T(w19<0>()) * R.C0; }
 
 // D3Q7 collision with the rooftop source and the top-sponge damping (js/live/kernel.js:369-386):
 // g^post = ((1 - omg) g + omg w th (1 + 4 c.u) + w S) (1 - sg)
+template <typename R>
 struct Scalar {
-  float a, b, cs0, cs1;   // a = (1-omg)(1-sg), b = omg w1 th (1-sg) (w1 = 1/8), source terms w S (1-sg)
+  R a, b, cs0, cs1;   // a = (1-omg)(1-sg), b = omg w1 th (1-sg) (w1 = 1/8), source terms w S (1-sg)
 };
 
-__device__ __forceinline__ Scalar scalar_coeffs(float th, float omg, float S, float sg) {
-  Scalar c;
-  const float ds = 1.0f - sg;
-  c.a = (1.0f - omg) * ds;
-  c.b = omg * 0.125f * th * ds;
-  c.cs0 = (omg * 0.25f * th + 0.25f * S) * ds;
-  c.cs1 = 0.125f * S * ds;
+template <typename R>
+__device__ __forceinline__ Scalar<R> scalar_coeffs(R th, R omg, R S, R sg) {
+  Scalar<R> c;
+  const R ds = R(1) - sg;
+  c.a = (R(1) - omg) * ds;
+  c.b = omg * R(0.125) * th * ds;
+  c.cs0 = (omg * R(0.25) * th + R(0.25) * S) * ds;
+  c.cs1 = R(0.125) * S * ds;
   return c;
 }
 

@@ -1,9 +1,10 @@
-// CDNA4 (gfx950) kernels of the coupled D3Q19 + D3Q7 lattice-Boltzmann step, f32 two-lattice pull scheme.
+// CDNA4 (gfx950) kernels of the coupled D3Q19 + D3Q7 lattice-Boltzmann step, two-lattice pull scheme, templated on
+// the storage format (storage.h: f32, f16 / bf16 deviations, f64 twin).
 //
 // Thread mapping: one thread per cell. The x-y plane is flattened into columns c = j*nx + i so that the 64 lanes of a
-// wavefront touch 64 consecutive cells of every [q][cell] stream (256 contiguous bytes; the stores and the
-// unshifted loads are 256-byte aligned whenever nx*ny is a multiple of 64). blockIdx.y is the level k, starting at the
-// lowest fluid level of the grid, so all-solid planes are never launched. Solid cells above that plane exit at once.
+// wavefront touch 64 consecutive cells of every [q][cell] stream (one contiguous access per stream per wavefront;
+// stores and unshifted loads are aligned whenever nx*ny is a multiple of 64). blockIdx.y is the level k, starting at
+// the lowest fluid level of the grid, so all-solid planes are never launched. Solid cells above that plane exit.
 //
 // Two streaming paths feed one collision:
 //   * fast path: interior cells (not on a lateral face, below the lid) at or above the column's kfast level, the first
@@ -13,27 +14,23 @@
 //   * boundary path: every branch of js/live/kernel.js:159-243 per population (lid first, then lateral faces, then the
 //     interior pull with half-way bounce-back from solid sources), with the 3 x 3 neighbour ktop values loaded once.
 #include "kernels.h"
+#include "storage.h"
 
 namespace aht {
 
+template <typename R>
 struct Outflow {
-  float rn, uxn, uyn, uzn, usqn, uxc, uyc, usqc;
+  R rn, uxn, uyn, uzn, usqn, uxc, uyc, usqc;
 };
 
-// raw buffer descriptor over a population array (gfx9 dword3); offsets are the 32-bit byte offsets of cells
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, -1, 0x00020000);
-}
-// the b32 builtins move raw bits: bit casts, never numeric conversions
-__device__ __forceinline__ float ldb(const float *base, uint32_t boff) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc(base), boff, 0, 0));
-}
-__device__ __forceinline__ void stb(float *base, uint32_t boff, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rsrc(base), boff, 0, 0);
-}
 // base address of population q shifted back by `off` cells (the pull offset), as a plain integer
-__device__ __forceinline__ const float *shifted(const float *a, size_t N, int q, int off) {
-  return (const float *)((uintptr_t)a + (uintptr_t)((int64_t)q * (int64_t)N - off) * 4u);
+template <class S>
+__device__ __forceinline__ const void *shifted(const void *a, size_t N, int q, int off) {
+  return (const void *)((uintptr_t)a + (uintptr_t)((int64_t)q * (int64_t)N - off) * (uintptr_t)S::bytes);
+}
+template <class S>
+__device__ __forceinline__ void *plane(void *a, size_t N, int q) {
+  return (void *)((uintptr_t)a + (uintptr_t)q * N * (uintptr_t)S::bytes);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -50,41 +47,69 @@ __device__ __forceinline__ void split_column(const Grid &g, int c, int &i, int &
   i = ii; j = jj;
 }
 
-// packed column word: kt | ktop << 10 | kfast << 20 (nz < 1024)
+// packed column word: kt | ktop << 10 | kfast << 20 (nz < 1023)
 __device__ __forceinline__ int col_kt(uint32_t x) { return (int)(x & 1023u); }
 __device__ __forceinline__ int col_ktop(uint32_t x) { return (int)((x >> 10) & 1023u); }
 __device__ __forceinline__ int col_kfast(uint32_t x) { return (int)(x >> 20); }
 
+// element access of population qq at cell index idx, decoded with weight w19<q> (deviation formats)
+template <class S, int q>
+__device__ __forceinline__ typename S::real ldf(const typename S::elem *__restrict__ f, size_t N, int qq, uint32_t idx) {
+  return S::dec_f(f[(size_t)qq * N + idx], w19<q>());
+}
+template <class S>
+__device__ __forceinline__ typename S::real ldg(const typename S::elem *__restrict__ g, size_t N, int qq, uint32_t idx) {
+  return S::dec_g(g[(size_t)qq * N + idx]);
+}
+
+template <int q>
+struct OwnPops {
+  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&a)[19], const typename S::elem *__restrict__ fA,
+                                             size_t N, uint32_t n) {
+    a[q] = ldf<S, q>(fA, N, q, n);
+    OwnPops<q + 1>::template run<S>(a, fA, N, n);
+  }
+};
+template <>
+struct OwnPops<19> {
+  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&)[19], const typename S::elem *__restrict__, size_t,
+                                             uint32_t) {}
+};
+
 // outflow-face prelude from the own cell's pre-stream populations (js/live/kernel.js:122-148)
-__device__ __forceinline__ Outflow outflow_prelude(const Grid &g, const float *__restrict__ fA, size_t N, uint32_t n,
-                                                   int i, int j) {
+template <class S>
+__device__ __forceinline__ Outflow<typename S::real> outflow_prelude(const Grid &g,
+                                                                     const typename S::elem *__restrict__ fA, size_t N,
+                                                                     uint32_t n, int i, int j) {
+  using R = typename S::real;
   const Params &P = g.P;
-  Outflow o;
-  o.rn = 1.0f; o.uxn = 0.0f; o.uyn = 0.0f; o.uzn = 0.0f;
+  Outflow<R> o;
+  o.rn = R(1); o.uxn = R(0); o.uyn = R(0); o.uzn = R(0);
   const bool onb = (i == 0 && P.inflow_w == 0) || (i == g.nx - 1 && P.inflow_e == 0) ||
                    (j == 0 && P.inflow_s == 0) || (j == g.ny - 1 && P.inflow_n == 0);
   if (onb) {
-    float a[19];
-#pragma unroll
-    for (int q = 0; q < 19; q++) a[q] = fA[(size_t)q * N + n];
-    float r = a[0], sx = 0.f, sy = 0.f, sz = 0.f;
+    R a[19];
+    OwnPops<0>::template run<S>(a, fA, N, n);
+    R r = a[0], sx = R(0), sy = R(0), sz = R(0);
 #pragma unroll
     for (int q = 1; q < 19; q++) {
       r += a[q];
-      sx += (float)CX[q] * a[q];
-      sy += (float)CY[q] * a[q];
-      sz += (float)CZ[q] * a[q];
+      sx += R(CX[q]) * a[q];
+      sy += R(CY[q]) * a[q];
+      sz += R(CZ[q]) * a[q];
     }
-    const float ir = rcp(r);
+    const R ir = rcp(r);
     o.rn = r; o.uxn = sx * ir; o.uyn = sy * ir; o.uzn = sz * ir;
   }
   o.usqn = o.uxn * o.uxn + o.uyn * o.uyn + o.uzn * o.uzn;
-  float uxc = o.uxn, uyc = o.uyn;
+  R uxc = o.uxn, uyc = o.uyn;
   if (P.oclip != 0) {                       // backflow-clipped outlet: no inward normal velocity
-    if (i == 0 && P.inflow_w == 0 && uxc > 0.0f) uxc = 0.0f;
-    if (i == g.nx - 1 && P.inflow_e == 0 && uxc < 0.0f) uxc = 0.0f;
-    if (j == 0 && P.inflow_s == 0 && uyc > 0.0f) uyc = 0.0f;
-    if (j == g.ny - 1 && P.inflow_n == 0 && uyc < 0.0f) uyc = 0.0f;
+    if (i == 0 && P.inflow_w == 0 && uxc > R(0)) uxc = R(0);
+    if (i == g.nx - 1 && P.inflow_e == 0 && uxc < R(0)) uxc = R(0);
+    if (j == 0 && P.inflow_s == 0 && uyc > R(0)) uyc = R(0);
+    if (j == g.ny - 1 && P.inflow_n == 0 && uyc < R(0)) uyc = R(0);
   }
   o.uxc = uxc; o.uyc = uyc;
   o.usqc = uxc * uxc + uyc * uyc + o.uzn * o.uzn;
@@ -92,9 +117,12 @@ __device__ __forceinline__ Outflow outflow_prelude(const Grid &g, const float *_
 }
 
 // post-stream population q of cell (i, j, k), every boundary branch (js/live/kernel.js:159-217)
-template <int q>
-__device__ __forceinline__ float pull19(const Grid &g, const float *__restrict__ fA, size_t N, uint32_t n, int i, int j,
-                                        int k, const int (&kn)[3][3], float uI, float vI, const Outflow &ob) {
+template <class S, int q>
+__device__ __forceinline__ typename S::real pull19(const Grid &g, const typename S::elem *__restrict__ fA, size_t N,
+                                                   uint32_t n, int i, int j, int k, const int (&kn)[3][3],
+                                                   typename S::real uI, typename S::real vI,
+                                                   const Outflow<typename S::real> &ob) {
+  using R = typename S::real;
   constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
   const Params &P = g.P;
   const int NX = g.nx, NY = g.ny;
@@ -102,117 +130,131 @@ __device__ __forceinline__ float pull19(const Grid &g, const float *__restrict__
     // free-slip lid: the mirrored population of the lateral neighbour at the same level
     constexpr int m = MIRZ[q];
     const int si = i - cx, sj = j - cy;
-    if (si < 0 || si >= NX || sj < 0 || sj >= NY) return fA[(size_t)m * N + n];
-    if (k < kn[1 - cx][1 - cy]) return fA[(size_t)m * N + n];
-    return fA[(size_t)m * N + (n - (uint32_t)(cx + cy * NX))];
+    if (si < 0 || si >= NX || sj < 0 || sj >= NY) return ldf<S, q>(fA, N, m, n);
+    if (k < kn[1 - cx][1 - cy]) return ldf<S, q>(fA, N, m, n);
+    return ldf<S, q>(fA, N, m, n - (uint32_t)(cx + cy * NX));
   }
   const bool outx = (cx == 1 && i == 0) || (cx == -1 && i == NX - 1);
   const bool outy = (cy == 1 && j == 0) || (cy == -1 && j == NY - 1);
   if (outx || outy) {
     const bool inflow = (cx == 1 && i == 0 && P.inflow_w != 0) || (cx == -1 && i == NX - 1 && P.inflow_e != 0) ||
                         (cy == 1 && j == 0 && P.inflow_s != 0) || (cy == -1 && j == NY - 1 && P.inflow_n != 0);
-    const float w = w19<q>();
-    if (inflow) return w * epoly(cdot<cx, cy, 0>(uI, vI, 0.0f), uI * uI + vI * vI);   // equilibrium, rho = 1
+    const R w = R(w19<q>());
+    if (inflow) return w * epoly(cdot<cx, cy, 0>(uI, vI, R(0)), uI * uI + vI * vI);   // equilibrium, rho = 1
     // pressure outlet: non-equilibrium extrapolation towards rho = 1, optionally backflow-clipped
-    const float cu = cdot<cx, cy, cz>(ob.uxn, ob.uyn, ob.uzn);
-    const float own = fA[(size_t)q * N + n];
+    const R cu = cdot<cx, cy, cz>(ob.uxn, ob.uyn, ob.uzn);
+    const R own = ldf<S, q>(fA, N, q, n);
     if (P.oclip != 0) {
-      const float cuc = cdot<cx, cy, cz>(ob.uxc, ob.uyc, ob.uzn);
+      const R cuc = cdot<cx, cy, cz>(ob.uxc, ob.uyc, ob.uzn);
       return own + w * (epoly(cuc, ob.usqc) - ob.rn * epoly(cu, ob.usqn));
     }
-    return own + w * (1.0f - ob.rn) * epoly(cu, ob.usqn);
+    return own + w * (R(1) - ob.rn) * epoly(cu, ob.usqn);
   }
   // interior: half-way bounce-back when the source cell is solid
   const int sk = k - cz;
-  if (sk < kn[1 - cx][1 - cy]) return fA[(size_t)OPP[q] * N + n];
+  if (sk < kn[1 - cx][1 - cy]) return ldf<S, q>(fA, N, OPP[q], n);
   const int off = cx + cy * NX + cz * (int)g.NXY;
-  return fA[(size_t)q * N + (uint32_t)((int)n - off)];
+  return ldf<S, q>(fA, N, q, (uint32_t)((int)n - off));
 }
 
 // post-stream theta population q (js/live/kernel.js:219-243)
-template <int q>
-__device__ __forceinline__ float pull7(const Grid &g, const float *__restrict__ gA, size_t N, uint32_t n, int i, int j,
-                                       int k, const int (&kn)[3][3]) {
+template <class S, int q>
+__device__ __forceinline__ typename S::real pull7(const Grid &g, const typename S::elem *__restrict__ gA, size_t N,
+                                                  uint32_t n, int i, int j, int k, const int (&kn)[3][3]) {
+  using R = typename S::real;
   constexpr int cx = CX7[q], cy = CY7[q], cz = CZ7[q];
   const Params &P = g.P;
-  if (cz == -1 && k == g.nz - 1) return gA[(size_t)q * N + n];   // zero-gradient top
+  if (cz == -1 && k == g.nz - 1) return ldg<S>(gA, N, q, n);   // zero-gradient top
   const bool outx = (cx == 1 && i == 0) || (cx == -1 && i == g.nx - 1);
   const bool outy = (cy == 1 && j == 0) || (cy == -1 && j == g.ny - 1);
   if (outx || outy) {
     const bool inflow = (cx == 1 && i == 0 && P.inflow_w != 0) || (cx == -1 && i == g.nx - 1 && P.inflow_e != 0) ||
                         (cy == 1 && j == 0 && P.inflow_s != 0) || (cy == -1 && j == g.ny - 1 && P.inflow_n != 0);
-    return inflow ? 0.0f : gA[(size_t)q * N + n];
+    return inflow ? R(0) : ldg<S>(gA, N, q, n);
   }
   const int sk = k - cz;
-  if (sk < kn[1 - cx][1 - cy]) return gA[(size_t)OPP7[q] * N + n];
+  if (sk < kn[1 - cx][1 - cy]) return ldg<S>(gA, N, OPP7[q], n);
   const int off = cx + cy * g.nx + cz * (int)g.NXY;
-  return gA[(size_t)q * N + (uint32_t)((int)n - off)];
+  return ldg<S>(gA, N, q, (uint32_t)((int)n - off));
 }
 
 template <int q>
 struct Pull19 {
-  __device__ __forceinline__ static void run(float (&f)[19], const Grid &g, const float *__restrict__ fA, size_t N,
-                                             uint32_t n, int i, int j, int k, const int (&kn)[3][3], float uI, float vI,
-                                             const Outflow &ob) {
-    f[q] = pull19<q>(g, fA, N, n, i, j, k, kn, uI, vI, ob);
-    Pull19<q + 1>::run(f, g, fA, N, n, i, j, k, kn, uI, vI, ob);
+  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&f)[19], const Grid &g,
+                                             const typename S::elem *__restrict__ fA, size_t N, uint32_t n, int i,
+                                             int j, int k, const int (&kn)[3][3], typename S::real uI,
+                                             typename S::real vI, const Outflow<typename S::real> &ob) {
+    f[q] = pull19<S, q>(g, fA, N, n, i, j, k, kn, uI, vI, ob);
+    Pull19<q + 1>::template run<S>(f, g, fA, N, n, i, j, k, kn, uI, vI, ob);
   }
 };
 template <>
 struct Pull19<19> {
-  __device__ __forceinline__ static void run(float (&)[19], const Grid &, const float *__restrict__, size_t, uint32_t,
-                                             int, int, int, const int (&)[3][3], float, float, const Outflow &) {}
+  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&)[19], const Grid &,
+                                             const typename S::elem *__restrict__, size_t, uint32_t, int, int, int,
+                                             const int (&)[3][3], typename S::real, typename S::real,
+                                             const Outflow<typename S::real> &) {}
 };
 
 template <int q>
 struct Pull7 {
-  __device__ __forceinline__ static void run(float (&gg)[7], const Grid &g, const float *__restrict__ gA, size_t N,
-                                             uint32_t n, int i, int j, int k, const int (&kn)[3][3]) {
-    gg[q] = pull7<q>(g, gA, N, n, i, j, k, kn);
-    Pull7<q + 1>::run(gg, g, gA, N, n, i, j, k, kn);
+  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&gg)[7], const Grid &g,
+                                             const typename S::elem *__restrict__ gA, size_t N, uint32_t n, int i,
+                                             int j, int k, const int (&kn)[3][3]) {
+    gg[q] = pull7<S, q>(g, gA, N, n, i, j, k, kn);
+    Pull7<q + 1>::template run<S>(gg, g, gA, N, n, i, j, k, kn);
   }
 };
 template <>
 struct Pull7<7> {
-  __device__ __forceinline__ static void run(float (&)[7], const Grid &, const float *__restrict__, size_t, uint32_t,
-                                             int, int, int, const int (&)[3][3]) {}
+  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&)[7], const Grid &,
+                                             const typename S::elem *__restrict__, size_t, uint32_t, int, int, int,
+                                             const int (&)[3][3]) {}
 };
 
 // fast path: all 26 sources are fluid interior cells; plain shifted loads through per-population descriptors
 template <int q>
 struct FastPull {
-  __device__ __forceinline__ static void run(float (&f)[19], float (&gg)[7], const Grid &g,
-                                             const float *__restrict__ fA, const float *__restrict__ gA, uint32_t boff) {
+  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
+                                             const void *fA, const void *gA, uint32_t boff) {
     const int off = CX[q] + CY[q] * g.nx + CZ[q] * (int)g.NXY;
-    f[q] = ldb(shifted(fA, g.N, q, off), boff);
-    if (q < 7) {
-      const int o7 = CX7[q < 7 ? q : 0] + CY7[q < 7 ? q : 0] * g.nx + CZ7[q < 7 ? q : 0] * (int)g.NXY;
-      gg[q < 7 ? q : 0] = ldb(shifted(gA, g.N, q, o7), boff);
+    f[q] = S::dec_f(S::ld_raw(shifted<S>(fA, g.N, q, off), boff), w19<q>());
+    if constexpr (q < 7) {
+      const int o7 = CX7[q] + CY7[q] * g.nx + CZ7[q] * (int)g.NXY;
+      gg[q] = S::dec_g(S::ld_raw(shifted<S>(gA, g.N, q, o7), boff));
     }
-    FastPull<q + 1>::run(f, gg, g, fA, gA, boff);
+    FastPull<q + 1>::template run<S>(f, gg, g, fA, gA, boff);
   }
 };
 template <>
 struct FastPull<19> {
-  __device__ __forceinline__ static void run(float (&)[19], float (&)[7], const Grid &, const float *__restrict__,
-                                             const float *__restrict__, uint32_t) {}
+  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&)[19], typename S::real (&)[7], const Grid &,
+                                             const void *, const void *, uint32_t) {}
 };
 
 template <int q>
 struct StorePairs {
-  __device__ __forceinline__ static void run(float *__restrict__ fB, size_t N, uint32_t boff, const Moments &m,
-                                             const Forces &F, const Relax &R) {
-    float a, b;
+  template <class S>
+  __device__ __forceinline__ static void run(void *fB, size_t N, uint32_t boff, const Moments<typename S::real> &m,
+                                             const Forces<typename S::real> &F, const Relax<typename S::real> &R) {
+    typename S::real a, b;
     collide_pair<q>(m, F, R, a, b);
-    stb(fB + (size_t)q * N, boff, a);
-    stb(fB + (size_t)(q + 1) * N, boff, b);
-    StorePairs<q + 2>::run(fB, N, boff, m, F, R);
+    S::st_raw(plane<S>(fB, N, q), boff, S::enc_f(a, w19<q>()));
+    S::st_raw(plane<S>(fB, N, q + 1), boff, S::enc_f(b, w19<q + 1>()));
+    StorePairs<q + 2>::template run<S>(fB, N, boff, m, F, R);
   }
 };
 template <>
 struct StorePairs<19> {
-  __device__ __forceinline__ static void run(float *__restrict__, size_t, uint32_t, const Moments &, const Forces &,
-                                             const Relax &) {}
+  template <class S>
+  __device__ __forceinline__ static void run(void *, size_t, uint32_t, const Moments<typename S::real> &,
+                                             const Forces<typename S::real> &, const Relax<typename S::real> &) {}
 };
 
 __device__ __forceinline__ void load_neighbours(const Grid &g, int i, int j, int kg, int (&kn)[3][3]) {
@@ -224,26 +266,28 @@ __device__ __forceinline__ void load_neighbours(const Grid &g, int i, int j, int
     for (int b = 0; b < 3; b++) kn[a][b] = (a == 1 && b == 1) ? kg : (int)g.ktop[ys[b] * g.nx + xs[a]];
 }
 
-// streaming + moments + forces of one fluid cell: the fast or the boundary path
-__device__ __forceinline__ void stream_cell(const Grid &g, const float *__restrict__ fA, const float *__restrict__ gA,
-                                            uint32_t n, int i, int j, int k, int kg, int ktc, bool fast, float uI,
-                                            float vI, float (&f)[19], float (&gg)[7]) {
+// streaming of one fluid cell: the fast or the boundary path
+template <class S>
+__device__ __forceinline__ void stream_cell(const Grid &g, const void *fA, const void *gA, uint32_t n, int i, int j,
+                                            int k, int kg, bool fast, typename S::real uI, typename S::real vI,
+                                            typename S::real (&f)[19], typename S::real (&gg)[7]) {
+  using E = typename S::elem;
   if (fast) {
-    FastPull<0>::run(f, gg, g, fA, gA, n * 4u);
+    FastPull<0>::template run<S>(f, gg, g, fA, gA, n * (uint32_t)S::bytes);
   } else {
     int kn[3][3];
     load_neighbours(g, i, j, kg, kn);
-    const Outflow ob = outflow_prelude(g, fA, g.N, n, i, j);
-    Pull19<0>::run(f, g, fA, g.N, n, i, j, k, kn, uI, vI, ob);
-    Pull7<0>::run(gg, g, gA, g.N, n, i, j, k, kn);
+    const Outflow<typename S::real> ob = outflow_prelude<S>(g, (const E *)fA, g.N, n, i, j);
+    Pull19<0>::template run<S>(f, g, (const E *)fA, g.N, n, i, j, k, kn, uI, vI, ob);
+    Pull7<0>::template run<S>(gg, g, (const E *)gA, g.N, n, i, j, k, kn);
   }
 }
 
 // the fused step (genStepWGSL, js/live/kernel.js:395-431) with FP64 accumulation (ACC) and debug outputs (OUT)
-template <bool ACC, bool OUT>
-__global__ __launch_bounds__(256) void step_f32(Grid g, const float *__restrict__ fA, float *__restrict__ fB,
-                                                const float *__restrict__ gA, float *__restrict__ gB, Acc acc,
-                                                float *__restrict__ out4) {
+template <class S, bool ACC, bool OUT>
+__global__ __launch_bounds__(256) void step_kernel(Grid g, const void *fA, void *fB, const void *gA, void *gB, Acc acc,
+                                                   float *__restrict__ out4) {
+  using R = typename S::real;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = g.kmin + blockIdx.y;
   const bool live = c < (int)g.NXY;
@@ -259,30 +303,30 @@ __global__ __launch_bounds__(256) void step_f32(Grid g, const float *__restrict_
     split_column(g, c, i, j);
     const float4 pr = g.prof[max(k - ktc, 0)];
     const bool fast = k >= col_kfast(cv.x) && k < g.nz - 1 && i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
-    float f[19], gg[7];
-    stream_cell(g, fA, gA, n, i, j, k, kg, ktc, fast, pr.x, pr.y, f, gg);
-    const Moments m = moments(f, gg);
-    const Forces F = forces<true, true>(g.P, g.nz, k, kg, ktc, __uint_as_float(cv.z), pr.x, pr.y, m);
-    const Relax R = relax(g.P, g.nx, g.ny, i, j, pr.z, m, F);
-    const uint32_t boff = n * 4u;
-    stb(fB, boff, collide_rest(R));
-    StorePairs<1>::run(fB, N, boff, m, F, R);
+    R f[19], gg[7];
+    stream_cell<S>(g, fA, gA, n, i, j, k, kg, fast, R(pr.x), R(pr.y), f, gg);
+    const Moments<R> m = moments(f, gg);
+    const Forces<R> F = forces<true, true>(g.P, g.nz, k, kg, ktc, R(__uint_as_float(cv.z)), R(pr.x), R(pr.y), m);
+    const Relax<R> Rx = relax(g.P, g.nx, g.ny, i, j, R(pr.z), m, F);
+    const uint32_t boff = n * (uint32_t)S::bytes;
+    S::st_raw(fB, boff, S::enc_f(collide_rest(Rx), w19<0>()));
+    StorePairs<1>::template run<S>(fB, N, boff, m, F, Rx);
     // D3Q7: g^post = a g + b (1 + 4 c.u) + w S (1 - sg)
-    const float S = (k == kg) ? __uint_as_float(cv.w) * g.P.srcscale : 0.0f;
-    const Scalar sc = scalar_coeffs(m.th, R.omg, S, F.sg);
-    const float b1 = sc.b + sc.cs1, b4 = 4.0f * sc.b;
-    stb(gB, boff, sc.a * gg[0] + sc.cs0);
-    stb(gB + N, boff, sc.a * gg[1] + (b1 + b4 * F.ux));
-    stb(gB + 2 * N, boff, sc.a * gg[2] + (b1 - b4 * F.ux));
-    stb(gB + 3 * N, boff, sc.a * gg[3] + (b1 + b4 * F.uy));
-    stb(gB + 4 * N, boff, sc.a * gg[4] + (b1 - b4 * F.uy));
-    stb(gB + 5 * N, boff, sc.a * gg[5] + (b1 + b4 * F.uz));
-    stb(gB + 6 * N, boff, sc.a * gg[6] + (b1 - b4 * F.uz));
+    const R Sq = (k == kg) ? R(__uint_as_float(cv.w)) * R(g.P.srcscale) : R(0);
+    const Scalar<R> sc = scalar_coeffs(m.th, Rx.omg, Sq, F.sg);
+    const R b1 = sc.b + sc.cs1, b4 = R(4) * sc.b;
+    S::st_raw(plane<S>(gB, N, 0), boff, S::enc_g(sc.a * gg[0] + sc.cs0));
+    S::st_raw(plane<S>(gB, N, 1), boff, S::enc_g(sc.a * gg[1] + (b1 + b4 * F.ux)));
+    S::st_raw(plane<S>(gB, N, 2), boff, S::enc_g(sc.a * gg[2] + (b1 - b4 * F.ux)));
+    S::st_raw(plane<S>(gB, N, 3), boff, S::enc_g(sc.a * gg[3] + (b1 + b4 * F.uy)));
+    S::st_raw(plane<S>(gB, N, 4), boff, S::enc_g(sc.a * gg[4] + (b1 - b4 * F.uy)));
+    S::st_raw(plane<S>(gB, N, 5), boff, S::enc_g(sc.a * gg[5] + (b1 + b4 * F.uz)));
+    S::st_raw(plane<S>(gB, N, 6), boff, S::enc_g(sc.a * gg[6] + (b1 - b4 * F.uz)));
     if (OUT) {
-      out4[n] = m.th;
-      out4[N + n] = F.ux;
-      out4[2 * N + n] = F.uy;
-      out4[3 * N + n] = F.uz;
+      out4[n] = (float)m.th;
+      out4[N + n] = (float)F.ux;
+      out4[2 * N + n] = (float)F.uy;
+      out4[3 * N + n] = (float)F.uz;
     }
     if (ACC) {
       th_cell = (double)m.th;
@@ -326,8 +370,10 @@ __global__ __launch_bounds__(256) void step_f32(Grid g, const float *__restrict_
 }
 
 // near-surface read (genSurfWGSL, js/live/kernel.js:435-471): out3 = [3][NXY] north-up theta, u*conv, v*conv
-__global__ __launch_bounds__(256) void surface_f32(Grid g, const float *__restrict__ fA, const float *__restrict__ gA,
-                                                   float conv, float *__restrict__ out3) {
+template <class S>
+__global__ __launch_bounds__(256) void surface_kernel(Grid g, const void *fA, const void *gA, float conv,
+                                                      float *__restrict__ out3) {
+  using R = typename S::real;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= (int)g.NXY) return;
   int i, j;
@@ -339,49 +385,49 @@ __global__ __launch_bounds__(256) void surface_f32(Grid g, const float *__restri
     out3[o] = 0.0f; out3[g.NXY + o] = 0.0f; out3[2 * g.NXY + o] = 0.0f;
     return;
   }
-  const size_t N = g.N;
   const uint32_t n = (uint32_t)k * g.NXY + (uint32_t)c;
-  const int zagl = max(k - ktc, 0);
-  const float4 pr = g.prof[zagl];
-  int kn[3][3];
-  load_neighbours(g, i, j, kg, kn);
-  const Outflow ob = outflow_prelude(g, fA, N, n, i, j);
-  float f[19], gg[7];
-  Pull19<0>::run(f, g, fA, N, n, i, j, k, kn, pr.x, pr.y, ob);
-  Pull7<0>::run(gg, g, gA, N, n, i, j, k, kn);
-  const Moments m = moments(f, gg);
-  const Forces F = forces<true, true>(g.P, g.nz, k, kg, ktc, __uint_as_float(cv.z), pr.x, pr.y, m);
-  out3[o] = m.th;
-  out3[g.NXY + o] = F.ux * conv;
-  out3[2 * g.NXY + o] = F.uy * conv;
+  const float4 pr = g.prof[max(k - ktc, 0)];
+  R f[19], gg[7];
+  stream_cell<S>(g, fA, gA, n, i, j, k, kg, false, R(pr.x), R(pr.y), f, gg);
+  const Moments<R> m = moments(f, gg);
+  const Forces<R> F = forces<true, true>(g.P, g.nz, k, kg, ktc, R(__uint_as_float(cv.z)), R(pr.x), R(pr.y), m);
+  out3[o] = (float)m.th;
+  out3[g.NXY + o] = (float)(F.ux * R(conv));
+  out3[2 * g.NXY + o] = (float)(F.uy * R(conv));
 }
 
-// equilibrium of the inflow profile at the local height above terrain, both lattices (genInitWGSL :524-553)
-// evaluated operation by operation as the reference (no contraction), so the state is bit-identical to the closed form
-__global__ __launch_bounds__(256) void init_f32(Grid g, float *__restrict__ f0, float *__restrict__ f1) {
+// equilibrium of the inflow profile at the local height above terrain, both lattices (genInitWGSL :524-553),
+// evaluated operation by operation as the reference (no contraction): the f32 state is bit-identical to the closed form
+template <class S>
+__global__ __launch_bounds__(256) void init_kernel(Grid g, void *f0, void *f1) {
 #pragma clang fp contract(off)
+  using R = typename S::real;
+  using E = typename S::elem;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.y;
   if (c >= (int)g.NXY) return;
   const int ktc = col_kt(g.colv[c].x);
   const int zagl = min(max(k - ktc, 0), g.nz - 1);
   const float4 pr = g.prof[zagl];
-  const float U = pr.x, V = pr.y, usq = U * U + V * V;
+  const R U = R(pr.x), V = R(pr.y), usq = U * U + V * V;
   const size_t N = g.N;
   const uint32_t n = (uint32_t)k * g.NXY + (uint32_t)c;
 #pragma unroll
   for (int q = 0; q < 19; q++) {
-    const float cu = (float)CX[q] * U + (float)CY[q] * V;
-    const float w = q == 0 ? (float)(1.0 / 3.0) : (q < 7 ? (float)(1.0 / 18.0) : (float)(1.0 / 36.0));
-    const float fq = w * (((1.0f + 3.0f * cu) + 4.5f * cu * cu) - 1.5f * usq);
-    f0[(size_t)q * N + n] = fq;
-    f1[(size_t)q * N + n] = fq;
+    const R cu = R(CX[q]) * U + R(CY[q]) * V;
+    const float wf = q == 0 ? (float)(1.0 / 3.0) : (q < 7 ? (float)(1.0 / 18.0) : (float)(1.0 / 36.0));
+    const R fq = R(wf) * (((R(1) + R(3) * cu) + R(4.5) * cu * cu) - R(1.5) * usq);
+    const E e = S::enc_f(fq, wf);
+    ((E *)f0)[(size_t)q * N + n] = e;
+    ((E *)f1)[(size_t)q * N + n] = e;
   }
 }
 
 // cells that turned from solid to fluid: rest state f = w_q, g = 0 (genReinitWGSL :557-575)
-__global__ __launch_bounds__(256) void reinit_f32(Grid g, const uint32_t *__restrict__ oldk, float *__restrict__ f,
-                                                  float *__restrict__ gq) {
+template <class S>
+__global__ __launch_bounds__(256) void reinit_kernel(Grid g, const uint32_t *__restrict__ oldk, void *f, void *gq) {
+  using R = typename S::real;
+  using E = typename S::elem;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= (int)g.NXY) return;
   const int k1 = min((int)oldk[c], g.nz);
@@ -389,16 +435,20 @@ __global__ __launch_bounds__(256) void reinit_f32(Grid g, const uint32_t *__rest
   for (int k = (int)g.ktop[c]; k < k1; k++) {
     const uint32_t n = (uint32_t)k * g.NXY + (uint32_t)c;
 #pragma unroll
-    for (int q = 0; q < 19; q++)
-      f[(size_t)q * N + n] = q == 0 ? (float)(1.0 / 3.0) : (q < 7 ? (float)(1.0 / 18.0) : (float)(1.0 / 36.0));
+    for (int q = 0; q < 19; q++) {
+      const float wf = q == 0 ? (float)(1.0 / 3.0) : (q < 7 ? (float)(1.0 / 18.0) : (float)(1.0 / 36.0));
+      ((E *)f)[(size_t)q * N + n] = S::enc_f(R(wf), wf);
+    }
 #pragma unroll
-    for (int q = 0; q < 7; q++) gq[(size_t)q * N + n] = 0.0f;
+    for (int q = 0; q < 7; q++) ((E *)gq)[(size_t)q * N + n] = S::enc_g(R(0));
   }
 }
 
 // u8 theta volume, north-up [k][row][col], 4 cells per u32, and max theta (genVolWGSL :475-520)
-__global__ __launch_bounds__(256) void volume_f32(Grid g, const float *__restrict__ gA, int nzv, float volscale,
-                                                  uint32_t *__restrict__ vout, uint32_t *__restrict__ tmax_bits) {
+template <class S>
+__global__ __launch_bounds__(256) void volume_kernel(Grid g, const void *gA, int nzv, float volscale,
+                                                     uint32_t *__restrict__ vout, uint32_t *__restrict__ tmax_bits) {
+  using R = typename S::real;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int nvol = nzv * (int)g.NXY;
   const int nw = (nvol + 3) / 4;
@@ -416,9 +466,10 @@ __global__ __launch_bounds__(256) void volume_f32(Grid g, const float *__restric
         if (k >= kg) {
           int kn[3][3];
           load_neighbours(g, i, j, kg, kn);
-          float gg[7];
-          Pull7<0>::run(gg, g, gA, g.N, (uint32_t)k * g.NXY + (uint32_t)c, i, j, k, kn);
-          th = gg[0] + gg[1] + gg[2] + gg[3] + gg[4] + gg[5] + gg[6];
+          R gg[7];
+          Pull7<0>::template run<S>(gg, g, (const typename S::elem *)gA, g.N, (uint32_t)k * g.NXY + (uint32_t)c, i, j,
+                                    k, kn);
+          th = (float)(gg[0] + gg[1] + gg[2] + gg[3] + gg[4] + gg[5] + gg[6]);
         }
         const uint32_t qv = (uint32_t)fminf(fmaxf(rintf(th * volscale), 0.0f), 255.0f);
         packed |= qv << (8u * (uint32_t)e);
@@ -440,41 +491,39 @@ __global__ __launch_bounds__(256) void volume_f32(Grid g, const float *__restric
 
 // diagnostics over the stored state (genStatsWGSL :580-633), FP64 per-block partials:
 // part[8*b + (0..6)] = sum theta, sum ke, sum (rho-1), fluid count, max theta, max |u|, non-finite count
-__global__ __launch_bounds__(256) void stats_f32(Grid g, const float *__restrict__ fA, const float *__restrict__ gA,
-                                                 double *__restrict__ part) {
+template <class S>
+__global__ __launch_bounds__(256) void stats_kernel(Grid g, const void *fA, const void *gA, double *__restrict__ part) {
+  using R = typename S::real;
+  using E = typename S::elem;
   const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
   double v[7] = {0, 0, 0, 0, 0, 0, 0};
   if (n < g.N) {
     const uint32_t c = n % g.NXY, k = n / g.NXY;
     if ((int)k >= (int)g.ktop[c]) {
       const size_t N = g.N;
-      float a[19];
+      R a[19];
+      OwnPops<0>::template run<S>(a, (const E *)fA, N, n);
+      R rho = a[0], jx = R(0), jy = R(0), jz = R(0);
 #pragma unroll
-      for (int q = 0; q < 19; q++) a[q] = fA[(size_t)q * N + n];
-      float rho = a[0];
-#pragma unroll
-      for (int q = 1; q < 19; q++) rho = rho + a[q];
-      float jx = 0.f, jy = 0.f, jz = 0.f;
-#pragma unroll
-      for (int q = 0; q < 19; q++) {
-        if (CX[q] == 1) jx = jx + a[q]; else if (CX[q] == -1) jx = jx - a[q];
-        if (CY[q] == 1) jy = jy + a[q]; else if (CY[q] == -1) jy = jy - a[q];
-        if (CZ[q] == 1) jz = jz + a[q]; else if (CZ[q] == -1) jz = jz - a[q];
+      for (int q = 1; q < 19; q++) {
+        rho += a[q];
+        jx += R(CX[q]) * a[q];
+        jy += R(CY[q]) * a[q];
+        jz += R(CZ[q]) * a[q];
       }
-      float th = gA[n];
+      R th = R(0);
 #pragma unroll
-      for (int q = 1; q < 7; q++) th = th + gA[(size_t)q * N + n];
-      const uint32_t br = __float_as_uint(rho) & 0x7f800000u, bt = __float_as_uint(th) & 0x7f800000u;
-      if (br == 0x7f800000u || bt == 0x7f800000u) {
+      for (int q = 0; q < 7; q++) th += ldg<S>((const E *)gA, N, q, n);
+      if (!isfinite((double)rho) || !isfinite((double)th)) {
         v[6] = 1.0;
       } else {
-        const float jj = jx * jx + jy * jy + jz * jz;
-        v[0] = th;
-        v[1] = 0.5f * jj / rho;
-        v[2] = rho - 1.0f;
+        const double jj = (double)jx * jx + (double)jy * jy + (double)jz * jz;
+        v[0] = (double)th;
+        v[1] = 0.5 * jj / (double)rho;
+        v[2] = (double)rho - 1.0;
         v[3] = 1.0;
-        v[4] = fmaxf(th, 0.0f);
-        v[5] = sqrtf(jj) / rho;
+        v[4] = fmax((double)th, 0.0);
+        v[5] = sqrt(jj) / (double)rho;
       }
     }
   }
@@ -535,6 +584,24 @@ __global__ __launch_bounds__(256) void sum_f64(const double *__restrict__ x, uin
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// storage <-> f32 / f64 conversion of one population plane (read_state / write_state); w = rest weight (f) or 0 (g)
+template <class S, typename T>
+__global__ __launch_bounds__(256) void decode_plane(const void *src, T *__restrict__ dst, uint32_t n, float w, int is_f) {
+  using E = typename S::elem;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const E e = ((const E *)src)[t];
+  dst[t] = (T)(is_f ? S::dec_f(e, w) : S::dec_g(e));
+}
+template <class S, typename T>
+__global__ __launch_bounds__(256) void encode_plane(const T *__restrict__ src, void *dst, uint32_t n, float w, int is_f) {
+  using E = typename S::elem;
+  using R = typename S::real;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  ((E *)dst)[t] = is_f ? S::enc_f(R(src[t]), w) : S::enc_g(R(src[t]));
+}
+
 // ---- launchers ------------------------------------------------------------------------------------------------
 size_t face_len(int face, int nx, int ny, int nz) { return face < 2 ? 2 * (size_t)nz * ny : 2 * (size_t)nz * nx; }
 size_t face_offset(int face, int nx, int ny, int nz) {
@@ -548,52 +615,87 @@ uint32_t content_slots(const Grid &g, int block) {
   return bx * (uint32_t)(g.nz - g.kmin) * (uint32_t)(block / 64);
 }
 
-hipError_t launch_step(const Grid &g, const float *fA, float *fB, const float *gA, float *gB, const Acc *acc,
+int storage_bytes(int storage) { return storage == 0 ? 4 : (storage == 3 ? 8 : 2); }
+
+#define AHT_DISPATCH(storage, CALL)                   \
+  switch (storage) {                                  \
+    case 0: { using S = StoreF32; CALL; } break;       \
+    case 1: { using S = StoreF16; CALL; } break;       \
+    case 2: { using S = StoreBF16; CALL; } break;      \
+    case 3: { using S = StoreF64; CALL; } break;       \
+    default: return hipErrorInvalidValue;             \
+  }
+
+hipError_t launch_step(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB, const Acc *acc,
                        float *out4, int block, hipStream_t st) {
   if (g.kmin >= g.nz) return hipSuccess;
   const dim3 grid((g.NXY + block - 1) / block, g.nz - g.kmin);
   const Acc a = acc ? *acc : Acc{nullptr, nullptr, nullptr};
-  if (acc && out4) hipLaunchKernelGGL((step_f32<true, true>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4);
-  else if (acc) hipLaunchKernelGGL((step_f32<true, false>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4);
-  else if (out4) hipLaunchKernelGGL((step_f32<false, true>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4);
-  else hipLaunchKernelGGL((step_f32<false, false>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4);
+  AHT_DISPATCH(storage, {
+    if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4);
+    else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4);
+    else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4);
+    else hipLaunchKernelGGL((step_kernel<S, false, false>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4);
+  })
   return hipGetLastError();
 }
 
-hipError_t launch_init(const Grid &g, float *f0, float *f1, hipStream_t st) {
+hipError_t launch_init(int storage, const Grid &g, void *f0, void *f1, hipStream_t st) {
   const dim3 grid((g.NXY + 255) / 256, g.nz);
-  hipLaunchKernelGGL(init_f32, grid, dim3(256), 0, st, g, f0, f1);
+  AHT_DISPATCH(storage, hipLaunchKernelGGL(init_kernel<S>, grid, dim3(256), 0, st, g, f0, f1))
   return hipGetLastError();
 }
 
-hipError_t launch_reinit(const Grid &g, const uint32_t *old_ktop, float *f, float *gq, hipStream_t st) {
-  hipLaunchKernelGGL(reinit_f32, dim3((g.NXY + 255) / 256), dim3(256), 0, st, g, old_ktop, f, gq);
+hipError_t launch_reinit(int storage, const Grid &g, const uint32_t *old_ktop, void *f, void *gq, hipStream_t st) {
+  AHT_DISPATCH(storage, hipLaunchKernelGGL(reinit_kernel<S>, dim3((g.NXY + 255) / 256), dim3(256), 0, st, g, old_ktop, f, gq))
   return hipGetLastError();
 }
 
-hipError_t launch_surface(const Grid &g, const float *fA, const float *gA, float conv, float *out3, hipStream_t st) {
-  hipLaunchKernelGGL(surface_f32, dim3((g.NXY + 255) / 256), dim3(256), 0, st, g, fA, gA, conv, out3);
+hipError_t launch_surface(int storage, const Grid &g, const void *fA, const void *gA, float conv, float *out3,
+                          hipStream_t st) {
+  AHT_DISPATCH(storage, hipLaunchKernelGGL(surface_kernel<S>, dim3((g.NXY + 255) / 256), dim3(256), 0, st, g, fA, gA, conv, out3))
   return hipGetLastError();
 }
 
-hipError_t launch_volume(const Grid &g, const float *gA, int nzv, float volscale, uint32_t *vout, uint32_t *tmax_bits,
-                         hipStream_t st) {
+hipError_t launch_volume(int storage, const Grid &g, const void *gA, int nzv, float volscale, uint32_t *vout,
+                         uint32_t *tmax_bits, hipStream_t st) {
   const uint32_t nw = ((uint32_t)nzv * g.NXY + 3) / 4;
-  hipLaunchKernelGGL(volume_f32, dim3((nw + 255) / 256), dim3(256), 0, st, g, gA, nzv, volscale, vout, tmax_bits);
+  AHT_DISPATCH(storage, hipLaunchKernelGGL(volume_kernel<S>, dim3((nw + 255) / 256), dim3(256), 0, st, g, gA, nzv, volscale, vout, tmax_bits))
   return hipGetLastError();
 }
 
 uint32_t stats_partials(const Grid &g) { return (g.N + 255) / 256; }
 
-hipError_t launch_stats(const Grid &g, const float *fA, const float *gA, double *part, double *out8, hipStream_t st) {
+hipError_t launch_stats(int storage, const Grid &g, const void *fA, const void *gA, double *part, double *out8,
+                        hipStream_t st) {
   const uint32_t nb = stats_partials(g);
-  hipLaunchKernelGGL(stats_f32, dim3(nb), dim3(256), 0, st, g, fA, gA, part);
+  AHT_DISPATCH(storage, hipLaunchKernelGGL(stats_kernel<S>, dim3(nb), dim3(256), 0, st, g, fA, gA, part))
   hipLaunchKernelGGL(stats_finish, dim3(1), dim3(256), 0, st, part, nb, out8);
   return hipGetLastError();
 }
 
 hipError_t launch_sum(const double *x, uint32_t n, double *out, hipStream_t st) {
   hipLaunchKernelGGL(sum_f64, dim3(1), dim3(256), 0, st, x, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_decode(int storage, const void *src, void *dst, int dst_f64, uint32_t n, float w, int is_f,
+                         hipStream_t st) {
+  const dim3 grid((n + 255) / 256);
+  AHT_DISPATCH(storage, {
+    if (dst_f64) hipLaunchKernelGGL((decode_plane<S, double>), grid, dim3(256), 0, st, src, (double *)dst, n, w, is_f);
+    else hipLaunchKernelGGL((decode_plane<S, float>), grid, dim3(256), 0, st, src, (float *)dst, n, w, is_f);
+  })
+  return hipGetLastError();
+}
+
+hipError_t launch_encode(int storage, const void *src, int src_f64, void *dst, uint32_t n, float w, int is_f,
+                         hipStream_t st) {
+  const dim3 grid((n + 255) / 256);
+  AHT_DISPATCH(storage, {
+    if (src_f64) hipLaunchKernelGGL((encode_plane<S, double>), grid, dim3(256), 0, st, (const double *)src, dst, n, w, is_f);
+    else hipLaunchKernelGGL((encode_plane<S, float>), grid, dim3(256), 0, st, (const float *)src, dst, n, w, is_f);
+  })
   return hipGetLastError();
 }
 

@@ -28,7 +28,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, 'ashburn-heat-twin_amd')
 METRIC = 'MLUPS & % HBM roofline at 600×600×64; scenarios/hour + p50/p99 latency'
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL = {0: 208, 1: 104, 2: 104}
+BYTES_PER_CELL = {0: 208, 1: 104, 2: 104, 3: 416}
+STORAGE = ['f32', 'f16', 'bf16', 'f64']
 # published reference number for this metric and grid: numba.cuda prototype step at 600x600x64, 10 m, RTX 4090
 # (README.md:37 of the reference, data/manifest.json:9468)
 REFERENCE_MLUPS_600 = 950.0
@@ -88,7 +89,7 @@ def main():
     ap.add_argument('--steps', type=int, default=200)
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--workload', default='c2')
-    ap.add_argument('--storage', type=int, default=0, help='0 f32, 1 f16 deviations, 2 bf16 deviations')
+    ap.add_argument('--storage', type=int, default=0, help='0 f32, 1 f16 deviations, 2 bf16 deviations, 3 f64 twin')
     ap.add_argument('--block', type=int, default=0)
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
     ap.add_argument('--no-cpu', action='store_true')
@@ -127,7 +128,7 @@ def main():
     achieved = bpc * nf / (ms_step * 1e-3) / 1e9
     log(f'{ms_step:.4f} ms/step (events), wall {wall * 1e3 / a.steps:.4f} ms/step, {mlups:.1f} MLUPS, '
         f'{achieved:.1f} GB/s, nonfinite {st["non_finite"]}')
-    kernel_tag = f'step_{["f32", "f16", "bf16"][a.storage]}_{a.workload}'
+    kernel_tag = f'step_{STORAGE[a.storage]}_{a.workload}'
     traffic = traffic_from_profiles(kernel_tag)
     cpu = None
     if not a.no_cpu:
@@ -141,11 +142,11 @@ def main():
         'warmup': a.warmup, 'ms_per_step': round(ms_step, 5), 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': round(mlups / REFERENCE_MLUPS_600, 3) if (nx, ny, nz) == (600, 600, 64) else None,
         'baseline_note': 'vs the reference numba.cuda prototype, 950 MLUPS at 600x600x64 on an RTX 4090',
-        'dtype': ['f32', 'f16 storage / f32 arithmetic', 'bf16 storage / f32 arithmetic'][a.storage],
+        'dtype': ['f32', 'f16 storage / f32 arithmetic', 'bf16 storage / f32 arithmetic', 'f64'][a.storage],
         'data': 'synthetic (seeded generator: hills, box buildings, source roofs)',
         'config': {'workload': f'{a.workload}: {nx}x{ny}x{nz} at {dom["dx"]} m, fused D3Q19+D3Q7 step',
                    'cells': N, 'fluid_cells': nf, 'bytes_per_fluid_cell': bpc, 'dt_s': dom['dt'],
-                   'storage': ['f32', 'f16', 'bf16'][a.storage], 'parallelism': 'single GPU'},
+                   'storage': STORAGE[a.storage], 'parallelism': 'single GPU'},
         'fluid_mlups': round(nf * a.steps / (ms * 1e3), 2),
         'wall_ms_per_step': round(wall * 1e3 / a.steps, 5),
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',

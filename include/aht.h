@@ -32,8 +32,9 @@ extern "C" {
 
 /* population storage formats */
 #define AHT_STORE_F32 0   /* f32 populations (208 B per fluid-cell update)                            */
-#define AHT_STORE_F16 1   /* f16 deviations f - w*rho0, theta_q (104 B per update), f32 arithmetic    */
+#define AHT_STORE_F16 1   /* f16 deviations (f_q - w_q)*2^10, theta_q*2^10 (104 B per update), f32 math */
 #define AHT_STORE_BF16 2  /* bf16 deviations, same layout as F16                                      */
+#define AHT_STORE_F64 3   /* f64 populations and f64 arithmetic: the FP64 twin (416 B per update)     */
 
 /* aht_step flags */
 #define AHT_STEP_ACC 1u   /* accumulate the FP64 time means (surface theta/u/v, face fluxes, heat content)  */
@@ -124,9 +125,12 @@ int aht_read_mean(aht_solver *s, double *surf3, aht_budget *budget);
 int aht_read_face_mean(aht_solver *s, int32_t face, double *out);
 
 /* raw state at the current parity, f32 [19][N] and [7][N] (replaces _dbg.readState/writeState,
- * js/live/solver.js:540-547); 16-bit storages convert on the device; f or g may be NULL */
+ * js/live/solver.js:540-547); the other storages convert on the device; f or g may be NULL */
 int aht_read_state(aht_solver *s, float *f, float *g);
 int aht_write_state(aht_solver *s, const float *f, const float *g);
+/* the same in f64 host format (exact for AHT_STORE_F64; no counterpart in the reference, whose storage is f32 only) */
+int aht_read_state_f64(aht_solver *s, double *f, double *g);
+int aht_write_state_f64(aht_solver *s, const double *f, const double *g);
 
 /* device bytes held by the solver (admission by footprint in the serving loop) */
 int64_t aht_footprint_bytes(const aht_dims *dims);
