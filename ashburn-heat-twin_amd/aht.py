@@ -86,6 +86,18 @@ SIGNATURES = {
     'aht_read_state_f64': (ctypes.c_int, [_vp, _vp, _vp]),
     'aht_write_state_f64': (ctypes.c_int, [_vp, _vp, _vp]),
     'aht_footprint_bytes': (_i64, [ctypes.POINTER(Dims)]),
+    'aht_batch_create': (ctypes.c_int, [_i32, _i32, ctypes.POINTER(_vp)]),
+    'aht_batch_destroy': (None, [_vp]),
+    'aht_batch_last_error': (ctypes.c_char_p, [_vp]),
+    'aht_batch_add': (ctypes.c_int, [_vp, _vp]),
+    'aht_batch_remove': (ctypes.c_int, [_vp, _vp]),
+    'aht_batch_size': (ctypes.c_int, [_vp]),
+    'aht_batch_step': (ctypes.c_int, [_vp, _i32, _vp, _vp, ctypes.POINTER(ctypes.c_double)]),
+    'aht_batch_sync': (ctypes.c_int, [_vp]),
+    'aht_batch_set_grid': (ctypes.c_int, [_vp, _i32, _i32]),
+    'aht_batch_grid': (ctypes.c_int, [_vp]),
+    'aht_autotune': (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_i32)]),
+    'aht_set_block': (ctypes.c_int, [_vp, _i32]),
 }
 
 _lib = None
@@ -212,6 +224,14 @@ class Solver:
     def steps(self):
         return int(lib().aht_steps_done(self.h))
 
+    def autotune(self, steps=20):
+        b = ctypes.c_int32()
+        self._check(lib().aht_autotune(self.h, int(steps), ctypes.byref(b)))
+        return b.value
+
+    def set_block(self, block):
+        self._check(lib().aht_set_block(self.h, int(block)))
+
     def benchmark(self, n):
         ms = ctypes.c_double()
         self._check(lib().aht_benchmark(self.h, int(n), ctypes.byref(ms)))
@@ -274,6 +294,71 @@ class Solver:
         fa = None if f is None else np.ascontiguousarray(f, np.float64)
         ga = None if g is None else np.ascontiguousarray(g, np.float64)
         self._check(lib().aht_write_state_f64(self.h, _ptr(fa), _ptr(ga)))
+
+
+class Batch:
+    """Many solvers advanced together: one persistent launch per step, chunks captured as hipGraphs."""
+
+    def __init__(self, storage=STORE_F32, device=0):
+        storage = STORAGE_NAMES.get(storage, storage)
+        h = ctypes.c_void_p()
+        rc = lib().aht_batch_create(device, storage, ctypes.byref(h))
+        self.h = h
+        if rc != AHT_OK:
+            msg = lib().aht_batch_last_error(h).decode() if h.value else lib().aht_error_string(rc).decode()
+            if h.value:
+                lib().aht_batch_destroy(h)
+            self.h = None
+            raise AhtError(rc, msg)
+        self.members = []
+
+    def _check(self, rc):
+        if rc != AHT_OK:
+            raise AhtError(rc, lib().aht_batch_last_error(self.h).decode())
+
+    def add(self, s):
+        self._check(lib().aht_batch_add(self.h, s.h))
+        self.members.append(s)
+
+    def remove(self, s):
+        self._check(lib().aht_batch_remove(self.h, s.h))
+        self.members.remove(s)
+
+    def step(self, n, nsteps=None, acc_from=None, timed=False):
+        """one chunk of n launches; nsteps / acc_from per member in add order (None: all n / no means)"""
+        ns = None if nsteps is None else np.ascontiguousarray(nsteps, np.int32)
+        af = None if acc_from is None else np.ascontiguousarray(acc_from, np.int32)
+        ms = ctypes.c_double()
+        self._check(lib().aht_batch_step(self.h, int(n), _ptr(ns), _ptr(af), ctypes.byref(ms) if timed else None))
+        return ms.value if timed else None
+
+    def sync(self):
+        self._check(lib().aht_batch_sync(self.h))
+
+    def set_grid(self, blocks, use_graph=True):
+        self._check(lib().aht_batch_set_grid(self.h, int(blocks), 1 if use_graph else 0))
+
+    @property
+    def grid(self):
+        return int(lib().aht_batch_grid(self.h))
+
+    def close(self):
+        if self.h:
+            lib().aht_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def budget_physical(b, dx, dt, conv):

@@ -425,6 +425,167 @@ static napi_value js_steps_done(napi_env env, napi_callback_info info) {
   return r;
 }
 
+/* ---- batches -------------------------------------------------------------------------------------------------- */
+typedef struct {
+  aht_batch *b;
+} bhandle_t;
+
+static void finalize_batch(napi_env env, void *data, void *hint) {
+  (void)env; (void)hint;
+  bhandle_t *h = (bhandle_t *)data;
+  if (h->b) aht_batch_destroy(h->b);
+  free(h);
+}
+
+static bhandle_t *get_bhandle(napi_env env, napi_value v) {
+  void *p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p || !((bhandle_t *)p)->b) {
+    napi_throw_error(env, NULL, "aht addon: invalid or destroyed batch handle");
+    return NULL;
+  }
+  return (bhandle_t *)p;
+}
+
+static napi_value throw_brc(napi_env env, aht_batch *b, int rc) {
+  char buf[600];
+  snprintf(buf, sizeof buf, "aht batch: %s (%s)", b ? aht_batch_last_error(b) : "", aht_error_string(rc));
+  napi_throw_error(env, NULL, buf);
+  return NULL;
+}
+
+/* batchCreate(device, storage) -> handle */
+static napi_value js_batch_create(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  int32_t dev, stg;
+  if (!get_i32(env, argv[0], &dev) || !get_i32(env, argv[1], &stg)) return NULL;
+  aht_batch *b = NULL;
+  int rc = aht_batch_create(dev, stg, &b);
+  if (rc) {
+    napi_value r = throw_brc(env, b, rc);
+    if (b) aht_batch_destroy(b);
+    return r;
+  }
+  bhandle_t *h = (bhandle_t *)calloc(1, sizeof *h);
+  h->b = b;
+  napi_value ext;
+  CHECK_NAPI(napi_create_external(env, h, finalize_batch, NULL, &ext));
+  return ext;
+}
+
+static napi_value js_batch_destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  void *p = NULL;
+  napi_get_value_external(env, argv[0], &p);
+  bhandle_t *h = (bhandle_t *)p;
+  if (h && h->b) { aht_batch_destroy(h->b); h->b = NULL; }
+  return undef(env);
+}
+
+/* batchAdd(b, solver), batchRemove(b, solver) */
+static napi_value js_batch_member(napi_env env, napi_callback_info info, int add) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  bhandle_t *b = get_bhandle(env, argv[0]);
+  if (!b) return NULL;
+  handle_t *h = get_handle(env, argv[1]);
+  if (!h) return NULL;
+  int rc = add ? aht_batch_add(b->b, h->s) : aht_batch_remove(b->b, h->s);
+  return rc ? throw_brc(env, b->b, rc) : undef(env);
+}
+static napi_value js_batch_add(napi_env env, napi_callback_info info) { return js_batch_member(env, info, 1); }
+static napi_value js_batch_remove(napi_env env, napi_callback_info info) { return js_batch_member(env, info, 0); }
+
+/* batchSetGrid(b, blocks, useGraph), batchGrid(b) -> blocks */
+static napi_value js_batch_set_grid(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  bhandle_t *b = get_bhandle(env, argv[0]);
+  if (!b) return NULL;
+  int32_t g, ug;
+  if (!get_i32(env, argv[1], &g) || !get_i32(env, argv[2], &ug)) return NULL;
+  int rc = aht_batch_set_grid(b->b, g, ug);
+  return rc ? throw_brc(env, b->b, rc) : undef(env);
+}
+static napi_value js_batch_grid(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  bhandle_t *b = get_bhandle(env, argv[0]);
+  if (!b) return NULL;
+  napi_value r;
+  napi_create_int32(env, aht_batch_grid(b->b), &r);
+  return r;
+}
+
+typedef struct {
+  napi_async_work work;
+  napi_deferred deferred;
+  napi_ref keep;
+  bhandle_t *b;
+  int32_t n;
+  int32_t *ns, *af;
+  double ms;
+  int rc;
+  char err[512];
+} bstep_job;
+
+static void bstep_execute(napi_env env, void *data) {
+  (void)env;
+  bstep_job *j = (bstep_job *)data;
+  j->rc = aht_batch_step(j->b->b, j->n, j->ns, j->af, &j->ms);
+  if (j->rc == AHT_OK) j->rc = aht_batch_sync(j->b->b);
+  if (j->rc) snprintf(j->err, sizeof j->err, "aht batch: %s (%s)", aht_batch_last_error(j->b->b), aht_error_string(j->rc));
+}
+
+static void bstep_complete(napi_env env, napi_status status, void *data) {
+  bstep_job *j = (bstep_job *)data;
+  if (status != napi_ok || j->rc) {
+    napi_value msg, e;
+    napi_create_string_utf8(env, j->rc ? j->err : "aht batch: step cancelled", NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, NULL, msg, &e);
+    napi_reject_deferred(env, j->deferred, e);
+  } else {
+    napi_value v;
+    napi_create_double(env, j->ms, &v);
+    napi_resolve_deferred(env, j->deferred, v);
+  }
+  napi_delete_reference(env, j->keep);
+  napi_delete_async_work(env, j->work);
+  free(j->ns);
+  free(j->af);
+  free(j);
+}
+
+/* batchStep(b, n, nsteps Int32Array|null, accFrom Int32Array|null) -> Promise<ms> (the arrays are copied) */
+static napi_value js_batch_step(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return NULL;
+  bhandle_t *b = get_bhandle(env, argv[0]);
+  if (!b) return NULL;
+  int32_t n;
+  if (!get_i32(env, argv[1], &n)) return NULL;
+  void *pn, *pa;
+  size_t bn, ba;
+  if (!get_buf(env, argv[2], &pn, &bn) || !get_buf(env, argv[3], &pa, &ba)) return NULL;
+  const size_t m = (size_t)aht_batch_size(b->b);
+  if ((pn && bn < m * 4) || (pa && ba < m * 4)) {
+    napi_throw_range_error(env, NULL, "aht batch: nsteps/accFrom need one entry per member");
+    return NULL;
+  }
+  bstep_job *j = (bstep_job *)calloc(1, sizeof *j);
+  j->b = b; j->n = n;
+  if (pn) { j->ns = (int32_t *)malloc(m * 4 + 4); memcpy(j->ns, pn, m * 4); }
+  if (pa) { j->af = (int32_t *)malloc(m * 4 + 4); memcpy(j->af, pa, m * 4); }
+  napi_value promise, name;
+  CHECK_NAPI(napi_create_promise(env, &j->deferred, &promise));
+  CHECK_NAPI(napi_create_reference(env, argv[0], 1, &j->keep));
+  CHECK_NAPI(napi_create_string_utf8(env, "aht_batch_step", NAPI_AUTO_LENGTH, &name));
+  CHECK_NAPI(napi_create_async_work(env, NULL, name, bstep_execute, bstep_complete, j, &j->work));
+  CHECK_NAPI(napi_queue_async_work(env, j->work));
+  return promise;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback cb; } fns[] = {
     {"create", js_create}, {"destroy", js_destroy}, {"setColumns", js_set_columns}, {"setParams", js_set_params},
@@ -433,6 +594,9 @@ static napi_value init(napi_env env, napi_value exports) {
     {"readVolume", js_read_volume}, {"stats", js_stats}, {"meanReset", js_mean_reset}, {"readMean", js_read_mean},
     {"readState", js_read_state}, {"writeState", js_write_state}, {"readOut", js_read_out},
     {"footprint", js_footprint}, {"stepsDone", js_steps_done},
+    {"batchCreate", js_batch_create}, {"batchDestroy", js_batch_destroy}, {"batchAdd", js_batch_add},
+    {"batchRemove", js_batch_remove}, {"batchSetGrid", js_batch_set_grid}, {"batchGrid", js_batch_grid},
+    {"batchStep", js_batch_step},
   };
   for (size_t k = 0; k < sizeof fns / sizeof fns[0]; k++) {
     napi_value f;

@@ -14,6 +14,8 @@
 #include "../../include/aht.h"
 #include "kernels.h"
 
+struct aht_batch;
+
 struct aht_solver {
   aht_dims dims{};
   int device = 0;
@@ -25,7 +27,8 @@ struct aht_solver {
   int eb = 4;                 // bytes per stored population element
   void *f[2] = {nullptr, nullptr}, *gq[2] = {nullptr, nullptr};
   void *stage = nullptr;      // one population plane in f64, for state conversion
-  uint4 *colv = nullptr;
+  uint32_t *colk = nullptr;
+  float2 *colcq = nullptr;
   uint32_t *ktop = nullptr, *oldk = nullptr;
   float4 *prof = nullptr;
   double *acc_surf = nullptr, *acc_face = nullptr, *acc_content = nullptr, *red_out = nullptr;
@@ -35,6 +38,11 @@ struct aht_solver {
   uint32_t *vol = nullptr, *tmax = nullptr;
   int parity = 0;
   bool has_cols = false, has_params = false, has_out = false;
+  // ordering against a batch's stream: the batch records `after_batch` after each chunk, the solver records `own`
+  // before the batch steps it
+  hipEvent_t after_batch = nullptr, own = nullptr;
+  bool batch_pending = false;
+  aht_batch *batch = nullptr;
   int64_t steps = 0, n_acc = 0;
   std::vector<float> h_q;
   std::string err;
@@ -76,6 +84,10 @@ int check_ready(aht_solver *s) {
 
 int set_device(aht_solver *s) {
   HIP_TRY(s, hipSetDevice(s->device));
+  if (s->batch_pending) {                 // work the solver enqueues next comes after the batch chunk that stepped it
+    HIP_TRY(s, hipStreamWaitEvent(s->st, s->after_batch, 0));
+    s->batch_pending = false;
+  }
   return AHT_OK;
 }
 
@@ -101,7 +113,7 @@ int64_t aht_footprint_bytes(const aht_dims *d) {
   const int64_t N = (int64_t)d->nx * d->ny * d->nz, NXY = (int64_t)d->nx * d->ny;
   const int eb = d->storage == AHT_STORE_F32 ? 4 : (d->storage == AHT_STORE_F64 ? 8 : 2);
   const int64_t pop = 2 * 26 * N * eb;
-  const int64_t cols = NXY * (16 + 4 + 4) + (int64_t)d->nz * 16;
+  const int64_t cols = NXY * (4 + 8 + 4 + 4) + (int64_t)d->nz * 16;
   const int64_t acc = 3 * NXY * 8 + (int64_t)face_total(d->nx, d->ny, d->nz) * 8 + (int64_t)max_content_slots(*d, 256) * 8;
   const int64_t misc = ((N + 255) / 256) * 64 + 3 * NXY * 4 + (N + 3) / 4 * 4 / 1 + 4096;
   return pop + cols + acc + misc;
@@ -135,6 +147,8 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   HIP_TRY(s, hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
   HIP_TRY(s, hipEventCreate(&s->ev0));
   HIP_TRY(s, hipEventCreate(&s->ev1));
+  HIP_TRY(s, hipEventCreateWithFlags(&s->after_batch, hipEventDisableTiming));
+  HIP_TRY(s, hipEventCreateWithFlags(&s->own, hipEventDisableTiming));
   const uint32_t NXY = (uint32_t)d.nx * d.ny;
   aht::Grid &g = s->g;
   g.nx = d.nx; g.ny = d.ny; g.nz = d.nz; g.kmin = d.nz; g.NXY = NXY; g.N = (uint32_t)N;
@@ -145,7 +159,8 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
     HIP_TRY(s, hipMemsetAsync(s->f[p], 0, 19 * N * s->eb, s->st));
     HIP_TRY(s, hipMemsetAsync(s->gq[p], 0, 7 * N * s->eb, s->st));
   }
-  HIP_TRY(s, hipMalloc(&s->colv, NXY * sizeof(uint4)));
+  HIP_TRY(s, hipMalloc(&s->colk, NXY * sizeof(uint32_t)));
+  HIP_TRY(s, hipMalloc(&s->colcq, NXY * sizeof(float2)));
   HIP_TRY(s, hipMalloc(&s->ktop, NXY * sizeof(uint32_t)));
   HIP_TRY(s, hipMalloc(&s->oldk, NXY * sizeof(uint32_t)));
   HIP_TRY(s, hipMalloc(&s->prof, d.nz * sizeof(float4)));
@@ -158,7 +173,7 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   HIP_TRY(s, hipMalloc(&s->surf3, 3 * (size_t)NXY * sizeof(float)));
   HIP_TRY(s, hipMalloc(&s->vol, ((N + 3) / 4) * sizeof(uint32_t)));
   HIP_TRY(s, hipMalloc(&s->tmax, sizeof(uint32_t)));
-  s->g.colv = s->colv; s->g.ktop = s->ktop; s->g.prof = s->prof;
+  s->g.colk = s->colk; s->g.colcq = s->colcq; s->g.ktop = s->ktop; s->g.prof = s->prof;
   rc = aht_mean_reset(s);
   if (rc) return rc;
   HIP_TRY(s, hipStreamSynchronize(s->st));
@@ -167,16 +182,19 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
 
 void aht_destroy(aht_solver *s) {
   if (!s) return;
+  if (s->batch) aht_batch_remove(s->batch, s);     // waits for the batch's last chunk
   if (s->st) {
     (void)hipSetDevice(s->device);
     (void)hipStreamSynchronize(s->st);
   }
-  void *bufs[] = {s->f[0], s->f[1], s->gq[0], s->gq[1], s->stage, s->colv, s->ktop, s->oldk, s->prof, s->acc_surf, s->acc_face,
+  void *bufs[] = {s->f[0], s->f[1], s->gq[0], s->gq[1], s->stage, s->colk, s->colcq, s->ktop, s->oldk, s->prof, s->acc_surf, s->acc_face,
                   s->acc_content, s->red_out, s->stat_part, s->out4, s->surf3, s->vol, s->tmax};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->after_batch) (void)hipEventDestroy(s->after_batch);
+  if (s->own) (void)hipEventDestroy(s->own);
   if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
 }
@@ -186,7 +204,8 @@ int aht_set_columns(aht_solver *s, const uint32_t *kt, const uint32_t *ktop, con
   const uint32_t NXY = s->g.NXY;
   const int nz = s->g.nz;
   const int nx = s->g.nx, ny = s->g.ny;
-  std::vector<uint4> cv(NXY);
+  std::vector<uint32_t> ckv(NXY);
+  std::vector<float2> cqv(NXY);
   int kmin = nz;
   for (uint32_t c = 0; c < NXY; c++) {
     if (kt[c] < 1 || kt[c] > ktop[c] || (int)ktop[c] > nz)
@@ -202,15 +221,14 @@ int aht_set_columns(aht_solver *s, const uint32_t *kt, const uint32_t *ktop, con
       for (int b = std::max(j - 1, 0); b <= std::min(j + 1, ny - 1); b++)
         for (int a = std::max(i - 1, 0); a <= std::min(i + 1, nx - 1); a++) m = std::max(m, ktop[(size_t)b * nx + a]);
       const uint32_t kfast = std::min<uint32_t>(m + 1, 1023u);
-      uint32_t cdb, qb;
-      memcpy(&cdb, &cd[c], 4);
-      memcpy(&qb, &q[c], 4);
-      cv[c] = make_uint4(kt[c] | (ktop[c] << 10) | (kfast << 20), 0u, cdb, qb);
+      ckv[c] = kt[c] | (ktop[c] << 10) | (kfast << 20);
+      cqv[c] = make_float2(cd[c], q[c]);
     }
   }
   int rc = set_device(s);
   if (rc) return rc;
-  HIP_TRY(s, hipMemcpyAsync(s->colv, cv.data(), NXY * sizeof(uint4), hipMemcpyHostToDevice, s->st));
+  HIP_TRY(s, hipMemcpyAsync(s->colk, ckv.data(), NXY * sizeof(uint32_t), hipMemcpyHostToDevice, s->st));
+  HIP_TRY(s, hipMemcpyAsync(s->colcq, cqv.data(), NXY * sizeof(float2), hipMemcpyHostToDevice, s->st));
   HIP_TRY(s, hipMemcpyAsync(s->ktop, ktop, NXY * sizeof(uint32_t), hipMemcpyHostToDevice, s->st));
   HIP_TRY(s, hipStreamSynchronize(s->st));
   s->g.kmin = kmin;
@@ -491,6 +509,291 @@ int aht_write_state(aht_solver *s, const float *f, const float *g) {
 int aht_read_state_f64(aht_solver *s, double *f, double *g) { return state_io(s, f, g, 1, 0); }
 int aht_write_state_f64(aht_solver *s, const double *f, const double *g) {
   return state_io(s, (void *)f, (void *)g, 1, 1);
+}
+
+// ---- batches ------------------------------------------------------------------------------------------------------
+struct aht_batch {
+  int device = 0, storage = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<aht_solver *> mem;
+  aht::Member *dmem = nullptr;        // device table
+  size_t dcap = 0;
+  std::vector<aht::Member> hmem;
+  uint32_t total = 0;
+  int version = 0;                    // bumps on membership change
+  int grid = 0;                       // persistent blocks; 0 = tune online
+  bool use_graph = true;
+  hipGraphExec_t gexec = nullptr;
+  int g_version = -1, g_n = -1, g_grid = -1;
+  // online tuner: candidate grids timed on real chunks, best kept per membership version
+  std::vector<int> cand;
+  std::vector<double> cand_ms;
+  int tune_version = -1, tune_next = 0;
+  int64_t steps = 0;
+  std::string err;
+};
+
+namespace {
+int bfail(aht_batch *b, int code, const char *fmt, ...) {
+  if (b) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    b->err = buf;
+  }
+  return code;
+}
+#define HIP_TRYB(b, call)                                                                                   \
+  do {                                                                                                      \
+    hipError_t e_ = (call);                                                                                 \
+    if (e_ != hipSuccess) return bfail((b), e_ == hipErrorOutOfMemory ? AHT_ENOMEM : AHT_EDEVICE, "%s: %s", \
+                                       #call, hipGetErrorString(e_));                                       \
+  } while (0)
+
+int cu_count(int device) {
+  int n = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) n = 256;
+  return n;
+}
+}  // namespace
+
+int aht_batch_create(int32_t device, int32_t storage, aht_batch **out) {
+  if (!out) return AHT_EINVAL;
+  *out = nullptr;
+  if (storage < AHT_STORE_F32 || storage > AHT_STORE_F64) return AHT_EINVAL;
+  aht_batch *b = new aht_batch();
+  *out = b;
+  b->device = device;
+  b->storage = storage;
+  HIP_TRYB(b, hipSetDevice(device));
+  HIP_TRYB(b, hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
+  HIP_TRYB(b, hipEventCreate(&b->ev0));
+  HIP_TRYB(b, hipEventCreate(&b->ev1));
+  const int cus = cu_count(device);
+  b->cand = {cus, 2 * cus, 4 * cus, 8 * cus};
+  b->cand_ms.assign(b->cand.size(), 0.0);
+  return AHT_OK;
+}
+
+void aht_batch_destroy(aht_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  for (aht_solver *s : b->mem) s->batch = nullptr;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->dmem) (void)hipFree(b->dmem);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+}
+
+const char *aht_batch_last_error(const aht_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int aht_batch_add(aht_batch *b, aht_solver *s) {
+  if (!b || !s) return AHT_EINVAL;
+  if (s->batch) return bfail(b, AHT_ESTATE, "solver already belongs to a batch");
+  if (s->device != b->device || s->storage != b->storage)
+    return bfail(b, AHT_EINVAL, "solver device/storage (%d/%d) differ from the batch's (%d/%d)", s->device, s->storage,
+                 b->device, b->storage);
+  if (!s->has_cols || !s->has_params) return bfail(b, AHT_ESTATE, "solver needs columns and parameters first");
+  if (s->block != 256) return bfail(b, AHT_EINVAL, "batch members use 256-thread blocks");
+  b->mem.push_back(s);
+  s->batch = b;
+  b->version++;
+  return AHT_OK;
+}
+
+int aht_batch_remove(aht_batch *b, aht_solver *s) {
+  if (!b || !s) return AHT_EINVAL;
+  for (size_t k = 0; k < b->mem.size(); k++) {
+    if (b->mem[k] == s) {
+      (void)hipSetDevice(b->device);
+      (void)hipStreamSynchronize(b->st);       // the member's last chunk has finished
+      b->mem.erase(b->mem.begin() + (long)k);
+      s->batch = nullptr;
+      s->batch_pending = false;
+      b->version++;
+      return AHT_OK;
+    }
+  }
+  return bfail(b, AHT_EINVAL, "solver is not a member of this batch");
+}
+
+int aht_batch_size(const aht_batch *b) { return b ? (int)b->mem.size() : -1; }
+
+int aht_batch_set_grid(aht_batch *b, int32_t blocks, int32_t use_graph) {
+  if (!b || blocks < 0) return AHT_EINVAL;
+  b->grid = blocks;
+  b->use_graph = use_graph != 0;
+  return AHT_OK;
+}
+
+int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t *acc_from, double *ms) {
+  if (!b || n < 0) return AHT_EINVAL;
+  if (ms) *ms = 0.0;
+  if (n == 0 || b->mem.empty()) return AHT_OK;
+  HIP_TRYB(b, hipSetDevice(b->device));
+  const size_t nm = b->mem.size();
+  // member table for this chunk
+  b->hmem.resize(nm);
+  uint32_t base = 0;
+  for (size_t d = 0; d < nm; d++) {
+    aht_solver *s = b->mem[d];
+    const int ns = nsteps ? nsteps[d] : n;
+    if (ns < 0 || ns > n) return bfail(b, AHT_EINVAL, "member %zu: nsteps %d outside 0..%d", d, ns, n);
+    aht::Member &M = b->hmem[d];
+    M.g = s->g;
+    M.f[0] = s->f[0]; M.f[1] = s->f[1]; M.gq[0] = s->gq[0]; M.gq[1] = s->gq[1];
+    M.acc = aht::Acc{s->acc_surf, s->acc_face, s->acc_content};
+    const uint32_t bx = (s->g.NXY + 255) / 256;
+    M.tiles = s->g.kmin < s->g.nz ? bx * (uint32_t)(s->g.nz - s->g.kmin) : 0;
+    M.tile_base = base;
+    base += M.tiles;
+    M.parity0 = s->parity;
+    M.nsteps = ns;
+    M.acc_from = acc_from ? acc_from[d] : n;
+    M.pad = 0;
+    // the batch stream waits for work the member enqueued on its own stream
+    HIP_TRYB(b, hipEventRecord(s->own, s->st));
+    HIP_TRYB(b, hipStreamWaitEvent(b->st, s->own, 0));
+  }
+  b->total = base;
+  if (b->dcap < nm) {
+    if (b->dmem) HIP_TRYB(b, hipFree(b->dmem));
+    b->dcap = nm + 16;
+    HIP_TRYB(b, hipMalloc(&b->dmem, b->dcap * sizeof(aht::Member)));
+    b->version++;
+  }
+  HIP_TRYB(b, hipMemcpyAsync(b->dmem, b->hmem.data(), nm * sizeof(aht::Member), hipMemcpyHostToDevice, b->st));
+  // grid: fixed, or the online tuner's pick (each candidate computes the same result)
+  int grid = b->grid;
+  int cand_idx = -1;
+  if (grid == 0) {
+    if (b->tune_version != b->version) {
+      b->tune_version = b->version;
+      b->tune_next = 0;
+      std::fill(b->cand_ms.begin(), b->cand_ms.end(), 0.0);
+    }
+    if (b->tune_next < (int)b->cand.size()) {
+      cand_idx = b->tune_next++;
+      grid = b->cand[cand_idx];
+    } else {
+      size_t best = 0;
+      for (size_t k = 1; k < b->cand.size(); k++)
+        if (b->cand_ms[k] < b->cand_ms[best]) best = k;
+      grid = b->cand[best];
+    }
+  }
+  HIP_TRYB(b, hipEventRecord(b->ev0, b->st));
+  if (b->use_graph) {
+    if (!b->gexec || b->g_version != b->version || b->g_n != n || b->g_grid != grid) {
+      if (b->gexec) HIP_TRYB(b, hipGraphExecDestroy(b->gexec));
+      b->gexec = nullptr;
+      hipGraph_t graph;
+      HIP_TRYB(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+      for (int it = 0; it < n; it++) {
+        const hipError_t e = aht::launch_batch_step(b->storage, b->dmem, (int)nm, b->total, it, grid, b->st);
+        if (e != hipSuccess) {
+          hipGraph_t dummy;
+          (void)hipStreamEndCapture(b->st, &dummy);
+          return bfail(b, AHT_EDEVICE, "batch launch during capture: %s", hipGetErrorString(e));
+        }
+      }
+      HIP_TRYB(b, hipStreamEndCapture(b->st, &graph));
+      HIP_TRYB(b, hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0));
+      HIP_TRYB(b, hipGraphDestroy(graph));
+      b->g_version = b->version; b->g_n = n; b->g_grid = grid;
+    }
+    HIP_TRYB(b, hipGraphLaunch(b->gexec, b->st));
+  } else {
+    for (int it = 0; it < n; it++)
+      HIP_TRYB(b, aht::launch_batch_step(b->storage, b->dmem, (int)nm, b->total, it, grid, b->st));
+  }
+  HIP_TRYB(b, hipEventRecord(b->ev1, b->st));
+  // member bookkeeping; their next own-stream work waits for this chunk
+  for (size_t d = 0; d < nm; d++) {
+    aht_solver *s = b->mem[d];
+    const aht::Member &M = b->hmem[d];
+    s->parity ^= (M.nsteps & 1);
+    s->steps += M.nsteps;
+    s->n_acc += std::max(0, M.nsteps - std::max(0, M.acc_from));
+    HIP_TRYB(b, hipEventRecord(s->after_batch, b->st));
+    s->batch_pending = true;
+  }
+  b->steps += n;
+  if (cand_idx >= 0 || ms) {
+    HIP_TRYB(b, hipEventSynchronize(b->ev1));
+    float t = 0.0f;
+    HIP_TRYB(b, hipEventElapsedTime(&t, b->ev0, b->ev1));
+    if (cand_idx >= 0) b->cand_ms[cand_idx] = t / n;
+    if (ms) *ms = t;
+  }
+  return AHT_OK;
+}
+
+int aht_batch_sync(aht_batch *b) {
+  if (!b) return AHT_EINVAL;
+  HIP_TRYB(b, hipSetDevice(b->device));
+  HIP_TRYB(b, hipStreamSynchronize(b->st));
+  return AHT_OK;
+}
+
+int aht_batch_grid(const aht_batch *b) {
+  if (!b) return -1;
+  if (b->grid) return b->grid;
+  if (b->tune_next < (int)b->cand.size()) return 0;
+  size_t best = 0;
+  for (size_t k = 1; k < b->cand.size(); k++)
+    if (b->cand_ms[k] < b->cand_ms[best]) best = k;
+  return b->cand[best];
+}
+
+// ---- block-size autotuner for one solver -------------------------------------------------------------------------
+int aht_autotune(aht_solver *s, int32_t steps, int32_t *block) {
+  int rc = check_ready(s);
+  if (rc) return rc;
+  if (steps < 1) return fail(s, AHT_EINVAL, "autotune needs steps >= 1");
+  if ((rc = set_device(s))) return rc;
+  const size_t N = s->g.N, fb = 19 * N * s->eb, gb = 7 * N * s->eb;
+  const int p0 = s->parity;
+  const int64_t steps0 = s->steps, nacc0 = s->n_acc;
+  void *bf = nullptr, *bg = nullptr;
+  HIP_TRY(s, hipMalloc(&bf, fb));
+  const hipError_t eg = hipMalloc(&bg, gb);
+  if (eg != hipSuccess) { (void)hipFree(bf); return fail(s, AHT_ENOMEM, "autotune backup: %s", hipGetErrorString(eg)); }
+  HIP_TRY(s, hipMemcpyAsync(bf, s->f[p0], fb, hipMemcpyDeviceToDevice, s->st));
+  HIP_TRY(s, hipMemcpyAsync(bg, s->gq[p0], gb, hipMemcpyDeviceToDevice, s->st));
+  const int cands[3] = {64, 128, 256};
+  double best_ms = 1e30;
+  int best = s->block;
+  for (int b : cands) {
+    s->block = b;
+    double ms = 0;
+    rc = aht_benchmark(s, steps, &ms);
+    if (rc) break;
+    if (ms < best_ms) { best_ms = ms; best = b; }
+  }
+  s->block = best;
+  // restore the state and the counters: tuning leaves the simulation where it was
+  (void)hipMemcpyAsync(s->f[p0], bf, fb, hipMemcpyDeviceToDevice, s->st);
+  (void)hipMemcpyAsync(s->gq[p0], bg, gb, hipMemcpyDeviceToDevice, s->st);
+  (void)hipStreamSynchronize(s->st);
+  (void)hipFree(bf);
+  (void)hipFree(bg);
+  s->parity = p0; s->steps = steps0; s->n_acc = nacc0;
+  if (rc) return rc;
+  if (block) *block = best;
+  return AHT_OK;
+}
+
+int aht_set_block(aht_solver *s, int32_t block) {
+  if (!s || block < 64 || block > 256 || block % 64) return s ? fail(s, AHT_EINVAL, "block must be 64, 128, 192 or 256") : AHT_EINVAL;
+  s->block = block;
+  return AHT_OK;
 }
 
 }  // extern "C"

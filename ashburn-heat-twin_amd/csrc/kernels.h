@@ -15,7 +15,8 @@ struct Grid {
   uint32_t N;          // nx * ny * nz (< 2^32; per-q offsets use 64-bit arithmetic)
   float inv_nx;        // 1 / nx, for the column -> (i, j) split
   int pad;
-  const uint4 *colv;   // per column (kt, ktop, cd bits, q bits), solver orientation
+  const uint32_t *colk; // per column: kt | ktop << 10 | kfast << 20 (read by every cell), solver orientation
+  const float2 *colcq;  // per column: (cd, q), read only at the first fluid cell
   const uint32_t *ktop;
   const float4 *prof;  // [nz] (u_in, v_in, kb, 0)
   Params P;
@@ -29,6 +30,21 @@ struct Acc {
 };
 
 constexpr int STEP_BLOCK = 256;
+constexpr uint32_t NUM_XCD = 8;   // MI355X: workgroups are dispatched round-robin over 8 XCDs, each with its own L2
+
+// one member of a batched launch (device-resident table, rebuilt by the host when the batch changes)
+struct Member {
+  Grid g;
+  void *f[2];
+  void *gq[2];
+  Acc acc;
+  uint32_t tiles;      // (256-column block, level) tiles of one step
+  uint32_t tile_base;  // first tile in the concatenated list
+  int parity0;         // lattice holding the state at launch index 0 of the chunk
+  int nsteps;          // launches of the chunk in which this member steps
+  int acc_from;        // first launch index that accumulates the FP64 means (>= nsteps: none)
+  int pad;
+};
 
 size_t face_offset(int face, int nx, int ny, int nz);   // offset of face 0..3 in Acc::face
 size_t face_len(int face, int nx, int ny, int nz);      // doubles per face (2 * nz * ny or 2 * nz * nx)
@@ -39,6 +55,9 @@ int storage_bytes(int storage);                         // bytes per population 
 // kernels are templated on the storage format; the launchers dispatch on the AHT_STORE_* id
 hipError_t launch_step(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB, const Acc *acc,
                        float *out4, int block, hipStream_t st);
+// one step of every batch member (launch index `it` of a chunk) on a persistent grid of `grid` blocks of 256
+hipError_t launch_batch_step(int storage, const Member *mem, int nmem, uint32_t total_tiles, int it, int grid,
+                             hipStream_t st);
 hipError_t launch_init(int storage, const Grid &g, void *f0, void *f1, hipStream_t st);
 hipError_t launch_reinit(int storage, const Grid &g, const uint32_t *old_ktop, void *f, void *gq, hipStream_t st);
 hipError_t launch_surface(int storage, const Grid &g, const void *fA, const void *gA, float conv, float *out3,

@@ -13,6 +13,8 @@
 //     only per-lane address is the cell's byte offset. Drag and the rooftop source cannot apply there (k > ktop).
 //   * boundary path: every branch of js/live/kernel.js:159-243 per population (lid first, then lateral faces, then the
 //     interior pull with half-way bounce-back from solid sources), with the 3 x 3 neighbour ktop values loaded once.
+#include <algorithm>
+
 #include "kernels.h"
 #include "storage.h"
 
@@ -45,6 +47,13 @@ __device__ __forceinline__ void split_column(const Grid &g, int c, int &i, int &
   if (ii < 0) { jj -= 1; ii += g.nx; }
   else if (ii >= g.nx) { jj += 1; ii -= g.nx; }
   i = ii; j = jj;
+}
+
+// XCD-aware tile order: block b runs on XCD b % 8; give every XCD a contiguous range of tiles so the cache lines that
+// neighbouring tiles share (the +-1 cell and +-1 row shifted pulls) are fetched once into that XCD's L2
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t G) {
+  const uint32_t q = G / NUM_XCD, r = G % NUM_XCD, x = b % NUM_XCD, i = b / NUM_XCD;
+  return x * q + min(x, r) + i;
 }
 
 // packed column word: kt | ktop << 10 | kfast << 20 (nz < 1023)
@@ -283,16 +292,16 @@ __device__ __forceinline__ void stream_cell(const Grid &g, const void *fA, const
   }
 }
 
-// the fused step (genStepWGSL, js/live/kernel.js:395-431) with FP64 accumulation (ACC) and debug outputs (OUT)
+// one cell of the fused step (genStepWGSL, js/live/kernel.js:395-431) with FP64 accumulation (ACC) and debug outputs
+// (OUT). Every thread of the block must call it (the ACC heat-content sum is a wavefront reduction); `slot` is the
+// wavefront's heat-content slot.
 template <class S, bool ACC, bool OUT>
-__global__ __launch_bounds__(256) void step_kernel(Grid g, const void *fA, void *fB, const void *gA, void *gB, Acc acc,
-                                                   float *__restrict__ out4) {
+__device__ __forceinline__ void step_cell(const Grid &g, const void *fA, void *fB, const void *gA, void *gB,
+                                          const Acc &acc, float *__restrict__ out4, int c, int k, uint32_t slot) {
   using R = typename S::real;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = g.kmin + blockIdx.y;
   const bool live = c < (int)g.NXY;
-  const uint4 cv = g.colv[live ? c : 0];
-  const int ktc = col_kt(cv.x), kg = col_ktop(cv.x);
+  const uint32_t ck = g.colk[live ? c : 0];
+  const int ktc = col_kt(ck), kg = col_ktop(ck);
   const bool fluid = live && k >= kg;
   if (!ACC && !fluid) return;
   double th_cell = 0.0;
@@ -302,17 +311,18 @@ __global__ __launch_bounds__(256) void step_kernel(Grid g, const void *fA, void 
     int i, j;
     split_column(g, c, i, j);
     const float4 pr = g.prof[max(k - ktc, 0)];
-    const bool fast = k >= col_kfast(cv.x) && k < g.nz - 1 && i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
+    const bool fast = k >= col_kfast(ck) && k < g.nz - 1 && i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
+    const float2 cq = (k == kg) ? g.colcq[c] : make_float2(0.0f, 0.0f);   // drag and source act at k == ktop only
     R f[19], gg[7];
     stream_cell<S>(g, fA, gA, n, i, j, k, kg, fast, R(pr.x), R(pr.y), f, gg);
     const Moments<R> m = moments(f, gg);
-    const Forces<R> F = forces<true, true>(g.P, g.nz, k, kg, ktc, R(__uint_as_float(cv.z)), R(pr.x), R(pr.y), m);
+    const Forces<R> F = forces<true, true>(g.P, g.nz, k, kg, ktc, R(cq.x), R(pr.x), R(pr.y), m);
     const Relax<R> Rx = relax(g.P, g.nx, g.ny, i, j, R(pr.z), m, F);
     const uint32_t boff = n * (uint32_t)S::bytes;
     S::st_raw(fB, boff, S::enc_f(collide_rest(Rx), w19<0>()));
     StorePairs<1>::template run<S>(fB, N, boff, m, F, Rx);
     // D3Q7: g^post = a g + b (1 + 4 c.u) + w S (1 - sg)
-    const R Sq = (k == kg) ? R(__uint_as_float(cv.w)) * R(g.P.srcscale) : R(0);
+    const R Sq = (k == kg) ? R(cq.y) * R(g.P.srcscale) : R(0);
     const Scalar<R> sc = scalar_coeffs(m.th, Rx.omg, Sq, F.sg);
     const R b1 = sc.b + sc.cs1, b4 = R(4) * sc.b;
     S::st_raw(plane<S>(gB, N, 0), boff, S::enc_g(sc.a * gg[0] + sc.cs0));
@@ -361,11 +371,49 @@ __global__ __launch_bounds__(256) void step_kernel(Grid g, const void *fA, void 
   }
   if (ACC) {
     // heat content: one slot per wavefront, each updated by that wavefront only (deterministic, no atomics)
-    const double s = wave_sum(th_cell);
-    if ((threadIdx.x & 63) == 0) {
-      const uint32_t slot = ((uint32_t)blockIdx.y * gridDim.x + blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-      acc.content[slot] += s;
-    }
+    const double sum = wave_sum(th_cell);
+    if ((threadIdx.x & 63) == 0) acc.content[slot] += sum;
+  }
+}
+
+// one block per (column block, level) tile, tiles numbered column block fastest, then level, in XCD-aware order
+template <class S, bool ACC, bool OUT>
+__global__ __launch_bounds__(256) void step_kernel(Grid g, const void *fA, void *fB, const void *gA, void *gB, Acc acc,
+                                                   float *__restrict__ out4, uint32_t bx) {
+  const uint32_t t = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t kk = t / bx, cb = t - kk * bx;
+  const int c = (int)(cb * blockDim.x + threadIdx.x);
+  const int k = g.kmin + (int)kk;
+  const uint32_t slot = t * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  step_cell<S, ACC, OUT>(g, fA, fB, gA, gB, acc, out4, c, k, slot);
+}
+
+// ---- many domains in one launch ----------------------------------------------------------------------------------
+// A persistent grid walks one step of every batch member: member d contributes tiles (256-column block, level) for
+// its fluid levels; block b handles the contiguous tile range [b*T/G, (b+1)*T/G) of the concatenated list. Per-member
+// fields make the result of a member independent of the mix: it steps only while `it < nsteps` (it = index of this
+// launch within the captured chunk), reads lattice parity0 ^ (it & 1), and accumulates means from launch acc_from on.
+template <class S>
+__global__ __launch_bounds__(256) void batch_step_kernel(const Member *__restrict__ mem, int nmem, uint32_t total,
+                                                         int it) {
+  const uint32_t t0 = (uint32_t)(((uint64_t)blockIdx.x * total) / gridDim.x);
+  const uint32_t t1 = (uint32_t)(((uint64_t)(blockIdx.x + 1) * total) / gridDim.x);
+  int d = 0;
+  for (uint32_t t = t0; t < t1; t++) {
+    while (d + 1 < nmem && mem[d + 1].tile_base <= t) d++;        // tiles are visited in increasing order
+    while (d > 0 && mem[d].tile_base > t) d--;
+    const Member &M = mem[d];
+    if (it >= M.nsteps) continue;
+    const uint32_t lt = t - M.tile_base;
+    const uint32_t bx = (M.g.NXY + 255) / 256;
+    const int k = M.g.kmin + (int)(lt / bx);
+    const int c = (int)(lt % bx) * 256 + (int)threadIdx.x;
+    const int p = M.parity0 ^ (it & 1);
+    const uint32_t slot = lt * 4 + (threadIdx.x >> 6);
+    if (it >= M.acc_from)
+      step_cell<S, true, false>(M.g, M.f[p], M.f[1 - p], M.gq[p], M.gq[1 - p], M.acc, nullptr, c, k, slot);
+    else
+      step_cell<S, false, false>(M.g, M.f[p], M.f[1 - p], M.gq[p], M.gq[1 - p], M.acc, nullptr, c, k, slot);
   }
 }
 
@@ -378,8 +426,8 @@ __global__ __launch_bounds__(256) void surface_kernel(Grid g, const void *fA, co
   if (c >= (int)g.NXY) return;
   int i, j;
   split_column(g, c, i, j);
-  const uint4 cv = g.colv[c];
-  const int ktc = col_kt(cv.x), kg = col_ktop(cv.x), k = kg;
+  const uint32_t ck = g.colk[c];
+  const int ktc = col_kt(ck), kg = col_ktop(ck), k = kg;
   const uint32_t o = (uint32_t)(g.ny - 1 - j) * g.nx + i;
   if (k >= g.nz) {
     out3[o] = 0.0f; out3[g.NXY + o] = 0.0f; out3[2 * g.NXY + o] = 0.0f;
@@ -390,7 +438,7 @@ __global__ __launch_bounds__(256) void surface_kernel(Grid g, const void *fA, co
   R f[19], gg[7];
   stream_cell<S>(g, fA, gA, n, i, j, k, kg, false, R(pr.x), R(pr.y), f, gg);
   const Moments<R> m = moments(f, gg);
-  const Forces<R> F = forces<true, true>(g.P, g.nz, k, kg, ktc, R(__uint_as_float(cv.z)), R(pr.x), R(pr.y), m);
+  const Forces<R> F = forces<true, true>(g.P, g.nz, k, kg, ktc, R(g.colcq[c].x), R(pr.x), R(pr.y), m);
   out3[o] = (float)m.th;
   out3[g.NXY + o] = (float)(F.ux * R(conv));
   out3[2 * g.NXY + o] = (float)(F.uy * R(conv));
@@ -406,7 +454,7 @@ __global__ __launch_bounds__(256) void init_kernel(Grid g, void *f0, void *f1) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.y;
   if (c >= (int)g.NXY) return;
-  const int ktc = col_kt(g.colv[c].x);
+  const int ktc = col_kt(g.colk[c]);
   const int zagl = min(max(k - ktc, 0), g.nz - 1);
   const float4 pr = g.prof[zagl];
   const R U = R(pr.x), V = R(pr.y), usq = U * U + V * V;
@@ -629,14 +677,23 @@ int storage_bytes(int storage) { return storage == 0 ? 4 : (storage == 3 ? 8 : 2
 hipError_t launch_step(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB, const Acc *acc,
                        float *out4, int block, hipStream_t st) {
   if (g.kmin >= g.nz) return hipSuccess;
-  const dim3 grid((g.NXY + block - 1) / block, g.nz - g.kmin);
+  const uint32_t bx = (g.NXY + block - 1) / block;
+  const dim3 grid(bx * (uint32_t)(g.nz - g.kmin));
   const Acc a = acc ? *acc : Acc{nullptr, nullptr, nullptr};
   AHT_DISPATCH(storage, {
-    if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4);
-    else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4);
-    else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4);
-    else hipLaunchKernelGGL((step_kernel<S, false, false>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4);
+    if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
+    else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
+    else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
+    else hipLaunchKernelGGL((step_kernel<S, false, false>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
   })
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_step(int storage, const Member *mem, int nmem, uint32_t total_tiles, int it, int grid,
+                             hipStream_t st) {
+  if (total_tiles == 0 || nmem == 0) return hipSuccess;
+  const int G = (int)std::min<uint32_t>((uint32_t)grid, total_tiles);
+  AHT_DISPATCH(storage, hipLaunchKernelGGL(batch_step_kernel<S>, dim3(G), dim3(256), 0, st, mem, nmem, total_tiles, it))
   return hipGetLastError();
 }
 

@@ -1,0 +1,226 @@
+'use strict';
+// Serving loop: siting scenarios of different grid sizes and winds arrive as a Poisson process; each runs a spin-up
+// and then accumulates the FP64 time mean of the near-ground warming until the mean window is complete.
+//
+// Continuous batching: every active scenario is a member of one aht batch (one persistent launch per step for all
+// members, chunks captured as hipGraphs). Arrivals wait in a FIFO queue and are admitted at chunk boundaries while
+// the summed device footprint stays under the HBM budget and a slot is free; finished scenarios are read, leave the
+// batch and free their memory, which admits the next ones. Each member steps exactly its own step count and averages
+// exactly its own window, so its result is bit-identical to running it alone (tests/test_gpu_serve.py).
+//
+// The baseline (`mode: 'sequential'`) runs the same requests one at a time with the plain per-solver path.
+// Usage: node serve.js --rate 5 --requests 24 [--mode batch|sequential] [--spinup 2200] [--mean 600] [--seed 1]
+//        [--chunk 50] [--budget-gb 200] [--slots 64] [--out file.json]
+
+const fs = require('fs');
+const { Scenario } = require('./scenario');
+const S = require('./synth');
+const { loadAddon } = require('./solver');
+const P = require('./physics');
+
+const STEP_ACC = 1;
+
+/** seeded request list: Poisson arrivals, grid mix, wind, campus */
+function makeRequests(o) {
+  const rng = S.mulberry32(o.seed || 1);
+  const grids = o.grids || S.SERVE_GRIDS;
+  const out = [];
+  let t = 0;
+  for (let k = 0; k < o.requests; k++) {
+    t += -Math.log(1 - rng()) / o.rate;                 // exponential inter-arrival times
+    let u = rng(), g = grids[grids.length - 1][0];
+    for (const [shape, p] of grids) { if (u < p) { g = shape; break; } u -= p; }
+    out.push({
+      id: k, arrival: t, grid: g, u10: 2 + 7 * rng(), dirFrom: 360 * rng(),
+      campusMW: 60 + 120 * rng(), seed: 1000 * (o.seed || 1) + k,
+    });
+  }
+  return out;
+}
+
+/** the scenario of one request: synthetic 6 x 6 km box at dx = 6000 / nx with a 300 x 200 m campus */
+function buildScenario(req, o) {
+  const [nx, ny, nz] = req.grid;
+  const dx = (o.boxM || 6000) / nx;
+  const sc = new Scenario({
+    synthetic: { nx, ny, hills: 12, buildings: o.buildings === undefined ? 4000 : o.buildings,
+      sources: o.sources === undefined ? 210 : o.sources, seed: req.seed,
+      hillAmp: o.hillAmp, hillWidth: o.hillWidth, side: o.side, height: o.height },
+    dx, nz, u10Max: req.u10, alpha: 0.2, existingFlux: 400,
+    met: { u10: req.u10, dirFrom: req.dirFrom, wstar: 1.67, blh: 903 },
+  });
+  const halls = [S.placeCampus(sc.agg, o.campusW || 300, o.campusH || 200, req.seed)];
+  sc.setCampuses([{ id: 'new', mw: req.campusMW, halls }]);
+  const spin = Math.ceil(o.spinupS / sc.dt), mean = Math.ceil(o.meanS / sc.dt);
+  return { sc, spin, mean, total: spin + mean };
+}
+
+function result(req, job, A, h, now) {
+  const sc = job.sc, NXY = sc.nx * sc.ny;
+  const s3 = new Float64Array(3 * NXY);
+  const b = A.readMean(h, s3);
+  let sum = 0, peak = -Infinity;
+  for (let k = 0; k < NXY; k++) { sum += s3[k]; if (s3[k] > peak) peak = s3[k]; }
+  const dx = sc.dx;
+  return {
+    id: req.id, grid: req.grid, steps: job.total, meanSteps: job.mean, dt: sc.dt,
+    warmingMean: sum / NXY, warmingPeak: peak, nAcc: b.n_acc,
+    heatInjectedMW: b.heat_in * P.RHO_CP * dx * dx * dx / sc.dt / 1e6,
+    heatAdvectedOutMW: P.RHO_CP * b.heat_out * sc.conv * dx * dx / 1e6,
+    massImbalance: b.mass_imbalance, arrival: req.arrival, done: now, latency: now - req.arrival,
+    surfMean: job.keepSurf ? Array.from(s3.subarray(0, NXY)) : undefined,
+  };
+}
+
+function createSolver(A, job, o) {
+  const sc = job.sc, c = sc.col;
+  const h = A.create({ nx: sc.nx, ny: sc.ny, nz: sc.nz, storage: o.storage || 0, device: o.device || 0 });
+  A.setColumns(h, c.kt, c.ktop, c.cd, c.q);
+  A.setParams(h, sc.params, sc.profBlock());
+  A.initFlow(h);
+  A.resetHeat(h);
+  A.meanReset(h);
+  return h;
+}
+
+const sleep = ms => new Promise(r => setTimeout(r, ms));
+const pct = (a, p) => { const s = a.slice().sort((x, y) => x - y); return s[Math.min(s.length - 1, Math.floor(p * (s.length - 1) + 0.5))]; };
+
+function summary(mode, o, results, t0, t1, peakBytes, busyMs) {
+  const lat = results.map(r => r.latency);
+  const span = t1 - t0;
+  return {
+    mode, rate: o.rate, requests: results.length, makespan_s: +span.toFixed(3),
+    scenarios_per_hour: +(results.length / span * 3600).toFixed(1),
+    latency_p50_s: +pct(lat, 0.5).toFixed(3), latency_p99_s: +pct(lat, 0.99).toFixed(3),
+    latency_mean_s: +(lat.reduce((a, b) => a + b, 0) / lat.length).toFixed(3),
+    peak_hbm_gb: +(peakBytes / 1e9).toFixed(2), gpu_busy_frac: +(busyMs / 1e3 / span).toFixed(3),
+  };
+}
+
+/** continuous batching over one aht batch */
+async function serveBatched(reqs, o) {
+  const A = loadAddon();
+  const B = A.batchCreate(o.device || 0, o.storage || 0);
+  A.batchSetGrid(B, o.grid || 0, o.useGraph === false ? 0 : 1);
+  const budget = (o.budgetGB || 200) * 1e9, slots = o.slots || 64, chunk = o.chunk || 50;
+  const future = reqs.slice().sort((a, b) => a.arrival - b.arrival);
+  const pending = [], active = [], results = [];
+  let used = 0, peak = 0, busy = 0;
+  const t0 = Date.now() / 1e3;
+  const now = () => Date.now() / 1e3 - t0;
+  while (future.length || pending.length || active.length) {
+    while (future.length && future[0].arrival <= now()) pending.push(future.shift());
+    // admission: FIFO, by footprint and slots
+    while (pending.length && active.length < slots) {
+      const r = pending[0];
+      const fp = A.footprint({ nx: r.grid[0], ny: r.grid[1], nz: r.grid[2], storage: o.storage || 0 });
+      if (used + fp > budget && active.length > 0) break;
+      pending.shift();
+      const job = buildScenario(r, o);
+      job.keepSurf = !!o.keepSurf;
+      const h = createSolver(A, job, o);
+      A.batchAdd(B, h);
+      active.push({ r, job, h, done: 0, fp });
+      used += fp;
+      peak = Math.max(peak, used);
+    }
+    if (!active.length) {
+      if (future.length) await sleep(Math.max(0, (future[0].arrival - now()) * 1e3));
+      continue;
+    }
+    const m = active.length, ns = new Int32Array(m), af = new Int32Array(m);
+    let n = 0;
+    for (let k = 0; k < m; k++) {
+      const a = active[k];
+      ns[k] = Math.min(chunk, a.job.total - a.done);
+      af[k] = Math.max(0, a.job.spin - a.done);
+      n = Math.max(n, ns[k]);
+    }
+    busy += await A.batchStep(B, n, ns, af);
+    for (let k = 0; k < m; k++) active[k].done += ns[k];
+    for (let k = active.length - 1; k >= 0; k--) {
+      const a = active[k];
+      if (a.done >= a.job.total) {
+        results.push(result(a.r, a.job, A, a.h, now()));
+        A.batchRemove(B, a.h);
+        A.destroy(a.h);
+        used -= a.fp;
+        active.splice(k, 1);
+      }
+    }
+  }
+  const t1 = now();
+  const grid = A.batchGrid(B);
+  A.batchDestroy(B);
+  results.sort((a, b) => a.id - b.id);
+  return { summary: Object.assign(summary('batch', o, results, 0, t1, peak, busy), { persistent_grid: grid }), results };
+}
+
+/** the baseline: one scenario at a time with the plain per-solver path */
+async function serveSequential(reqs, o) {
+  const A = loadAddon();
+  const chunk = o.chunk || 50;
+  const queue = reqs.slice().sort((a, b) => a.arrival - b.arrival);
+  const results = [];
+  let peak = 0, busy = 0;
+  const t0 = Date.now() / 1e3;
+  const now = () => Date.now() / 1e3 - t0;
+  for (const r of queue) {
+    if (r.arrival > now()) await sleep((r.arrival - now()) * 1e3);
+    const job = buildScenario(r, o);
+    job.keepSurf = !!o.keepSurf;
+    const h = createSolver(A, job, o);
+    peak = Math.max(peak, A.footprint({ nx: r.grid[0], ny: r.grid[1], nz: r.grid[2], storage: o.storage || 0 }));
+    for (let done = 0; done < job.total;) {
+      const n = Math.min(chunk, job.total - done), spinLeft = job.spin - done;
+      if (spinLeft >= n) busy += await A.benchmark(h, n);
+      else {
+        if (spinLeft > 0) busy += await A.benchmark(h, spinLeft);
+        const t = Date.now();
+        await A.step(h, n - Math.max(0, spinLeft), STEP_ACC);
+        busy += Date.now() - t;
+      }
+      done += n;
+    }
+    results.push(result(r, job, A, h, now()));
+    A.destroy(h);
+  }
+  const t1 = now();
+  results.sort((a, b) => a.id - b.id);
+  return { summary: summary('sequential', o, results, 0, t1, peak, busy), results };
+}
+
+function parseArgs(argv) {
+  const a = {};
+  for (let i = 0; i < argv.length; i++) {
+    if (argv[i].startsWith('--')) {
+      const k = argv[i].slice(2), v = argv[i + 1];
+      if (v === undefined || v.startsWith('--')) a[k] = true; else { a[k] = v; i++; }
+    }
+  }
+  return a;
+}
+
+async function main() {
+  const a = parseArgs(process.argv.slice(2));
+  const o = {
+    rate: +(a.rate || 5), requests: +(a.requests || 24), seed: +(a.seed || 1), chunk: +(a.chunk || 50),
+    spinupS: +(a.spinup || 2200), meanS: +(a.mean || 600), budgetGB: +(a['budget-gb'] || 200), slots: +(a.slots || 64),
+    storage: +(a.storage || 0), grid: +(a.grid || 0), useGraph: a['no-graph'] ? false : true,
+  };
+  if (a.grids) o.grids = JSON.parse(a.grids);
+  const reqs = makeRequests(o);
+  const modes = (a.mode || 'batch').split(',');
+  const out = {};
+  for (const mode of modes) {
+    const r = mode === 'sequential' ? await serveSequential(reqs, o) : await serveBatched(reqs, o);
+    out[mode] = r;
+    console.log(JSON.stringify(r.summary));
+  }
+  if (a.out) fs.writeFileSync(a.out, JSON.stringify(out));
+}
+
+module.exports = { makeRequests, buildScenario, serveBatched, serveSequential };
+
+if (require.main === module) main().catch(e => { console.error(e && e.stack || e); process.exit(1); });

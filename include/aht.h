@@ -132,6 +132,30 @@ int aht_write_state(aht_solver *s, const float *f, const float *g);
 int aht_read_state_f64(aht_solver *s, double *f, double *g);
 int aht_write_state_f64(aht_solver *s, const double *f, const double *g);
 
+/* ---- batches: many uneven domains advanced by one persistent launch per step (no counterpart in the reference, whose
+ * host runs one domain per device). Members keep their own state, accumulators and read-back entry points. */
+typedef struct aht_batch aht_batch;
+int aht_batch_create(int32_t device, int32_t storage, aht_batch **out);
+void aht_batch_destroy(aht_batch *b);
+const char *aht_batch_last_error(const aht_batch *b);
+int aht_batch_add(aht_batch *b, aht_solver *s);       /* same device and storage, 256-thread blocks        */
+int aht_batch_remove(aht_batch *b, aht_solver *s);
+int aht_batch_size(const aht_batch *b);
+/* one chunk of n launches: member d (in add order) steps nsteps[d] <= n times (NULL: n) and accumulates the FP64
+ * means from its launch acc_from[d] on (NULL or >= nsteps[d]: none). The n launches are captured once as a hipGraph
+ * and replayed while membership, n and the grid stay the same. *ms (may be NULL) = event time of the chunk. */
+int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t *acc_from, double *ms);
+int aht_batch_sync(aht_batch *b);
+/* persistent grid size: blocks > 0 fixes it, 0 = online tuning over 1, 2, 4, 8 blocks per CU on the first chunks of
+ * each membership (every candidate computes the same result); use_graph 0 launches without graph capture */
+int aht_batch_set_grid(aht_batch *b, int32_t blocks, int32_t use_graph);
+int aht_batch_grid(const aht_batch *b);              /* chosen grid (0 while still tuning)                 */
+
+/* block-size autotuner of one solver: times 64/128/256-thread blocks over `steps` steps each, keeps the fastest, and
+ * restores the state and counters (no counterpart in the reference) */
+int aht_autotune(aht_solver *s, int32_t steps, int32_t *block);
+int aht_set_block(aht_solver *s, int32_t block);
+
 /* device bytes held by the solver (admission by footprint in the serving loop) */
 int64_t aht_footprint_bytes(const aht_dims *dims);
 

@@ -1,0 +1,24 @@
+'use strict';
+// Continuous batching keeps every result exact: serve a burst of requests through one batch, then run each request
+// alone with the plain per-solver path and compare the time-mean warming fields bit for bit. Prints one JSON object.
+const path = require('path');
+const S = require(path.join(__dirname, '..', '..', 'ashburn-heat-twin_amd', 'js', 'serve.js'));
+
+async function main() {
+  const o = { rate: 50, requests: 6, seed: 3, chunk: 16, spinupS: 200, meanS: 100, budgetGB: 50, slots: 4,
+    grids: [[[24, 20, 12], 0.4], [[32, 28, 14], 0.3], [[40, 24, 16], 0.3]], buildings: 40, sources: 4,
+    keepSurf: true };
+  const reqs = S.makeRequests(o);
+  const batched = await S.serveBatched(reqs, o);
+  const out = { summary: batched.summary, identical: [], nAcc: [] };
+  for (const r of reqs) {
+    const alone = await S.serveSequential([Object.assign({}, r, { arrival: 0 })], o);
+    const a = alone.results[0], b = batched.results.find(x => x.id === r.id);
+    let same = a.surfMean.length === b.surfMean.length;
+    for (let k = 0; same && k < a.surfMean.length; k++) if (a.surfMean[k] !== b.surfMean[k]) same = false;
+    out.identical.push(same && a.warmingMean === b.warmingMean && a.heatAdvectedOutMW === b.heatAdvectedOutMW);
+    out.nAcc.push([a.nAcc, b.nAcc, a.meanSteps]);
+  }
+  console.log(JSON.stringify(out));
+}
+main().catch(e => { console.error(e && e.stack || e); process.exit(1); });

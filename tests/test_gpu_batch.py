@@ -1,0 +1,101 @@
+"""Batched stepping of uneven domains in one persistent launch per step (hipGraph-captured chunks).
+
+The bar is exact: a member's state and FP64 means after a batched run must be bit-identical to the same domain run
+alone through aht_step with the same steps and averaging window, whatever the mix, the chunking, the persistent grid
+size (online-tuned or fixed) and graph capture on or off. Tiny domains of different sizes, winds and outlets.
+"""
+import numpy as np
+import pytest
+
+from conftest import dump_domain
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [('24,20,12,7', 210, 1), ('40,28,14,3', 30, 0), ('16,36,10,5', 95, 1)]
+
+
+def domains(tmp_path):
+    out = []
+    for k, (shape, wdir, oclip) in enumerate(SHAPES):
+        out.append(dump_domain(tmp_path / f'd{k}', '--tiny', shape, '--dx', 30, '--dir', wdir, '--u10', 5,
+                               '--oclip', oclip, '--flux', 3000))
+    return out
+
+
+def alone(pkg, dom, steps, acc_from):
+    with pkg.Solver.from_dump(dom) as s:
+        s.step(acc_from)
+        s.mean_reset()
+        s.step(steps - acc_from, acc=True)
+        f, g = s.read_state()
+        s3, b = s.read_mean()
+    return f, g, s3, b
+
+
+def test_batch_matches_alone_bitwise(tmp_path, pkg):
+    doms = domains(tmp_path)
+    steps = [37, 52, 45]
+    acc_from = [20, 30, 0]
+    ref = [alone(pkg, d, n, a) for d, n, a in zip(doms, steps, acc_from)]
+    for use_graph, grid in ((True, 0), (False, 64), (True, 3)):
+        sol = [pkg.Solver.from_dump(d) for d in doms]
+        with pkg.Batch() as b:
+            for s in sol:
+                s.mean_reset()
+                b.add(s)
+            b.set_grid(grid, use_graph)
+            done = [0, 0, 0]
+            chunk = 10
+            while any(d < n for d, n in zip(done, steps)):
+                ns = [min(chunk, n - d) for d, n in zip(done, steps)]
+                af = [max(0, a - d) for d, a in zip(done, acc_from)]
+                b.step(chunk, ns, af)
+                done = [d + k for d, k in zip(done, ns)]
+            b.sync()
+            for k, s in enumerate(sol):
+                f, g = s.read_state()
+                s3, bud = s.read_mean()
+                assert np.array_equal(f.view(np.uint32), ref[k][0].view(np.uint32)), (use_graph, grid, k)
+                assert np.array_equal(g.view(np.uint32), ref[k][1].view(np.uint32)), (use_graph, grid, k)
+                assert np.array_equal(s3, ref[k][2]), (use_graph, grid, k)
+                assert bud['n_acc'] == ref[k][3]['n_acc'] == steps[k] - acc_from[k]
+                assert bud['heat_out'] == ref[k][3]['heat_out'] and bud['content'] == ref[k][3]['content']
+                assert s.steps == steps[k]
+            for s in sol:
+                b.remove(s)
+        for s in sol:
+            s.close()
+
+
+def test_members_join_and_leave_mid_run(tmp_path, pkg):
+    """continuous batching: a member added after others have stepped, another removed early"""
+    doms = domains(tmp_path)
+    ref0 = alone(pkg, doms[0], 40, 40)
+    ref2 = alone(pkg, doms[2], 20, 20)
+    s0, s1, s2 = (pkg.Solver.from_dump(d) for d in doms)
+    with pkg.Batch() as b:
+        b.add(s0)
+        b.add(s1)
+        b.step(20)
+        b.remove(s1)                     # leaves after 20 steps
+        b.add(s2)                        # joins now
+        b.step(20)
+        b.sync()
+        f0, g0 = s0.read_state()
+        f2, g2 = s2.read_state()
+        assert np.array_equal(f0, ref0[0]) and np.array_equal(g0, ref0[1])
+        assert np.array_equal(f2, ref2[0]) and np.array_equal(g2, ref2[1])
+        assert s1.steps == 20 and s0.steps == 40 and s2.steps == 20
+    for s in (s0, s1, s2):
+        s.close()
+
+
+def test_autotune_restores_state(tmp_path, pkg):
+    dom = domains(tmp_path)[0]
+    with pkg.Solver.from_dump(dom) as s:
+        s.step(5)
+        f0, g0 = s.read_state()
+        best = s.autotune(3)
+        assert best in (64, 128, 256)
+        f1, g1 = s.read_state()
+        assert np.array_equal(f0, f1) and np.array_equal(g0, g1) and s.steps == 5
