@@ -21,6 +21,8 @@ __global__ __launch_bounds__(256) void copy_flat(const float4 *__restrict__ a, f
   if (t < n) b[t] = a[t];
 }
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ uint32_t swz(uint32_t blk, uint32_t G) {
   const uint32_t q = G / 8, r = G % 8, x = blk % 8, i = blk / 8;
   return x * q + (x < r ? x : r) + i;
@@ -41,7 +43,7 @@ __global__ __launch_bounds__(256) void copy_unroll4(const float4 *__restrict__ a
   for (int u = 0; u < 4; u++) {
     const size_t t = base + (size_t)u * 256;
     if (t < n) {
-      if (NT) __builtin_nontemporal_store(v[u], &b[t]);
+      if (NT) __builtin_nontemporal_store(*(const f4v *)&v[u], (f4v *)&b[t]);
       else b[t] = v[u];
     }
   }
