@@ -16,7 +16,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(HERE, 'lib')
-LIB_PATH = os.path.join(LIB_DIR, 'libaht.so')
+# AHT_LIB selects another build of the same C-ABI (e.g. one made with other compile options)
+LIB_PATH = os.environ.get('AHT_LIB') or os.path.join(LIB_DIR, 'libaht.so')
 ROOT = os.path.dirname(HERE)
 
 AHT_OK, AHT_EINVAL, AHT_ENOMEM, AHT_EDEVICE, AHT_ESTATE = 0, -1, -2, -3, -4
@@ -36,6 +37,15 @@ class AhtError(RuntimeError):
 class Dims(ctypes.Structure):
     _fields_ = [('nx', ctypes.c_int32), ('ny', ctypes.c_int32), ('nz', ctypes.c_int32), ('storage', ctypes.c_int32),
                 ('device', ctypes.c_int32), ('block', ctypes.c_int32), ('reserved', ctypes.c_int32 * 2)]
+
+
+class Launch(ctypes.Structure):
+    """launch shape of the step kernel: threads per block, cells per thread along z, tile window (1 = linear)"""
+    _fields_ = [('block', ctypes.c_int32), ('levels', ctypes.c_int32), ('window', ctypes.c_int32),
+                ('reserved', ctypes.c_int32)]
+
+    def as_dict(self):
+        return {'block': self.block, 'levels': self.levels, 'window': self.window}
 
 
 PARAM_F = ('tau0', 'cs2smag', 'gbeta', 'prt', 'kfloor', 'momfac', 'srcscale', 'sigmax')
@@ -96,8 +106,9 @@ SIGNATURES = {
     'aht_batch_sync': (ctypes.c_int, [_vp]),
     'aht_batch_set_grid': (ctypes.c_int, [_vp, _i32, _i32]),
     'aht_batch_grid': (ctypes.c_int, [_vp]),
-    'aht_autotune': (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_i32)]),
-    'aht_set_block': (ctypes.c_int, [_vp, _i32]),
+    'aht_autotune': (ctypes.c_int, [_vp, _i32, ctypes.POINTER(Launch)]),
+    'aht_set_launch': (ctypes.c_int, [_vp, ctypes.POINTER(Launch)]),
+    'aht_get_launch': (ctypes.c_int, [_vp, ctypes.POINTER(Launch)]),
 }
 
 _lib = None
@@ -225,12 +236,19 @@ class Solver:
         return int(lib().aht_steps_done(self.h))
 
     def autotune(self, steps=20):
-        b = ctypes.c_int32()
-        self._check(lib().aht_autotune(self.h, int(steps), ctypes.byref(b)))
-        return b.value
+        """the launch shape {block, levels, window} the autotuner keeps (now set); the state is left unchanged"""
+        best = Launch()
+        self._check(lib().aht_autotune(self.h, int(steps), ctypes.byref(best)))
+        return best.as_dict()
 
-    def set_block(self, block):
-        self._check(lib().aht_set_block(self.h, int(block)))
+    def set_launch(self, block=256, levels=1, window=1):
+        self._check(lib().aht_set_launch(self.h, ctypes.byref(Launch(int(block), int(levels), int(window), 0))))
+
+    @property
+    def launch(self):
+        l = Launch()
+        self._check(lib().aht_get_launch(self.h, ctypes.byref(l)))
+        return l.as_dict()
 
     def benchmark(self, n):
         ms = ctypes.c_double()

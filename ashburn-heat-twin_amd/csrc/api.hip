@@ -11,7 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/aht.h"
+#include "aht.h"
 #include "kernels.h"
 
 struct aht_batch;
@@ -23,6 +23,7 @@ struct aht_solver {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   aht::Grid g{};
   int block = aht::STEP_BLOCK;
+  int levels = 1;             // cells per thread along z
   int storage = AHT_STORE_F32;
   int eb = 4;                 // bytes per stored population element
   void *f[2] = {nullptr, nullptr}, *gq[2] = {nullptr, nullptr};
@@ -71,9 +72,13 @@ int fail(aht_solver *s, int code, const char *fmt, ...) {
 
 size_t face_total(int nx, int ny, int nz) { return aht::face_offset(4, nx, ny, nz); }
 
-uint32_t max_content_slots(const aht_dims &d, int block) {
+// wavefront slots of the step grid, over every block size a launch shape may pick (64, 128, 192, 256)
+uint32_t max_content_slots(const aht_dims &d) {
   const uint32_t nxy = (uint32_t)d.nx * (uint32_t)d.ny;
-  return ((nxy + block - 1) / block) * (uint32_t)d.nz * (uint32_t)(block / 64);
+  uint32_t m = 0;
+  for (uint32_t block = 64; block <= 256; block += 64)
+    m = std::max(m, ((nxy + block - 1) / block) * (uint32_t)d.nz * (block / 64));
+  return m;
 }
 
 int check_ready(aht_solver *s) {
@@ -114,7 +119,7 @@ int64_t aht_footprint_bytes(const aht_dims *d) {
   const int eb = d->storage == AHT_STORE_F32 ? 4 : (d->storage == AHT_STORE_F64 ? 8 : 2);
   const int64_t pop = 2 * 26 * N * eb;
   const int64_t cols = NXY * (4 + 8 + 4 + 4) + (int64_t)d->nz * 16;
-  const int64_t acc = 3 * NXY * 8 + (int64_t)face_total(d->nx, d->ny, d->nz) * 8 + (int64_t)max_content_slots(*d, 256) * 8;
+  const int64_t acc = 3 * NXY * 8 + (int64_t)face_total(d->nx, d->ny, d->nz) * 8 + (int64_t)max_content_slots(*d) * 8;
   const int64_t misc = ((N + 255) / 256) * 64 + 3 * NXY * 4 + (N + 3) / 4 * 4 / 1 + 4096;
   return pop + cols + acc + misc;
 }
@@ -153,6 +158,7 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   aht::Grid &g = s->g;
   g.nx = d.nx; g.ny = d.ny; g.nz = d.nz; g.kmin = d.nz; g.NXY = NXY; g.N = (uint32_t)N;
   g.inv_nx = 1.0f / (float)d.nx;
+  g.win = 1;
   for (int p = 0; p < 2; p++) {
     HIP_TRY(s, hipMalloc(&s->f[p], 19 * N * s->eb));
     HIP_TRY(s, hipMalloc(&s->gq[p], 7 * N * s->eb));
@@ -166,7 +172,7 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   HIP_TRY(s, hipMalloc(&s->prof, d.nz * sizeof(float4)));
   HIP_TRY(s, hipMalloc(&s->acc_surf, 3 * (size_t)NXY * sizeof(double)));
   HIP_TRY(s, hipMalloc(&s->acc_face, face_total(d.nx, d.ny, d.nz) * sizeof(double)));
-  s->content_cap = max_content_slots(d, block);
+  s->content_cap = max_content_slots(d);
   HIP_TRY(s, hipMalloc(&s->acc_content, s->content_cap * sizeof(double)));
   HIP_TRY(s, hipMalloc(&s->red_out, 8 * sizeof(double)));
   HIP_TRY(s, hipMalloc(&s->stat_part, 8 * (size_t)((N + 255) / 256) * sizeof(double)));
@@ -295,7 +301,7 @@ int aht_step(aht_solver *s, int32_t n, uint32_t flags) {
     float *o4 = (out && it == n - 1) ? s->out4 : nullptr;
     if (o4) HIP_TRY(s, hipMemsetAsync(o4, 0, 4 * (size_t)s->g.N * sizeof(float), s->st));
     HIP_TRY(s, aht::launch_step(s->storage, s->g, s->f[p], s->f[1 - p], s->gq[p], s->gq[1 - p], acc ? &a : nullptr, o4,
-                                s->block, s->st));
+                                s->block, s->levels, s->st));
     s->parity ^= 1;
     s->steps++;
     if (acc) s->n_acc++;
@@ -752,8 +758,22 @@ int aht_batch_grid(const aht_batch *b) {
   return b->cand[best];
 }
 
-// ---- block-size autotuner for one solver -------------------------------------------------------------------------
-int aht_autotune(aht_solver *s, int32_t steps, int32_t *block) {
+// ---- launch-shape autotuner for one solver ------------------------------------------------------------------------
+namespace {
+int check_launch(aht_solver *s, const aht_launch &l) {
+  if (l.block < 64 || l.block > 256 || l.block % 64) return fail(s, AHT_EINVAL, "block must be 64, 128, 192 or 256");
+  if (l.levels != 1 && l.levels != 2 && l.levels != 4) return fail(s, AHT_EINVAL, "levels must be 1, 2 or 4");
+  if (l.window < 0 || l.window > 4096) return fail(s, AHT_EINVAL, "window must be 0 (slabs) or 1..4096");
+  return AHT_OK;
+}
+void apply_launch(aht_solver *s, const aht_launch &l) {
+  s->block = l.block;
+  s->levels = l.levels;
+  s->g.win = (uint32_t)l.window;
+}
+}  // namespace
+
+int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best) {
   int rc = check_ready(s);
   if (rc) return rc;
   if (steps < 1) return fail(s, AHT_EINVAL, "autotune needs steps >= 1");
@@ -767,17 +787,23 @@ int aht_autotune(aht_solver *s, int32_t steps, int32_t *block) {
   if (eg != hipSuccess) { (void)hipFree(bf); return fail(s, AHT_ENOMEM, "autotune backup: %s", hipGetErrorString(eg)); }
   HIP_TRY(s, hipMemcpyAsync(bf, s->f[p0], fb, hipMemcpyDeviceToDevice, s->st));
   HIP_TRY(s, hipMemcpyAsync(bg, s->gq[p0], gb, hipMemcpyDeviceToDevice, s->st));
-  const int cands[3] = {64, 128, 256};
+  const int cb[3] = {64, 128, 256}, cl[3] = {1, 2, 4}, cw[5] = {1, 2, 4, 16, 0};
   double best_ms = 1e30;
-  int best = s->block;
-  for (int b : cands) {
-    s->block = b;
-    double ms = 0;
-    rc = aht_benchmark(s, steps, &ms);
+  aht_launch keep{s->block, s->levels, (int32_t)s->g.win, 0};
+  for (int b : cb) {
+    for (int l : cl) {
+      for (int w : cw) {
+        apply_launch(s, aht_launch{b, l, w, 0});
+        double ms = 0;
+        rc = aht_benchmark(s, steps, &ms);
+        if (rc) break;
+        if (ms < best_ms) { best_ms = ms; keep = aht_launch{b, l, w, 0}; }
+      }
+      if (rc) break;
+    }
     if (rc) break;
-    if (ms < best_ms) { best_ms = ms; best = b; }
   }
-  s->block = best;
+  apply_launch(s, keep);
   // restore the state and the counters: tuning leaves the simulation where it was
   (void)hipMemcpyAsync(s->f[p0], bf, fb, hipMemcpyDeviceToDevice, s->st);
   (void)hipMemcpyAsync(s->gq[p0], bg, gb, hipMemcpyDeviceToDevice, s->st);
@@ -786,13 +812,22 @@ int aht_autotune(aht_solver *s, int32_t steps, int32_t *block) {
   (void)hipFree(bg);
   s->parity = p0; s->steps = steps0; s->n_acc = nacc0;
   if (rc) return rc;
-  if (block) *block = best;
+  if (best) *best = keep;
   return AHT_OK;
 }
 
-int aht_set_block(aht_solver *s, int32_t block) {
-  if (!s || block < 64 || block > 256 || block % 64) return s ? fail(s, AHT_EINVAL, "block must be 64, 128, 192 or 256") : AHT_EINVAL;
-  s->block = block;
+int aht_set_launch(aht_solver *s, const aht_launch *l) {
+  if (!s) return AHT_EINVAL;
+  if (!l) return fail(s, AHT_EINVAL, "null launch shape");
+  const int rc = check_launch(s, *l);
+  if (rc) return rc;
+  apply_launch(s, *l);
+  return AHT_OK;
+}
+
+int aht_get_launch(const aht_solver *s, aht_launch *l) {
+  if (!s || !l) return AHT_EINVAL;
+  *l = aht_launch{s->block, s->levels, (int32_t)s->g.win, 0};
   return AHT_OK;
 }
 

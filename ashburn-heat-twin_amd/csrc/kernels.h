@@ -14,7 +14,7 @@ struct Grid {
   uint32_t NXY;
   uint32_t N;          // nx * ny * nz (< 2^32; per-q offsets use 64-bit arithmetic)
   float inv_nx;        // 1 / nx, for the column -> (i, j) split
-  int pad;
+  uint32_t win;        // tile order: 1 = linear, W > 1 = windows of W consecutive tiles per XCD, 0 = one slab per XCD
   const uint32_t *colk; // per column: kt | ktop << 10 | kfast << 20 (read by every cell), solver orientation
   const float2 *colcq;  // per column: (cd, q), read only at the first fluid cell
   const uint32_t *ktop;
@@ -53,8 +53,10 @@ uint32_t content_slots(const Grid &g, int block);       // wavefront slots of th
 int storage_bytes(int storage);                         // bytes per population element of an AHT_STORE_* format
 
 // kernels are templated on the storage format; the launchers dispatch on the AHT_STORE_* id
+// levels = cells per thread along z (1, 2 or 4): the tile is (block of columns, chunk of `levels` levels);
+// g.win sets the order in which blocks take tiles
 hipError_t launch_step(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB, const Acc *acc,
-                       float *out4, int block, hipStream_t st);
+                       float *out4, int block, int levels, hipStream_t st);
 // one step of every batch member (launch index `it` of a chunk) on a persistent grid of `grid` blocks of 256
 hipError_t launch_batch_step(int storage, const Member *mem, int nmem, uint32_t total_tiles, int it, int grid,
                              hipStream_t st);

@@ -131,7 +131,8 @@ __device__ __forceinline__ Forces<R> forces(const Params &P, int NZ, int k, int 
 // relaxation (js/live/kernel.js:308-338)
 template <typename R>
 struct Relax {
-  R om45, fp3, fp9, omg, Nxx, Nyy, Nzz, Nxy, Nxz, Nyz, C0;
+  R fp3, fp9, omg, Nxx, Nyy, Nzz, Nxy, Nxz, Nyz, C0;
+  R oa, od, oat, odt, o0t;   // om a_q for a_q = 4.5 w_q of the axis (0.25) and diagonal (0.125) classes, times trN
 };
 
 template <typename R>
@@ -159,7 +160,7 @@ __device__ __forceinline__ Relax<R> relax(const Params &P, int NX, int NY, int i
   if (P.inflow_n == 0) dmin = min(dmin, NY - 1 - j);
   if (dmin < P.nsw) {
     const R rr = (R(P.nsw) - R(dmin)) / R(P.nsw);
-    tb += R(0.15f) * rr * rr;
+    tb += R(0.15) * rr * rr;
   }
   const R tau = R(0.5) * (tb + fsqrt(tb * tb + R(P.cs2smag) * Pn * F.inv_rho));
   const R nusgs = (tau - tb) * third;
@@ -167,20 +168,26 @@ __device__ __forceinline__ Relax<R> relax(const Params &P, int NX, int NY, int i
   const R itau = rcp(tau);
   const R om = R(1) - itau, fpre = R(1) - R(0.5) * itau;
   r.omg = rcp(R(0.5) + R(4) * kap);
-  r.om45 = R(4.5) * om;
   r.fp3 = R(3) * fpre;
   r.fp9 = R(9) * fpre;
   const R usq = ux * ux + uy * uy + uz * uz;
   const R trN = (r.Nxx + r.Nyy + r.Nzz) * third;
   const R uF = ux * F.Fx + uy * F.Fy + uz * F.Fz;
-  // the part of w_q^-1 f_q^post common to every q: rho (1 - 1.5 u^2) - 4.5 om trN - 3 fpre u.F
-  r.C0 = rho * (R(1) - R(1.5) * usq) - r.om45 * trN - r.fp3 * uF;
+  // the part of w_q^-1 (f_q^eq + F_q) common to every q: rho (1 - 1.5 u^2) - 3 fpre u.F
+  r.C0 = rho * (R(1) - R(1.5) * usq) - r.fp3 * uF;
+  // the regularised non-equilibrium part om a_q ((c c):N - trN) carries its own coefficient a_q, the reference's
+  // literal 4.5 w_q (exactly 1.5, 0.25, 0.125), not 4.5 times the rounded weight
+  r.oa = R(0.25) * om;
+  r.od = R(0.125) * om;
+  r.oat = r.oa * trN;
+  r.odt = r.od * trN;
+  r.o0t = R(1.5) * om * trN;
   return r;
 }
 
 // regularised collision with Guo forcing of the pair (q, q+1), q odd (js/live/kernel.js:340-367):
-// f_q^post / w_q = C0 + 3 rho cu + 4.5 rho cu^2 + 4.5 om (c c):N + fpre (3 cF + 9 cu cF), split into the part
-// even in c (E) and the odd part (O); the opposite population gets E - O.
+// f_q^post = w_q (C0 + 3 rho cu + 4.5 rho cu^2 + fpre (3 cF + 9 cu cF)) + om a_q ((c c):N - trN); the part of the
+// bracket even in c (E) and the odd part (O) are formed once, the opposite population gets E - O.
 template <int q, typename T>
 __device__ __forceinline__ void collide_pair(const Moments<T> &m, const Forces<T> &F, const Relax<T> &R, T &fq, T &fo) {
   constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
@@ -193,15 +200,16 @@ __device__ __forceinline__ void collide_pair(const Moments<T> &m, const Forces<T
   if (cx && cy) qn += T(2 * cx * cy) * R.Nxy;
   if (cx && cz) qn += T(2 * cx * cz) * R.Nxz;
   if (cy && cz) qn += T(2 * cy * cz) * R.Nyz;
-  const T E = R.C0 + cu * (T(4.5) * m.rho * cu + R.fp9 * cF) + R.om45 * qn;
+  const T E = R.C0 + cu * (T(4.5) * m.rho * cu + R.fp9 * cF);
   const T O = T(3) * m.rho * cu + R.fp3 * cF;
+  const T t = q < 7 ? R.oa * qn - R.oat : R.od * qn - R.odt;
   const T w = T(w19<q>());
-  fq = w * (E + O);
-  fo = w * (E - O);
+  fq = w * (E + O) + t;
+  fo = w * (E - O) + t;
 }
 
 template <typename T>
-__device__ __forceinline__ T collide_rest(const Relax<T> &R) { return T(w19<0>()) * R.C0; }
+__device__ __forceinline__ T collide_rest(const Relax<T> &R) { return T(w19<0>()) * R.C0 - R.o0t; }
 
 // D3Q7 collision with the rooftop source and the top-sponge damping (js/live/kernel.js:369-386):
 // g^post = ((1 - omg) g + omg w th (1 + 4 c.u) + w S) (1 - sg)

@@ -49,11 +49,21 @@ __device__ __forceinline__ void split_column(const Grid &g, int c, int &i, int &
   i = ii; j = jj;
 }
 
-// XCD-aware tile order: block b runs on XCD b % 8; give every XCD a contiguous range of tiles so the cache lines that
-// neighbouring tiles share (the +-1 cell and +-1 row shifted pulls) are fetched once into that XCD's L2
-__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t G) {
-  const uint32_t q = G / NUM_XCD, r = G % NUM_XCD, x = b % NUM_XCD, i = b / NUM_XCD;
-  return x * q + min(x, r) + i;
+// Tile order. Workgroup b runs on XCD b % 8 and is the (b / 8)-th one there; `win` picks the tile it takes:
+//   win == 1: tile b (linear; the eight XCDs share one front moving through memory);
+//   win >= 2: windows of 8 * win tiles, XCD x taking the x-th run of `win` consecutive tiles of each window, so the
+//             lines that neighbouring tiles share (the +-1 cell and +-1 row pulls) mostly meet in one L2 while the
+//             front stays compact; the last partial window keeps the linear order;
+//   win == 0: XCD x takes the x-th contiguous eighth of all tiles (eight fronts).
+__device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t G, uint32_t win) {
+  const uint32_t x = b % NUM_XCD, i = b / NUM_XCD;
+  if (win == 0) {
+    const uint32_t q = G / NUM_XCD, r = G % NUM_XCD;
+    return x * q + min(x, r) + i;
+  }
+  const uint32_t span = win * NUM_XCD;
+  if (win == 1 || b >= (G / span) * span) return b;
+  return (i / win) * span + x * win + i % win;
 }
 
 // packed column word: kt | ktop << 10 | kfast << 20 (nz < 1023)
@@ -296,14 +306,11 @@ __device__ __forceinline__ void stream_cell(const Grid &g, const void *fA, const
 // (OUT). Every thread of the block must call it (the ACC heat-content sum is a wavefront reduction); `slot` is the
 // wavefront's heat-content slot.
 template <class S, bool ACC, bool OUT>
-__device__ __forceinline__ void step_cell(const Grid &g, const void *fA, void *fB, const void *gA, void *gB,
-                                          const Acc &acc, float *__restrict__ out4, int c, int k, uint32_t slot) {
+__device__ __forceinline__ double step_cell_body(const Grid &g, const void *fA, void *fB, const void *gA, void *gB,
+                                                 const Acc &acc, float *__restrict__ out4, int c, int k, uint32_t ck) {
   using R = typename S::real;
-  const bool live = c < (int)g.NXY;
-  const uint32_t ck = g.colk[live ? c : 0];
   const int ktc = col_kt(ck), kg = col_ktop(ck);
-  const bool fluid = live && k >= kg;
-  if (!ACC && !fluid) return;
+  const bool fluid = c < (int)g.NXY && k >= kg && k < g.nz;
   double th_cell = 0.0;
   if (fluid) {
     const size_t N = g.N;
@@ -369,23 +376,40 @@ __device__ __forceinline__ void step_cell(const Grid &g, const void *fA, void *f
       }
     }
   }
-  if (ACC) {
-    // heat content: one slot per wavefront, each updated by that wavefront only (deterministic, no atomics)
-    const double sum = wave_sum(th_cell);
-    if ((threadIdx.x & 63) == 0) acc.content[slot] += sum;
-  }
+  return th_cell;
 }
 
-// one block per (column block, level) tile, tiles numbered column block fastest, then level, in XCD-aware order
+// heat content: one slot per wavefront, each updated by that wavefront only (deterministic, no atomics)
+__device__ __forceinline__ void content_add(const Acc &acc, uint32_t slot, double th) {
+  const double sum = wave_sum(th);
+  if ((threadIdx.x & 63) == 0) acc.content[slot] += sum;
+}
+
 template <class S, bool ACC, bool OUT>
+__device__ __forceinline__ void step_cell(const Grid &g, const void *fA, void *fB, const void *gA, void *gB,
+                                          const Acc &acc, float *__restrict__ out4, int c, int k, uint32_t slot) {
+  const uint32_t ck = g.colk[c < (int)g.NXY ? c : 0];
+  const double th = step_cell_body<S, ACC, OUT>(g, fA, fB, gA, gB, acc, out4, c, k, ck);
+  if (ACC) content_add(acc, slot, th);
+}
+
+// one block per (column block, KB-level chunk) tile, numbered column block fastest, then level chunk, taken in the
+// order g.win selects; a thread walks its column through the KB levels (column word and index math once per column)
+template <class S, bool ACC, bool OUT, int KB>
 __global__ __launch_bounds__(256) void step_kernel(Grid g, const void *fA, void *fB, const void *gA, void *gB, Acc acc,
                                                    float *__restrict__ out4, uint32_t bx) {
-  const uint32_t t = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t t = tile_of_block(blockIdx.x, gridDim.x, g.win);
   const uint32_t kk = t / bx, cb = t - kk * bx;
   const int c = (int)(cb * blockDim.x + threadIdx.x);
-  const int k = g.kmin + (int)kk;
-  const uint32_t slot = t * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  step_cell<S, ACC, OUT>(g, fA, fB, gA, gB, acc, out4, c, k, slot);
+  const int k0 = g.kmin + (int)kk * KB;
+  const uint32_t ck = g.colk[c < (int)g.NXY ? c : 0];
+  double th = 0.0;
+  if (KB == 1) {
+    th = step_cell_body<S, ACC, OUT>(g, fA, fB, gA, gB, acc, out4, c, k0, ck);
+  } else {
+    for (int u = 0; u < KB; u++) th += step_cell_body<S, ACC, OUT>(g, fA, fB, gA, gB, acc, out4, c, k0 + u, ck);
+  }
+  if (ACC) content_add(acc, t * (blockDim.x >> 6) + (threadIdx.x >> 6), th);
 }
 
 // ---- many domains in one launch ----------------------------------------------------------------------------------
@@ -674,17 +698,26 @@ int storage_bytes(int storage) { return storage == 0 ? 4 : (storage == 3 ? 8 : 2
     default: return hipErrorInvalidValue;             \
   }
 
-hipError_t launch_step(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB, const Acc *acc,
-                       float *out4, int block, hipStream_t st) {
-  if (g.kmin >= g.nz) return hipSuccess;
+template <class S, int KB>
+static void launch_step_kb(const Grid &g, const void *fA, void *fB, const void *gA, void *gB, const Acc *acc,
+                           float *out4, int block, hipStream_t st) {
   const uint32_t bx = (g.NXY + block - 1) / block;
-  const dim3 grid(bx * (uint32_t)(g.nz - g.kmin));
+  const uint32_t nk = (uint32_t)(g.nz - g.kmin + KB - 1) / KB;
+  const dim3 grid(bx * nk);
   const Acc a = acc ? *acc : Acc{nullptr, nullptr, nullptr};
+  if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
+  else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
+  else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
+  else hipLaunchKernelGGL((step_kernel<S, false, false, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
+}
+
+hipError_t launch_step(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB, const Acc *acc,
+                       float *out4, int block, int levels, hipStream_t st) {
+  if (g.kmin >= g.nz) return hipSuccess;
   AHT_DISPATCH(storage, {
-    if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
-    else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
-    else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
-    else hipLaunchKernelGGL((step_kernel<S, false, false>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
+    if (levels == 4) (launch_step_kb<S, 4>)(g, fA, fB, gA, gB, acc, out4, block, st);
+    else if (levels == 2) (launch_step_kb<S, 2>)(g, fA, fB, gA, gB, acc, out4, block, st);
+    else (launch_step_kb<S, 1>)(g, fA, fB, gA, gB, acc, out4, block, st);
   })
   return hipGetLastError();
 }

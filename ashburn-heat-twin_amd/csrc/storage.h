@@ -11,6 +11,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// cache-policy operand of the population buffer loads and stores (gfx950 encoding: 1 = sc0, 2 = nt, 16 = sc1);
+// a build option, 0 = default policy
+#ifndef AHT_LD_CPOL
+#define AHT_LD_CPOL 0
+#endif
+#ifndef AHT_ST_CPOL
+#define AHT_ST_CPOL 0
+#endif
+
 namespace aht {
 
 constexpr float DEV_SCALE = 1024.0f;
@@ -36,10 +45,10 @@ struct StoreF32 {
   __device__ static real dec_g(elem e) { return e; }
   __device__ static elem enc_g(real v) { return v; }
   __device__ static elem ld_raw(const void *base, uint32_t boff) {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc(base), boff, 0, 0));
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc(base), boff, 0, AHT_LD_CPOL));
   }
   __device__ static void st_raw(void *base, uint32_t boff, elem v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rsrc(base), boff, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rsrc(base), boff, 0, AHT_ST_CPOL);
   }
 };
 
@@ -53,10 +62,10 @@ struct StoreF16 {
   __device__ static real dec_g(elem e) { return (float)e * DEV_INV; }
   __device__ static elem enc_g(real v) { return (elem)(v * DEV_SCALE); }
   __device__ static elem ld_raw(const void *base, uint32_t boff) {
-    return __builtin_bit_cast(_Float16, (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc(base), boff, 0, 0));
+    return __builtin_bit_cast(_Float16, (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc(base), boff, 0, AHT_LD_CPOL));
   }
   __device__ static void st_raw(void *base, uint32_t boff, elem v) {
-    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), rsrc(base), boff, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), rsrc(base), boff, 0, AHT_ST_CPOL);
   }
 };
 
@@ -74,10 +83,10 @@ struct StoreBF16 {
   __device__ static real dec_g(elem e) { return bf16_to_f32(e.bits) * DEV_INV; }
   __device__ static elem enc_g(real v) { return bf16_t{f32_to_bf16(v * DEV_SCALE)}; }
   __device__ static elem ld_raw(const void *base, uint32_t boff) {
-    return bf16_t{(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc(base), boff, 0, 0)};
+    return bf16_t{(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc(base), boff, 0, AHT_LD_CPOL)};
   }
   __device__ static void st_raw(void *base, uint32_t boff, elem v) {
-    __builtin_amdgcn_raw_buffer_store_b16(v.bits, rsrc(base), boff, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b16(v.bits, rsrc(base), boff, 0, AHT_ST_CPOL);
   }
 };
 
@@ -91,11 +100,11 @@ struct StoreF64 {
   __device__ static real dec_g(elem e) { return e; }
   __device__ static elem enc_g(real v) { return v; }
   __device__ static elem ld_raw(const void *base, uint32_t boff) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc(base), boff, 0, 0));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc(base), boff, 0, AHT_LD_CPOL));
   }
   __device__ static void st_raw(void *base, uint32_t boff, elem v) {
-    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(rsrc(base), boff, 0, 0));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), rsrc(base), boff, 0, 0);
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(rsrc(base), boff, 0, AHT_LD_CPOL));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), rsrc(base), boff, 0, AHT_ST_CPOL);
   }
 };
 

@@ -84,6 +84,16 @@ function createSolver(A, job, o) {
 }
 
 const sleep = ms => new Promise(r => setTimeout(r, ms));
+// progress on stderr at most every 15 s (long runs must not look idle)
+function progress(o, mode) {
+  let last = Date.now();
+  return (done, total, active, queued) => {
+    if (!o.quiet && Date.now() - last > 15000) {
+      last = Date.now();
+      console.error(`[serve ${mode} rate ${o.rate}] ${done}/${total} done, ${active} active, ${queued} queued`);
+    }
+  };
+}
 const pct = (a, p) => { const s = a.slice().sort((x, y) => x - y); return s[Math.min(s.length - 1, Math.floor(p * (s.length - 1) + 0.5))]; };
 
 function summary(mode, o, results, t0, t1, peakBytes, busyMs) {
@@ -109,7 +119,9 @@ async function serveBatched(reqs, o) {
   let used = 0, peak = 0, busy = 0;
   const t0 = Date.now() / 1e3;
   const now = () => Date.now() / 1e3 - t0;
+  const prog = progress(o, 'batch');
   while (future.length || pending.length || active.length) {
+    prog(results.length, reqs.length, active.length, pending.length);
     while (future.length && future[0].arrival <= now()) pending.push(future.shift());
     // admission: FIFO, by footprint and slots
     while (pending.length && active.length < slots) {
@@ -166,7 +178,9 @@ async function serveSequential(reqs, o) {
   let peak = 0, busy = 0;
   const t0 = Date.now() / 1e3;
   const now = () => Date.now() / 1e3 - t0;
+  const prog = progress(o, 'sequential');
   for (const r of queue) {
+    prog(results.length, reqs.length, 1, 0);
     if (r.arrival > now()) await sleep((r.arrival - now()) * 1e3);
     const job = buildScenario(r, o);
     job.keepSurf = !!o.keepSurf;
@@ -182,6 +196,7 @@ async function serveSequential(reqs, o) {
         busy += Date.now() - t;
       }
       done += n;
+      prog(results.length, reqs.length, 1, 0);
     }
     results.push(result(r, job, A, h, now()));
     A.destroy(h);

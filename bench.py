@@ -90,7 +90,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--workload', default='c2')
     ap.add_argument('--storage', type=int, default=0, help='0 f32, 1 f16 deviations, 2 bf16 deviations, 3 f64 twin')
-    ap.add_argument('--block', type=int, default=0)
+    ap.add_argument('--launch', default=None, help='fixed launch shape block,levels,window (no tuning)')
+    ap.add_argument('--tune-steps', type=int, default=4, help='steps per launch-shape candidate of the autotuner')
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--dump', default=None, help='existing domain dump directory')
@@ -113,7 +114,13 @@ def main():
     N = nx * ny * nz
     nf = int(sum(nz - int(k) for k in dom['ktop']))
     log(f'grid {nx}x{ny}x{nz} cells {N} fluid {nf}')
-    s = m.Solver.from_dump(dom, storage=a.storage, block=a.block)
+    s = m.Solver.from_dump(dom, storage=a.storage)
+    if a.launch:
+        b, l, w = (int(x) for x in a.launch.split(','))
+        s.set_launch(b, l, w)
+    else:
+        s.step(2)
+        log('autotuned launch shape', s.autotune(a.tune_steps))
     s.step(a.warmup)
     s.sync()
     t0 = time.perf_counter()
@@ -121,6 +128,7 @@ def main():
     s.sync()
     wall = time.perf_counter() - t0
     st = s.stats()
+    launch = s.launch
     s.close()
     ms_step = ms / a.steps
     mlups = N * a.steps / (ms * 1e3)
@@ -146,7 +154,7 @@ def main():
         'data': 'synthetic (seeded generator: hills, box buildings, source roofs)',
         'config': {'workload': f'{a.workload}: {nx}x{ny}x{nz} at {dom["dx"]} m, fused D3Q19+D3Q7 step',
                    'cells': N, 'fluid_cells': nf, 'bytes_per_fluid_cell': bpc, 'dt_s': dom['dt'],
-                   'storage': STORAGE[a.storage], 'parallelism': 'single GPU'},
+                   'storage': STORAGE[a.storage], 'launch': launch, 'parallelism': 'single GPU'},
         'fluid_mlups': round(nf * a.steps / (ms * 1e3), 2),
         'wall_ms_per_step': round(wall * 1e3 / a.steps, 5),
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',

@@ -151,10 +151,20 @@ int aht_batch_sync(aht_batch *b);
 int aht_batch_set_grid(aht_batch *b, int32_t blocks, int32_t use_graph);
 int aht_batch_grid(const aht_batch *b);              /* chosen grid (0 while still tuning)                 */
 
-/* block-size autotuner of one solver: times 64/128/256-thread blocks over `steps` steps each, keeps the fastest, and
- * restores the state and counters (no counterpart in the reference) */
-int aht_autotune(aht_solver *s, int32_t steps, int32_t *block);
-int aht_set_block(aht_solver *s, int32_t block);
+/* launch shape of one solver's step kernel (no counterpart in the reference: its WGSL kernel has one fixed shape) */
+typedef struct {
+  int32_t block;         /* threads per block: 64, 128, 192 or 256                                        */
+  int32_t levels;        /* cells per thread along z: 1, 2 or 4                                           */
+  int32_t window;        /* tile order: 1 linear, W in 2..4096 = runs of W consecutive tiles per XCD,
+                            0 = one contiguous slab of tiles per XCD                                      */
+  int32_t reserved;
+} aht_launch;
+
+/* launch-shape autotuner of one solver: times every block x levels x window candidate over `steps` steps each,
+ * keeps the fastest (written to *best, may be NULL), and restores the state and counters */
+int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best);
+int aht_set_launch(aht_solver *s, const aht_launch *l);
+int aht_get_launch(const aht_solver *s, aht_launch *l);
 
 /* device bytes held by the solver (admission by footprint in the serving loop) */
 int64_t aht_footprint_bytes(const aht_dims *dims);

@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Writes the long-run known answers of a synthetic workload from the CPU oracle (f32 build, the reference kernel's
+expressions and rounding): tests/golden/long_<case>.json.
+
+The run is the workload's own protocol: `steps` steps from rest, the last `mean_steps` of them accumulating the
+time means the reference's validation accumulates (tools/live/validate_live.py:110-129, :216-240) — near-ground
+theta/u/v at the first fluid cell of every column, theta and the normal velocity on the four lateral faces, and the
+fluid-volume heat content — in float64. From those it records what the GPU test compares:
+  * the 10-minute mean near-ground warming (spatial mean, max and its column, 99th percentile) and a subsample of the
+    mean field (every `--every`-th column, north-up order, as the GPU's aht_read_mean returns it);
+  * the heat budget of the mean fields in lattice units (the C-ABI's aht_budget definitions) and in MW / MJ;
+  * a fingerprint of the domain (column sums) so the test knows it rebuilt the identical inputs.
+
+Usage: python tests/golden/make_long_reference.py --case c2s [--threads 6] [--ckpt DIR] [--ckpt-every 100]
+The 600 x 600 x 64 case takes about an hour on 8 cores; --ckpt resumes an interrupted run.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+
+PROTOCOL = {  # case: (workload, steps, mean_steps)
+    'c1': ('c1', 2000 + 1719, 1719),   # the 200 x 200 x 32 config: 2,000 steps, then the 10-minute mean (dt 0.349 s)
+    'c2s': ('c2', 1300, 300),          # the 600 x 600 x 64 domain, shortened (the full 20,000 steps take ~14 h here)
+}
+
+
+def log(*a):
+    print(time.strftime('%H:%M:%S'), *a, file=sys.stderr, flush=True)
+
+
+def fingerprint(dom):
+    return {'kt_sum': int(dom['kt'].astype(np.int64).sum()), 'ktop_sum': int(dom['ktop'].astype(np.int64).sum()),
+            'q_sum': float(dom['q'].astype(np.float64).sum()), 'cd_sum': float(dom['cd'].astype(np.float64).sum()),
+            'dt': dom['dt'], 'dx': dom['dx'], 'nx': dom['nx'], 'ny': dom['ny'], 'nz': dom['nz']}
+
+
+class Means:
+    """float64 sums of the accumulated fields over the averaging steps"""
+
+    def __init__(self, nx, ny, nz):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.surf = np.zeros((3, ny * nx))
+        self.face = {f: np.zeros((2, nz, ny if f in 'WE' else nx)) for f in 'WESN'}
+        self.content = 0.0
+        self.n = 0
+
+    def add(self, out, ktop, fluid):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        th, ux, uy = (out[c].reshape(nz, ny, nx) for c in range(3))
+        cols = np.arange(nx * ny)
+        kk = ktop.astype(np.int64)
+        for c, a in enumerate((th, ux, uy)):
+            self.surf[c] += a.reshape(nz, -1)[kk, cols]
+        fl = fluid.reshape(nz, ny, nx)
+        z = lambda a: np.where(fl, a, 0.0)
+        self.face['W'] += np.stack([z(th)[:, :, 0], z(ux)[:, :, 0]])
+        self.face['E'] += np.stack([z(th)[:, :, -1], z(ux)[:, :, -1]])
+        self.face['S'] += np.stack([z(th)[:, 0, :], z(uy)[:, 0, :]])
+        self.face['N'] += np.stack([z(th)[:, -1, :], z(uy)[:, -1, :]])
+        self.content += float(np.where(fluid, out[0], 0).astype(np.float64).sum())
+        self.n += 1
+
+    def state(self):
+        return {'surf': self.surf, 'content': self.content, 'n': self.n,
+                **{f'face_{f}': v for f, v in self.face.items()}}
+
+    def load(self, d):
+        self.surf = d['surf']
+        self.content = float(d['content'])
+        self.n = int(d['n'])
+        for f in 'WESN':
+            self.face[f] = d[f'face_{f}']
+
+
+def budget(M, q_cols):
+    """aht_read_mean's aht_budget from the sums (include/aht.h)"""
+    inv = 1.0 / M.n
+    hf, qf = {}, {}
+    for f in 'WESN':
+        th, un = M.face[f][0] * inv, M.face[f][1] * inv
+        hf[f] = float((th * un).sum())
+        qf[f] = float(un.sum())
+    thru = sum(abs(v) for v in qf.values())
+    return {'n_acc': M.n, 'heat_in': float(np.asarray(q_cols, np.float64).sum()), 'content': M.content * inv,
+            'heat_out': hf['E'] - hf['W'] + hf['N'] - hf['S'],
+            'q_w': qf['W'], 'q_e': qf['E'], 'q_s': qf['S'], 'q_n': qf['N'],
+            'mass_imbalance': abs(qf['W'] - qf['E'] + qf['S'] - qf['N']) / max(thru, 1e-300)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--case', default='c1', choices=sorted(PROTOCOL))
+    ap.add_argument('--threads', type=int, default=0)
+    ap.add_argument('--ckpt', default=None)
+    ap.add_argument('--ckpt-every', type=int, default=1000)
+    ap.add_argument('--every', type=int, default=16, help='column stride of the recorded mean-field subsample')
+    ap.add_argument('--out', default=None)
+    a = ap.parse_args()
+    if a.threads:
+        os.environ['OMP_NUM_THREADS'] = str(a.threads)
+    import oracle as O   # after OMP_NUM_THREADS is set
+    workload, steps, mean_steps = PROTOCOL[a.case]
+    ddir = tempfile.mkdtemp(prefix='aht_long_')
+    subprocess.run(['node', os.path.join(ROOT, 'ashburn-heat-twin_amd', 'js', 'dump.js'), '--workload', workload,
+                    '--out', ddir], check=True, stdout=subprocess.DEVNULL)
+    dom = O.load_dump(ddir)
+    subprocess.run(['rm', '-rf', ddir])
+    nx, ny, nz = dom['nx'], dom['ny'], dom['nz']
+    o = O.Oracle(dom, 'f32')
+    o.init_flow()
+    fluid = o.fluid_mask()
+    M = Means(nx, ny, nz)
+    spin = steps - mean_steps
+    ck = os.path.join(a.ckpt, f'{a.case}.npz') if a.ckpt else None
+    t_run = 0.0
+    if ck and os.path.exists(ck):
+        d = np.load(ck)
+        o.f[:] = d['f']
+        o.g[:] = d['g']
+        o.steps = int(d['steps'])
+        t_run = float(d['t_run'])
+        if o.steps > spin:
+            M.load(d)
+        log(f'resumed at step {o.steps}')
+    log(f'{a.case}: {nx}x{ny}x{nz}, {steps} steps, mean over the last {mean_steps}, '
+        f'{O.lib().oracle_num_threads()} threads')
+    while o.steps < steps:
+        t0 = time.time()
+        if o.steps < spin:
+            n = min(a.ckpt_every, spin - o.steps)
+            o.step(n)
+        else:
+            n = min(a.ckpt_every, steps - o.steps)
+            for _ in range(n):
+                if o.steps == spin:
+                    M = Means(nx, ny, nz)
+                out = o.step(1, out=True)
+                M.add(out, o._cols[1], fluid)
+        t_run += time.time() - t0
+        log(f'step {o.steps}/{steps} ({t_run / o.steps:.3f} s/step)')
+        if ck and o.steps < steps:
+            os.makedirs(a.ckpt, exist_ok=True)
+            np.savez(ck + '.tmp.npz', f=o.f, g=o.g, steps=o.steps, t_run=t_run,
+                     **(M.state() if o.steps > spin else {}))
+            os.replace(ck + '.tmp.npz', ck)
+    st = o.stats()
+    b = budget(M, dom['q'])
+    inv = 1.0 / M.n
+    # surface means north-up, as aht_read_mean returns them
+    surf = (M.surf * inv).reshape(3, ny, nx)[:, ::-1, :].reshape(3, -1)
+    th = surf[0]
+    dx, dt, conv = dom['dx'], dom['dt'], dom['conv']
+    rho_cp = 1.16 * 1005.0
+    rec = {
+        'case': a.case, 'workload': workload, 'generator': 'oracle f32 (oracle/lbm_oracle_body.h)', 'steps': steps,
+        'mean_steps': mean_steps, 'domain': fingerprint(dom), 'conv': conv, 'threads': int(O.lib().oracle_num_threads()),
+        'cpu_s_per_step': t_run / steps,
+        'warming': {'mean': float(th.mean()), 'max': float(th.max()), 'argmax': int(th.argmax()),
+                    'p99': float(np.percentile(th, 99))},
+        'warming_subsample': {'every': a.every, 'theta': [float(x) for x in th[::a.every]]},
+        'budget': b,
+        'budget_physical': {'heat_injected_MW': b['heat_in'] * rho_cp * dx ** 3 / dt / 1e6,
+                            'heat_advected_out_MW': rho_cp * b['heat_out'] * conv * dx * dx / 1e6,
+                            'heat_content_MJ': b['content'] * rho_cp * dx ** 3 / 1e6,
+                            'mass_imbalance_frac': b['mass_imbalance']},
+        'final_stats': {k: (int(v) if k == 'non_finite' else float(v)) for k, v in st.items()},
+    }
+    out = a.out or os.path.join(HERE, f'long_{a.case}.json')
+    with open(out, 'w') as fh:
+        json.dump(rec, fh)
+    log('wrote', out, json.dumps(rec['warming']), json.dumps(rec['budget_physical']))
+    if ck and os.path.exists(ck):
+        os.remove(ck)
+
+
+if __name__ == '__main__':
+    main()

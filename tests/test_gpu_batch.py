@@ -96,6 +96,21 @@ def test_autotune_restores_state(tmp_path, pkg):
         s.step(5)
         f0, g0 = s.read_state()
         best = s.autotune(3)
-        assert best in (64, 128, 256)
+        assert best['block'] in (64, 128, 256) and best['levels'] in (1, 2, 4) and best['window'] in (0, 1, 2, 4, 16)
+        assert s.launch == best
         f1, g1 = s.read_state()
         assert np.array_equal(f0, f1) and np.array_equal(g0, g1) and s.steps == 5
+
+
+@pytest.mark.parametrize('block,levels,window', [(256, 1, 1), (128, 2, 4), (64, 4, 0), (192, 1, 16), (256, 4, 2)])
+def test_launch_shapes_give_identical_states(tmp_path, pkg, block, levels, window):
+    """every launch shape updates every cell with the same arithmetic: states are bit-identical to the default"""
+    dom = domains(tmp_path)[2]
+    with pkg.Solver.from_dump(dom) as a, pkg.Solver.from_dump(dom) as b:
+        b.set_launch(block, levels, window)
+        a.step(13)
+        b.step(13)
+        fa, ga = a.read_state()
+        fb, gb = b.read_state()
+        assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32))
+        assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32))
