@@ -415,6 +415,56 @@ static napi_value js_footprint(napi_env env, napi_callback_info info) {
   return r;
 }
 
+static napi_value launch_obj(napi_env env, const aht_launch *l) {
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "block", l->block);
+  set_num(env, o, "levels", l->levels);
+  set_num(env, o, "window", l->window);
+  return o;
+}
+
+/* autotune(h, steps) -> {block, levels, window}: times every launch-shape candidate for `steps` steps, keeps the
+ * fastest, leaves the state and step counters as they were (blocks the calling thread for the tuning steps) */
+static napi_value js_autotune(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  int32_t steps;
+  if (!h || !get_i32(env, argv[1], &steps)) return NULL;
+  aht_launch best;
+  const int rc = aht_autotune(h->s, steps, &best);
+  if (rc) return throw_rc(env, h->s, rc);
+  return launch_obj(env, &best);
+}
+
+/* setLaunch(h, {block, levels, window}); getLaunch(h) -> {block, levels, window} */
+static napi_value js_set_launch(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  aht_launch l;
+  memset(&l, 0, sizeof l);
+  if (!get_prop_i32(env, argv[1], "block", &l.block, 256) || !get_prop_i32(env, argv[1], "levels", &l.levels, 1) ||
+      !get_prop_i32(env, argv[1], "window", &l.window, 1))
+    return NULL;
+  const int rc = aht_set_launch(h->s, &l);
+  if (rc) return throw_rc(env, h->s, rc);
+  return undef(env);
+}
+
+static napi_value js_get_launch(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  handle_t *h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  aht_launch l;
+  const int rc = aht_get_launch(h->s, &l);
+  if (rc) return throw_rc(env, h->s, rc);
+  return launch_obj(env, &l);
+}
+
 static napi_value js_steps_done(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
@@ -594,6 +644,7 @@ static napi_value init(napi_env env, napi_value exports) {
     {"readVolume", js_read_volume}, {"stats", js_stats}, {"meanReset", js_mean_reset}, {"readMean", js_read_mean},
     {"readState", js_read_state}, {"writeState", js_write_state}, {"readOut", js_read_out},
     {"footprint", js_footprint}, {"stepsDone", js_steps_done},
+    {"autotune", js_autotune}, {"setLaunch", js_set_launch}, {"getLaunch", js_get_launch},
     {"batchCreate", js_batch_create}, {"batchDestroy", js_batch_destroy}, {"batchAdd", js_batch_add},
     {"batchRemove", js_batch_remove}, {"batchSetGrid", js_batch_set_grid}, {"batchGrid", js_batch_grid},
     {"batchStep", js_batch_step},

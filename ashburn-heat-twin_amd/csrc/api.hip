@@ -26,6 +26,7 @@ struct aht_solver {
   int levels = 1;             // cells per thread along z
   int storage = AHT_STORE_F32;
   int eb = 4;                 // bytes per stored population element
+  size_t gbytes = 0;          // bytes of one D3Q7 lattice (theta-referenced formats add the f32 theta plane)
   void *f[2] = {nullptr, nullptr}, *gq[2] = {nullptr, nullptr};
   void *stage = nullptr;      // one population plane in f64, for state conversion
   uint32_t *colk = nullptr;
@@ -117,7 +118,7 @@ int64_t aht_footprint_bytes(const aht_dims *d) {
   if (!d || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return -1;
   const int64_t N = (int64_t)d->nx * d->ny * d->nz, NXY = (int64_t)d->nx * d->ny;
   const int eb = d->storage == AHT_STORE_F32 ? 4 : (d->storage == AHT_STORE_F64 ? 8 : 2);
-  const int64_t pop = 2 * 26 * N * eb;
+  const int64_t pop = 2 * (19 * N * eb + (int64_t)aht::g_lattice_bytes(d->storage, (size_t)N));
   const int64_t cols = NXY * (4 + 8 + 4 + 4) + (int64_t)d->nz * 16;
   const int64_t acc = 3 * NXY * 8 + (int64_t)face_total(d->nx, d->ny, d->nz) * 8 + (int64_t)max_content_slots(*d) * 8;
   const int64_t misc = ((N + 255) / 256) * 64 + 3 * NXY * 4 + (N + 3) / 4 * 4 / 1 + 4096;
@@ -131,13 +132,17 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   if (d.nx < 2 || d.ny < 2 || d.nz < 2 || d.nz > 1022) return AHT_EINVAL;
   const uint64_t N = (uint64_t)d.nx * d.ny * d.nz;
   if (d.storage < AHT_STORE_F32 || d.storage > AHT_STORE_F64) return AHT_EINVAL;
-  if (N * (uint64_t)aht::storage_bytes(d.storage) >= (1ull << 32)) return AHT_EINVAL;   // byte offsets stay 32-bit
+  // byte offsets within one population plane and within a D3Q7 lattice (theta plane) stay 32-bit
+  if (N * (uint64_t)aht::storage_bytes(d.storage) >= (1ull << 32)) return AHT_EINVAL;
+  if (aht::storage_theta_ref(d.storage) && (uint64_t)aht::g_lattice_bytes(d.storage, (size_t)N) >= (1ull << 32))
+    return AHT_EINVAL;
   const int block = d.block ? d.block : aht::STEP_BLOCK;
   if (block < 64 || block > 256 || block % 64) return AHT_EINVAL;
   aht_solver *s = new aht_solver();
   s->dims = d;
   s->storage = d.storage;
   s->eb = aht::storage_bytes(d.storage);
+  s->gbytes = aht::g_lattice_bytes(d.storage, (size_t)N);
   s->device = d.device;
   s->block = block;
   int ndev = 0;
@@ -159,11 +164,12 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   g.nx = d.nx; g.ny = d.ny; g.nz = d.nz; g.kmin = d.nz; g.NXY = NXY; g.N = (uint32_t)N;
   g.inv_nx = 1.0f / (float)d.nx;
   g.win = 1;
+  g.th_off = (uint32_t)aht::theta_offset(d.storage, (size_t)N);
   for (int p = 0; p < 2; p++) {
     HIP_TRY(s, hipMalloc(&s->f[p], 19 * N * s->eb));
-    HIP_TRY(s, hipMalloc(&s->gq[p], 7 * N * s->eb));
+    HIP_TRY(s, hipMalloc(&s->gq[p], s->gbytes));
     HIP_TRY(s, hipMemsetAsync(s->f[p], 0, 19 * N * s->eb, s->st));
-    HIP_TRY(s, hipMemsetAsync(s->gq[p], 0, 7 * N * s->eb, s->st));
+    HIP_TRY(s, hipMemsetAsync(s->gq[p], 0, s->gbytes, s->st));
   }
   HIP_TRY(s, hipMalloc(&s->colk, NXY * sizeof(uint32_t)));
   HIP_TRY(s, hipMalloc(&s->colcq, NXY * sizeof(float2)));
@@ -238,6 +244,7 @@ int aht_set_columns(aht_solver *s, const uint32_t *kt, const uint32_t *ktop, con
   HIP_TRY(s, hipMemcpyAsync(s->ktop, ktop, NXY * sizeof(uint32_t), hipMemcpyHostToDevice, s->st));
   HIP_TRY(s, hipStreamSynchronize(s->st));
   s->g.kmin = kmin;
+
   s->h_q.assign(q, q + NXY);
   s->has_cols = true;
   return AHT_OK;
@@ -271,7 +278,7 @@ int aht_reset_heat(aht_solver *s) {
   if (!s) return AHT_EINVAL;
   int rc = set_device(s);
   if (rc) return rc;
-  const size_t b = 7 * (size_t)s->g.N * s->eb;      // zero bits are 0.0 in every storage format
+  const size_t b = s->gbytes;                        // zero bits are 0.0 in every storage format (theta too)
   HIP_TRY(s, hipMemsetAsync(s->gq[0], 0, b, s->st));
   HIP_TRY(s, hipMemsetAsync(s->gq[1], 0, b, s->st));
   return AHT_OK;
@@ -487,19 +494,34 @@ static int state_io(aht_solver *s, void *f, void *g, int f64, int write) {
   if (!s->stage) HIP_TRY(s, hipMalloc(&s->stage, N * 8));
   const float w0 = (float)(1.0 / 3.0), w1 = (float)(1.0 / 18.0), w2 = (float)(1.0 / 36.0);
   const float W19[19] = {w0, w1, w1, w1, w1, w1, w1, w2, w2, w2, w2, w2, w2, w2, w2, w2, w2, w2, w2};
+  const float W7[7] = {0.25f, 0.125f, 0.125f, 0.125f, 0.125f, 0.125f, 0.125f};
+  const bool thref = aht::storage_theta_ref(s->storage);
   for (int is_f = 1; is_f >= 0; is_f--) {
     char *host = (char *)(is_f ? f : g);
     if (!host) continue;
     char *dev = (char *)(is_f ? s->f[p] : s->gq[p]);
+    const float *th = (!is_f && thref) ? (const float *)(dev + s->g.th_off) : nullptr;
+    if (th && write) {
+      // theta plane of the lattice: theta = sum_q g_q of the state being written, then the deviations against it
+      std::vector<float> t(N);
+      for (size_t n = 0; n < N; n++) {
+        double a = 0.0;
+        for (int q = 0; q < 7; q++)
+          a += f64 ? ((const double *)host)[(size_t)q * N + n] : (double)((const float *)host)[(size_t)q * N + n];
+        t[n] = (float)a;
+      }
+      HIP_TRY(s, hipMemcpyAsync((void *)th, t.data(), N * 4, hipMemcpyHostToDevice, s->st));
+      HIP_TRY(s, hipStreamSynchronize(s->st));
+    }
     const int nq = is_f ? 19 : 7;
     for (int q = 0; q < nq; q++) {
-      const float w = is_f ? W19[q] : 0.0f;
+      const float w = is_f ? W19[q] : W7[q];
       char *plane = dev + (size_t)q * N * s->eb;
       if (write) {
         HIP_TRY(s, hipMemcpyAsync(s->stage, host + (size_t)q * N * hb, N * hb, hipMemcpyHostToDevice, s->st));
-        HIP_TRY(s, aht::launch_encode(s->storage, s->stage, f64, plane, (uint32_t)N, w, is_f, s->st));
+        HIP_TRY(s, aht::launch_encode(s->storage, s->stage, f64, plane, (uint32_t)N, w, is_f, th, s->st));
       } else {
-        HIP_TRY(s, aht::launch_decode(s->storage, plane, s->stage, f64, (uint32_t)N, w, is_f, s->st));
+        HIP_TRY(s, aht::launch_decode(s->storage, plane, s->stage, f64, (uint32_t)N, w, is_f, th, s->st));
         HIP_TRY(s, hipMemcpyAsync(host + (size_t)q * N * hb, s->stage, N * hb, hipMemcpyDeviceToHost, s->st));
       }
     }
@@ -532,10 +554,12 @@ struct aht_batch {
   bool use_graph = true;
   hipGraphExec_t gexec = nullptr;
   int g_version = -1, g_n = -1, g_grid = -1;
-  // online tuner: candidate grids timed on real chunks, best kept per membership version
+  // online tuner: candidate grids timed on real chunks (device time per tile-step, so chunks with different members
+  // compare), tuned again only when the tile count moves by more than 2x from where the last tuning started
   std::vector<int> cand;
   std::vector<double> cand_ms;
-  int tune_version = -1, tune_next = 0;
+  int tune_next = 0;
+  uint32_t tune_total = 0;
   int64_t steps = 0;
   std::string err;
 };
@@ -579,7 +603,8 @@ int aht_batch_create(int32_t device, int32_t storage, aht_batch **out) {
   HIP_TRYB(b, hipEventCreate(&b->ev0));
   HIP_TRYB(b, hipEventCreate(&b->ev1));
   const int cus = cu_count(device);
-  b->cand = {cus, 2 * cus, 4 * cus, 8 * cus};
+  // persistent (8 or 32 blocks per CU, each looping over tiles) or one block per tile (clamped to the tile count)
+  b->cand = {8 * cus, 32 * cus, 1 << 30};
   b->cand_ms.assign(b->cand.size(), 0.0);
   return AHT_OK;
 }
@@ -676,11 +701,13 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
   }
   HIP_TRYB(b, hipMemcpyAsync(b->dmem, b->hmem.data(), nm * sizeof(aht::Member), hipMemcpyHostToDevice, b->st));
   // grid: fixed, or the online tuner's pick (each candidate computes the same result)
+  double work = 0.0;                                  // tile-steps of this chunk
+  for (size_t d = 0; d < nm; d++) work += (double)b->hmem[d].tiles * (double)b->hmem[d].nsteps;
   int grid = b->grid;
   int cand_idx = -1;
   if (grid == 0) {
-    if (b->tune_version != b->version) {
-      b->tune_version = b->version;
+    if (b->tune_total == 0 || b->total > 2 * b->tune_total || 2 * b->total < b->tune_total) {
+      b->tune_total = std::max<uint32_t>(b->total, 1);
       b->tune_next = 0;
       std::fill(b->cand_ms.begin(), b->cand_ms.end(), 0.0);
     }
@@ -694,7 +721,6 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
       grid = b->cand[best];
     }
   }
-  HIP_TRYB(b, hipEventRecord(b->ev0, b->st));
   if (b->use_graph) {
     if (!b->gexec || b->g_version != b->version || b->g_n != n || b->g_grid != grid) {
       if (b->gexec) HIP_TRYB(b, hipGraphExecDestroy(b->gexec));
@@ -714,8 +740,10 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
       HIP_TRYB(b, hipGraphDestroy(graph));
       b->g_version = b->version; b->g_n = n; b->g_grid = grid;
     }
+    HIP_TRYB(b, hipEventRecord(b->ev0, b->st));       // after the (host-side) capture: times the device work only
     HIP_TRYB(b, hipGraphLaunch(b->gexec, b->st));
   } else {
+    HIP_TRYB(b, hipEventRecord(b->ev0, b->st));
     for (int it = 0; it < n; it++)
       HIP_TRYB(b, aht::launch_batch_step(b->storage, b->dmem, (int)nm, b->total, it, grid, b->st));
   }
@@ -735,7 +763,7 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
     HIP_TRYB(b, hipEventSynchronize(b->ev1));
     float t = 0.0f;
     HIP_TRYB(b, hipEventElapsedTime(&t, b->ev0, b->ev1));
-    if (cand_idx >= 0) b->cand_ms[cand_idx] = t / n;
+    if (cand_idx >= 0) b->cand_ms[cand_idx] = t / std::max(work, 1.0);
     if (ms) *ms = t;
   }
   return AHT_OK;
@@ -750,12 +778,15 @@ int aht_batch_sync(aht_batch *b) {
 
 int aht_batch_grid(const aht_batch *b) {
   if (!b) return -1;
-  if (b->grid) return b->grid;
-  if (b->tune_next < (int)b->cand.size()) return 0;
-  size_t best = 0;
-  for (size_t k = 1; k < b->cand.size(); k++)
-    if (b->cand_ms[k] < b->cand_ms[best]) best = k;
-  return b->cand[best];
+  int g = b->grid;
+  if (!g) {
+    if (b->tune_next < (int)b->cand.size()) return 0;
+    size_t best = 0;
+    for (size_t k = 1; k < b->cand.size(); k++)
+      if (b->cand_ms[k] < b->cand_ms[best]) best = k;
+    g = b->cand[best];
+  }
+  return (int)std::min<uint32_t>((uint32_t)g, std::max<uint32_t>(b->total, 1));   // blocks actually launched
 }
 
 // ---- launch-shape autotuner for one solver ------------------------------------------------------------------------
@@ -763,7 +794,8 @@ namespace {
 int check_launch(aht_solver *s, const aht_launch &l) {
   if (l.block < 64 || l.block > 256 || l.block % 64) return fail(s, AHT_EINVAL, "block must be 64, 128, 192 or 256");
   if (l.levels != 1 && l.levels != 2 && l.levels != 4) return fail(s, AHT_EINVAL, "levels must be 1, 2 or 4");
-  if (l.window < 0 || l.window > 4096) return fail(s, AHT_EINVAL, "window must be 0 (slabs) or 1..4096");
+  if (l.window < 0 || l.window > 4096 || (l.window & (l.window - 1)))
+    return fail(s, AHT_EINVAL, "window must be 0 (slabs) or a power of two up to 4096");
   return AHT_OK;
 }
 void apply_launch(aht_solver *s, const aht_launch &l) {
@@ -778,7 +810,7 @@ int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best) {
   if (rc) return rc;
   if (steps < 1) return fail(s, AHT_EINVAL, "autotune needs steps >= 1");
   if ((rc = set_device(s))) return rc;
-  const size_t N = s->g.N, fb = 19 * N * s->eb, gb = 7 * N * s->eb;
+  const size_t N = s->g.N, fb = 19 * N * s->eb, gb = s->gbytes;
   const int p0 = s->parity;
   const int64_t steps0 = s->steps, nacc0 = s->n_acc;
   void *bf = nullptr, *bg = nullptr;

@@ -7,10 +7,12 @@
 // `lattice_pull` reads at the pull source cell - c_q (clamped), as the two-lattice pull step does; `lattice_aa_even`
 // and `lattice_aa_odd` work in place on one lattice as the two steps of the AA pattern do (even: read slot q and
 // write slot opp(q) of the cell itself; odd: read slot opp(q) at cell - c_q, write slot q at cell + c_q, interior
-// cells). They give the rates attainable with 52 concurrent streams, a tighter bound for the step than the plain copy.
+// cells); `lattice_pull_nt` is the pull with non-temporal stores (as the step stores), `lattice16_pull_nt` the same
+// for the 16-bit formats: 26 arrays of 2-byte elements plus an f32 theta array pulled at the 7 D3Q7 offsets. They
+// give the rates attainable with 52+ concurrent streams, a tighter bound for the step than the plain copy.
 // Usage: aht_copy_peak [bytes] [reps] [variant]
 //   (variant: all | grid | flat | unroll4 | unroll4_xcd | unroll4_nt | lattice | lattice_pull | lattice_aa_even
-//    | lattice_aa_odd)
+//    | lattice_aa_odd | lattice_pull_nt | lattice16_pull_nt)
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -62,7 +64,7 @@ __constant__ int LCX[26] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0,
 __constant__ int LCY[26] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 1, -1, 0, 0};
 __constant__ int LCZ[26] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0, 0, 1, -1};
 
-template <bool PULL>
+template <bool PULL, bool NT>
 __global__ __launch_bounds__(256) void copy_lattice(const float *__restrict__ a, float *__restrict__ b, uint32_t N,
                                                     int nx, int nxy) {
   const uint32_t n = blockIdx.x * 256 + threadIdx.x;
@@ -78,7 +80,31 @@ __global__ __launch_bounds__(256) void copy_lattice(const float *__restrict__ a,
     v[q] = a[(size_t)q * N + (size_t)src];
   }
 #pragma unroll
-  for (int q = 0; q < 26; q++) b[(size_t)q * N + n] = v[q];
+  for (int q = 0; q < 26; q++) {
+    if (NT) __builtin_nontemporal_store(v[q], &b[(size_t)q * N + n]);
+    else b[(size_t)q * N + n] = v[q];
+  }
+}
+
+// 16-bit lattice: 26 arrays of 2-byte elements [q][cell] then an f32 theta array, pulled; non-temporal stores
+__global__ __launch_bounds__(256) void copy_lattice16(const uint16_t *__restrict__ a, uint16_t *__restrict__ b,
+                                                      uint32_t N, int nx, int nxy) {
+  const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  const float *ta = (const float *)(a + (size_t)26 * N);
+  float *tb = (float *)(b + (size_t)26 * N);
+  uint16_t v[26];
+  float th = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 26; q++) {
+    int64_t src = (int64_t)n - (LCX[q] + LCY[q] * nx + LCZ[q] * nxy);
+    src = src < 0 ? 0 : (src >= (int64_t)N ? (int64_t)N - 1 : src);
+    v[q] = a[(size_t)q * N + (size_t)src];
+    if (q >= 19) th += ta[src];
+  }
+#pragma unroll
+  for (int q = 0; q < 26; q++) __builtin_nontemporal_store(v[q], &b[(size_t)q * N + n]);
+  __builtin_nontemporal_store(th, &tb[n]);
 }
 
 __constant__ int LOPP[26] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17, 19, 21, 20, 23, 22, 25, 24};
@@ -127,13 +153,13 @@ int main(int argc, char **argv) {
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
-  const char *names[9] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
-                          "lattice_aa_even", "lattice_aa_odd"};
+  const char *names[11] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
+                           "lattice_aa_even", "lattice_aa_odd", "lattice_pull_nt", "lattice16_pull_nt"};
   const int lnx = 600, lnxy = 600 * 600;
   const uint32_t lN = 600u * 600u * 64u;   // lattice variants: 26 * lN floats each way (2.4 GB)
   double best = 0;
   const char *best_name = "";
-  for (int v = 0; v < 9; v++) {
+  for (int v = 0; v < 11; v++) {
     if (strcmp(which, "all") && strcmp(which, names[v])) continue;
     if (v >= 5 && n * 16 < (size_t)26 * lN * 4) continue;   // the lattice needs 2.4 GB per buffer
     auto launch = [&]() {
@@ -143,10 +169,12 @@ int main(int argc, char **argv) {
         case 2: hipLaunchKernelGGL((copy_unroll4<false, false>), dim3((unsigned)((n + 1023) / 1024)), dim3(256), 0, 0, a, b, n); break;
         case 3: hipLaunchKernelGGL((copy_unroll4<true, false>), dim3((unsigned)((n + 1023) / 1024)), dim3(256), 0, 0, a, b, n); break;
         case 4: hipLaunchKernelGGL((copy_unroll4<true, true>), dim3((unsigned)((n + 1023) / 1024)), dim3(256), 0, 0, a, b, n); break;
-        case 5: hipLaunchKernelGGL((copy_lattice<false>), dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
-        case 6: hipLaunchKernelGGL((copy_lattice<true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
+        case 5: hipLaunchKernelGGL((copy_lattice<false, false>), dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
+        case 6: hipLaunchKernelGGL((copy_lattice<true, false>), dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 7: hipLaunchKernelGGL((copy_lattice_aa<false>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
         case 8: hipLaunchKernelGGL((copy_lattice_aa<true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
+        case 9: hipLaunchKernelGGL((copy_lattice<true, true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
+        case 10: hipLaunchKernelGGL(copy_lattice16, dim3((lN + 255) / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy); break;
       }
     };
     launch();
@@ -157,7 +185,7 @@ int main(int argc, char **argv) {
     CK(hipEventSynchronize(e1));
     float ms = 0;
     CK(hipEventElapsedTime(&ms, e0, e1));
-    const size_t moved = v >= 5 ? (size_t)26 * lN * 4 : n * 16;
+    const size_t moved = v == 10 ? (size_t)28 * lN * 2 : (v >= 5 ? (size_t)26 * lN * 4 : n * 16);
     const double per = ms / reps, gbs = 2.0 * moved / (per * 1e-3) / 1e9;
     printf("{\"kernel\": \"copy_%s\", \"bytes_copied\": %zu, \"reps\": %d, \"ms_per_launch\": %.5f, "
            "\"GBps_read_plus_write\": %.1f, \"cus\": %d}\n", names[v], moved, reps, per, gbs, cus);

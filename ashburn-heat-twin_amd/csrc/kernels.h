@@ -15,6 +15,8 @@ struct Grid {
   uint32_t N;          // nx * ny * nz (< 2^32; per-q offsets use 64-bit arithmetic)
   float inv_nx;        // 1 / nx, for the column -> (i, j) split
   uint32_t win;        // tile order: 1 = linear, W > 1 = windows of W consecutive tiles per XCD, 0 = one slab per XCD
+  uint32_t th_off;     // theta-referenced D3Q7 lattices: byte offset of the f32 theta plane (0 otherwise)
+  uint32_t pad;
   const uint32_t *colk; // per column: kt | ktop << 10 | kfast << 20 (read by every cell), solver orientation
   const float2 *colcq;  // per column: (cd, q), read only at the first fluid cell
   const uint32_t *ktop;
@@ -51,6 +53,9 @@ size_t face_len(int face, int nx, int ny, int nz);      // doubles per face (2 *
 uint32_t content_slots(const Grid &g, int block);       // wavefront slots of the step grid
 
 int storage_bytes(int storage);                         // bytes per population element of an AHT_STORE_* format
+bool storage_theta_ref(int storage);                    // D3Q7 lattice = f32 theta + deviations (f16, bf16)
+size_t theta_offset(int storage, size_t N);             // byte offset of the theta plane in a D3Q7 lattice (or 0)
+size_t g_lattice_bytes(int storage, size_t N);          // bytes of one D3Q7 lattice
 
 // kernels are templated on the storage format; the launchers dispatch on the AHT_STORE_* id
 // levels = cells per thread along z (1, 2 or 4): the tile is (block of columns, chunk of `levels` levels);
@@ -73,9 +78,11 @@ uint32_t stats_partials(const Grid &g);
 // fixed-order sum of n doubles into *out (one block)
 hipError_t launch_sum(const double *x, uint32_t n, double *out, hipStream_t st);
 // one population plane of n cells between storage and f32 (or f64) host-format buffers; w = rest weight for f planes
+// g planes of theta-referenced formats decode / encode against the lattice's theta plane `th` (w = the plane's D3Q7
+// weight); otherwise th is unused
 hipError_t launch_decode(int storage, const void *src, void *dst, int dst_f64, uint32_t n, float w, int is_f,
-                         hipStream_t st);
+                         const float *th, hipStream_t st);
 hipError_t launch_encode(int storage, const void *src, int src_f64, void *dst, uint32_t n, float w, int is_f,
-                         hipStream_t st);
+                         const float *th, hipStream_t st);
 
 }  // namespace aht

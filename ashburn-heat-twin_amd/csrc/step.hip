@@ -49,21 +49,35 @@ __device__ __forceinline__ void split_column(const Grid &g, int c, int &i, int &
   i = ii; j = jj;
 }
 
-// Tile order. Workgroup b runs on XCD b % 8 and is the (b / 8)-th one there; `win` picks the tile it takes:
+// Tile order. Workgroup b runs on XCD b % 8 and is the (b / 8)-th one there; `win` (0 or a power of two) picks the
+// tile it takes:
 //   win == 1: tile b (linear; the eight XCDs share one front moving through memory);
 //   win >= 2: windows of 8 * win tiles, XCD x taking the x-th run of `win` consecutive tiles of each window, so the
 //             lines that neighbouring tiles share (the +-1 cell and +-1 row pulls) mostly meet in one L2 while the
 //             front stays compact; the last partial window keeps the linear order;
 //   win == 0: XCD x takes the x-th contiguous eighth of all tiles (eight fronts).
+// Shifts and masks only: this runs once per wavefront.
 __device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t G, uint32_t win) {
-  const uint32_t x = b % NUM_XCD, i = b / NUM_XCD;
+  const uint32_t x = b & (NUM_XCD - 1), i = b >> 3;
   if (win == 0) {
-    const uint32_t q = G / NUM_XCD, r = G % NUM_XCD;
+    const uint32_t q = G >> 3, r = G & (NUM_XCD - 1);
     return x * q + min(x, r) + i;
   }
-  const uint32_t span = win * NUM_XCD;
-  if (win == 1 || b >= (G / span) * span) return b;
-  return (i / win) * span + x * win + i % win;
+  if (win == 1) return b;
+  const uint32_t lw = (uint32_t)__builtin_ctz(win), span_mask = (win << 3) - 1u;
+  if (b >= (G & ~span_mask)) return b;
+  return ((i >> lw) << (lw + 3)) + (x << lw) + (i & (win - 1u));
+}
+
+// q = t / d, r = t - q d for a wave-uniform t < 2^26 and 1 <= d < 2^20, from a host-computed float 1/d (the float
+// estimate is off by at most one, corrected once): a few instructions instead of an emulated integer division
+__device__ __forceinline__ uint32_t div_by(uint32_t t, uint32_t d, float inv_d, uint32_t &r) {
+  uint32_t q = (uint32_t)((float)t * inv_d);
+  int32_t rr = (int32_t)(t - q * d);
+  if (rr < 0) { q -= 1; rr += (int32_t)d; }
+  else if (rr >= (int32_t)d) { q += 1; rr -= (int32_t)d; }
+  r = (uint32_t)rr;
+  return q;
 }
 
 // packed column word: kt | ktop << 10 | kfast << 20 (nz < 1023)
@@ -76,9 +90,29 @@ template <class S, int q>
 __device__ __forceinline__ typename S::real ldf(const typename S::elem *__restrict__ f, size_t N, int qq, uint32_t idx) {
   return S::dec_f(f[(size_t)qq * N + idx], w19<q>());
 }
+__device__ __forceinline__ float w7q(int q) { return q == 0 ? 0.25f : 0.125f; }
+
+// theta population qq at cell index idx (theta-referenced formats add w_qq times the stored theta of that cell)
 template <class S>
-__device__ __forceinline__ typename S::real ldg(const typename S::elem *__restrict__ g, size_t N, int qq, uint32_t idx) {
-  return S::dec_g(g[(size_t)qq * N + idx]);
+__device__ __forceinline__ typename S::real ldg(const Grid &g, const void *gA, int qq, uint32_t idx) {
+  using E = typename S::elem;
+  const E e = ((const E *)gA)[(size_t)qq * g.N + idx];
+  if constexpr (S::theta_ref) return S::dec_g(e, w7q(qq), ld_theta(gA, g.th_off + idx * 4u));
+  else return S::dec_g(e, 0.0f, typename S::real(0));
+}
+
+// the seven post-collision theta populations p of cell n: raw, or theta = sum_q p_q in f32 plus the deviations
+template <class S>
+__device__ __forceinline__ void store_g(const Grid &g, void *gB, uint32_t n, const typename S::real (&p)[7]) {
+  using R = typename S::real;
+  const uint32_t boff = n * (uint32_t)S::bytes;
+  R th = R(0);
+  if constexpr (S::theta_ref) {
+    th = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + p[6]);
+    st_theta(gB, g.th_off + n * 4u, th);
+  }
+#pragma unroll
+  for (int q = 0; q < 7; q++) S::st_raw(plane<S>(gB, g.N, q), boff, S::enc_g(p[q], w7q(q), th));
 }
 
 template <int q>
@@ -178,23 +212,23 @@ __device__ __forceinline__ typename S::real pull19(const Grid &g, const typename
 
 // post-stream theta population q (js/live/kernel.js:219-243)
 template <class S, int q>
-__device__ __forceinline__ typename S::real pull7(const Grid &g, const typename S::elem *__restrict__ gA, size_t N,
-                                                  uint32_t n, int i, int j, int k, const int (&kn)[3][3]) {
+__device__ __forceinline__ typename S::real pull7(const Grid &g, const void *gA, uint32_t n, int i, int j, int k,
+                                                  const int (&kn)[3][3]) {
   using R = typename S::real;
   constexpr int cx = CX7[q], cy = CY7[q], cz = CZ7[q];
   const Params &P = g.P;
-  if (cz == -1 && k == g.nz - 1) return ldg<S>(gA, N, q, n);   // zero-gradient top
+  if (cz == -1 && k == g.nz - 1) return ldg<S>(g, gA, q, n);   // zero-gradient top
   const bool outx = (cx == 1 && i == 0) || (cx == -1 && i == g.nx - 1);
   const bool outy = (cy == 1 && j == 0) || (cy == -1 && j == g.ny - 1);
   if (outx || outy) {
     const bool inflow = (cx == 1 && i == 0 && P.inflow_w != 0) || (cx == -1 && i == g.nx - 1 && P.inflow_e != 0) ||
                         (cy == 1 && j == 0 && P.inflow_s != 0) || (cy == -1 && j == g.ny - 1 && P.inflow_n != 0);
-    return inflow ? R(0) : ldg<S>(gA, N, q, n);
+    return inflow ? R(0) : ldg<S>(g, gA, q, n);
   }
   const int sk = k - cz;
-  if (sk < kn[1 - cx][1 - cy]) return ldg<S>(gA, N, OPP7[q], n);
+  if (sk < kn[1 - cx][1 - cy]) return ldg<S>(g, gA, OPP7[q], n);
   const int off = cx + cy * g.nx + cz * (int)g.NXY;
-  return ldg<S>(gA, N, q, (uint32_t)((int)n - off));
+  return ldg<S>(g, gA, q, (uint32_t)((int)n - off));
 }
 
 template <int q>
@@ -220,19 +254,17 @@ struct Pull19<19> {
 template <int q>
 struct Pull7 {
   template <class S>
-  __device__ __forceinline__ static void run(typename S::real (&gg)[7], const Grid &g,
-                                             const typename S::elem *__restrict__ gA, size_t N, uint32_t n, int i,
-                                             int j, int k, const int (&kn)[3][3]) {
-    gg[q] = pull7<S, q>(g, gA, N, n, i, j, k, kn);
-    Pull7<q + 1>::template run<S>(gg, g, gA, N, n, i, j, k, kn);
+  __device__ __forceinline__ static void run(typename S::real (&gg)[7], const Grid &g, const void *gA, uint32_t n,
+                                             int i, int j, int k, const int (&kn)[3][3]) {
+    gg[q] = pull7<S, q>(g, gA, n, i, j, k, kn);
+    Pull7<q + 1>::template run<S>(gg, g, gA, n, i, j, k, kn);
   }
 };
 template <>
 struct Pull7<7> {
   template <class S>
-  __device__ __forceinline__ static void run(typename S::real (&)[7], const Grid &,
-                                             const typename S::elem *__restrict__, size_t, uint32_t, int, int, int,
-                                             const int (&)[3][3]) {}
+  __device__ __forceinline__ static void run(typename S::real (&)[7], const Grid &, const void *, uint32_t, int, int,
+                                             int, const int (&)[3][3]) {}
 };
 
 // fast path: all 26 sources are fluid interior cells; plain shifted loads through per-population descriptors
@@ -240,14 +272,21 @@ template <int q>
 struct FastPull {
   template <class S>
   __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
-                                             const void *fA, const void *gA, uint32_t boff) {
+                                             const void *fA, const void *gA, uint32_t n) {
+    const uint32_t boff = n * (uint32_t)S::bytes;
     const int off = CX[q] + CY[q] * g.nx + CZ[q] * (int)g.NXY;
     f[q] = S::dec_f(S::ld_raw(shifted<S>(fA, g.N, q, off), boff), w19<q>());
     if constexpr (q < 7) {
       const int o7 = CX7[q] + CY7[q] * g.nx + CZ7[q] * (int)g.NXY;
-      gg[q] = S::dec_g(S::ld_raw(shifted<S>(gA, g.N, q, o7), boff));
+      const typename S::elem e = S::ld_raw(shifted<S>(gA, g.N, q, o7), boff);
+      if constexpr (S::theta_ref) {
+        const void *thb = (const void *)((uintptr_t)gA + g.th_off - (intptr_t)o7 * 4);   // theta plane, pulled
+        gg[q] = S::dec_g(e, w7q(q), ld_theta(thb, n * 4u));
+      } else {
+        gg[q] = S::dec_g(e, 0.0f, typename S::real(0));
+      }
     }
-    FastPull<q + 1>::template run<S>(f, gg, g, fA, gA, boff);
+    FastPull<q + 1>::template run<S>(f, gg, g, fA, gA, n);
   }
 };
 template <>
@@ -292,19 +331,18 @@ __device__ __forceinline__ void stream_cell(const Grid &g, const void *fA, const
                                             typename S::real (&f)[19], typename S::real (&gg)[7]) {
   using E = typename S::elem;
   if (fast) {
-    FastPull<0>::template run<S>(f, gg, g, fA, gA, n * (uint32_t)S::bytes);
+    FastPull<0>::template run<S>(f, gg, g, fA, gA, n);
   } else {
     int kn[3][3];
     load_neighbours(g, i, j, kg, kn);
     const Outflow<typename S::real> ob = outflow_prelude<S>(g, (const E *)fA, g.N, n, i, j);
     Pull19<0>::template run<S>(f, g, (const E *)fA, g.N, n, i, j, k, kn, uI, vI, ob);
-    Pull7<0>::template run<S>(gg, g, (const E *)gA, g.N, n, i, j, k, kn);
+    Pull7<0>::template run<S>(gg, g, gA, n, i, j, k, kn);
   }
 }
 
 // one cell of the fused step (genStepWGSL, js/live/kernel.js:395-431) with FP64 accumulation (ACC) and debug outputs
-// (OUT). Every thread of the block must call it (the ACC heat-content sum is a wavefront reduction); `slot` is the
-// wavefront's heat-content slot.
+// (OUT). Every thread of the block must call it (the ACC heat-content sum is a wavefront reduction).
 template <class S, bool ACC, bool OUT>
 __device__ __forceinline__ double step_cell_body(const Grid &g, const void *fA, void *fB, const void *gA, void *gB,
                                                  const Acc &acc, float *__restrict__ out4, int c, int k, uint32_t ck) {
@@ -332,13 +370,10 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const void *fA, 
     const R Sq = (k == kg) ? R(cq.y) * R(g.P.srcscale) : R(0);
     const Scalar<R> sc = scalar_coeffs(m.th, Rx.omg, Sq, F.sg);
     const R b1 = sc.b + sc.cs1, b4 = R(4) * sc.b;
-    S::st_raw(plane<S>(gB, N, 0), boff, S::enc_g(sc.a * gg[0] + sc.cs0));
-    S::st_raw(plane<S>(gB, N, 1), boff, S::enc_g(sc.a * gg[1] + (b1 + b4 * F.ux)));
-    S::st_raw(plane<S>(gB, N, 2), boff, S::enc_g(sc.a * gg[2] + (b1 - b4 * F.ux)));
-    S::st_raw(plane<S>(gB, N, 3), boff, S::enc_g(sc.a * gg[3] + (b1 + b4 * F.uy)));
-    S::st_raw(plane<S>(gB, N, 4), boff, S::enc_g(sc.a * gg[4] + (b1 - b4 * F.uy)));
-    S::st_raw(plane<S>(gB, N, 5), boff, S::enc_g(sc.a * gg[5] + (b1 + b4 * F.uz)));
-    S::st_raw(plane<S>(gB, N, 6), boff, S::enc_g(sc.a * gg[6] + (b1 - b4 * F.uz)));
+    const R gp[7] = {sc.a * gg[0] + sc.cs0,          sc.a * gg[1] + (b1 + b4 * F.ux), sc.a * gg[2] + (b1 - b4 * F.ux),
+                     sc.a * gg[3] + (b1 + b4 * F.uy), sc.a * gg[4] + (b1 - b4 * F.uy), sc.a * gg[5] + (b1 + b4 * F.uz),
+                     sc.a * gg[6] + (b1 - b4 * F.uz)};
+    store_g<S>(g, gB, n, gp);
     if (OUT) {
       out4[n] = (float)m.th;
       out4[N + n] = (float)F.ux;
@@ -397,35 +432,44 @@ __device__ __forceinline__ void step_cell(const Grid &g, const void *fA, void *f
 // order g.win selects; a thread walks its column through the KB levels (column word and index math once per column)
 template <class S, bool ACC, bool OUT, int KB>
 __global__ __launch_bounds__(256) void step_kernel(Grid g, const void *fA, void *fB, const void *gA, void *gB, Acc acc,
-                                                   float *__restrict__ out4, uint32_t bx) {
+                                                   float *__restrict__ out4, uint32_t bx, float inv_bx) {
   const uint32_t t = tile_of_block(blockIdx.x, gridDim.x, g.win);
-  const uint32_t kk = t / bx, cb = t - kk * bx;
+  uint32_t cb;
+  const uint32_t kk = div_by(t, bx, inv_bx, cb);
   const int c = (int)(cb * blockDim.x + threadIdx.x);
   const int k0 = g.kmin + (int)kk * KB;
   const uint32_t ck = g.colk[c < (int)g.NXY ? c : 0];
   double th = 0.0;
-  if (KB == 1) {
-    th = step_cell_body<S, ACC, OUT>(g, fA, fB, gA, gB, acc, out4, c, k0, ck);
-  } else {
-    for (int u = 0; u < KB; u++) th += step_cell_body<S, ACC, OUT>(g, fA, fB, gA, gB, acc, out4, c, k0 + u, ck);
-  }
+  for (int u = 0; u < KB; u++) th += step_cell_body<S, ACC, OUT>(g, fA, fB, gA, gB, acc, out4, c, k0 + u, ck);
   if (ACC) content_add(acc, t * (blockDim.x >> 6) + (threadIdx.x >> 6), th);
 }
 
 // ---- many domains in one launch ----------------------------------------------------------------------------------
-// A persistent grid walks one step of every batch member: member d contributes tiles (256-column block, level) for
-// its fluid levels; block b handles the contiguous tile range [b*T/G, (b+1)*T/G) of the concatenated list. Per-member
-// fields make the result of a member independent of the mix: it steps only while `it < nsteps` (it = index of this
-// launch within the captured chunk), reads lattice parity0 ^ (it & 1), and accumulates means from launch acc_from on.
+// One grid of G blocks walks one step of every batch member: member d contributes tiles (256-column block, level) for
+// its fluid levels, concatenated in member order. The grid strides through the list in windows of G tiles, taking
+// the tiles of a window in the XCD-aware order of the single-domain step (runs of 64 tiles per XCD), so the blocks in
+// flight keep one compact front through memory instead of G separate ones. G from a few blocks per CU (persistent)
+// up to one block per tile is the batch tuner's choice. Per-member fields make the result of a
+// member independent of the mix: it steps only while `it < nsteps` (it = index of this launch within the captured
+// chunk), reads lattice parity0 ^ (it & 1), and accumulates means from launch acc_from on.
 template <class S>
-__global__ __launch_bounds__(256) void batch_step_kernel(const Member *__restrict__ mem, int nmem, uint32_t total,
-                                                         int it) {
-  const uint32_t t0 = (uint32_t)(((uint64_t)blockIdx.x * total) / gridDim.x);
-  const uint32_t t1 = (uint32_t)(((uint64_t)(blockIdx.x + 1) * total) / gridDim.x);
+__global__ __launch_bounds__(256)
+void batch_step_kernel(const Member *__restrict__ mem, int nmem, uint32_t total, int it) {
+  const uint32_t G = gridDim.x, tb = tile_of_block(blockIdx.x, G, 64);
+  // member of the block's first tile by binary search over tile_base (ascending), then advanced as t grows
   int d = 0;
-  for (uint32_t t = t0; t < t1; t++) {
-    while (d + 1 < nmem && mem[d + 1].tile_base <= t) d++;        // tiles are visited in increasing order
-    while (d > 0 && mem[d].tile_base > t) d--;
+  if (tb < total) {
+    int lo = 0, hi = nmem - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (mem[mid].tile_base <= tb) lo = mid; else hi = mid - 1;
+    }
+    d = lo;
+  }
+  for (uint32_t base = 0; base < total; base += G) {
+    const uint32_t t = base + tb;
+    if (t >= total) continue;                                   // partial last window
+    while (d + 1 < nmem && mem[d + 1].tile_base <= t) d++;      // t grows with base: the member index only advances
     const Member &M = mem[d];
     if (it >= M.nsteps) continue;
     const uint32_t lt = t - M.tile_base;
@@ -512,7 +556,8 @@ __global__ __launch_bounds__(256) void reinit_kernel(Grid g, const uint32_t *__r
       ((E *)f)[(size_t)q * N + n] = S::enc_f(R(wf), wf);
     }
 #pragma unroll
-    for (int q = 0; q < 7; q++) ((E *)gq)[(size_t)q * N + n] = S::enc_g(R(0));
+    for (int q = 0; q < 7; q++) ((E *)gq)[(size_t)q * N + n] = S::enc_g(R(0), w7q(q), R(0));
+    if constexpr (S::theta_ref) ((float *)((char *)gq + g.th_off))[n] = 0.0f;
   }
 }
 
@@ -539,8 +584,7 @@ __global__ __launch_bounds__(256) void volume_kernel(Grid g, const void *gA, int
           int kn[3][3];
           load_neighbours(g, i, j, kg, kn);
           R gg[7];
-          Pull7<0>::template run<S>(gg, g, (const typename S::elem *)gA, g.N, (uint32_t)k * g.NXY + (uint32_t)c, i, j,
-                                    k, kn);
+          Pull7<0>::template run<S>(gg, g, gA, (uint32_t)k * g.NXY + (uint32_t)c, i, j, k, kn);
           th = (float)(gg[0] + gg[1] + gg[2] + gg[3] + gg[4] + gg[5] + gg[6]);
         }
         const uint32_t qv = (uint32_t)fminf(fmaxf(rintf(th * volscale), 0.0f), 255.0f);
@@ -585,7 +629,7 @@ __global__ __launch_bounds__(256) void stats_kernel(Grid g, const void *fA, cons
       }
       R th = R(0);
 #pragma unroll
-      for (int q = 0; q < 7; q++) th += ldg<S>((const E *)gA, N, q, n);
+      for (int q = 0; q < 7; q++) th += ldg<S>(g, gA, q, n);
       if (!isfinite((double)rho) || !isfinite((double)th)) {
         v[6] = 1.0;
       } else {
@@ -656,22 +700,26 @@ __global__ __launch_bounds__(256) void sum_f64(const double *__restrict__ x, uin
   if (threadIdx.x == 0) *out = red[0];
 }
 
-// storage <-> f32 / f64 conversion of one population plane (read_state / write_state); w = rest weight (f) or 0 (g)
+// storage <-> f32 / f64 conversion of one population plane (read_state / write_state); w = the plane's weight (f: the
+// D3Q19 weight of its deviation; g: the D3Q7 weight); th = theta plane of a theta-referenced D3Q7 lattice
 template <class S, typename T>
-__global__ __launch_bounds__(256) void decode_plane(const void *src, T *__restrict__ dst, uint32_t n, float w, int is_f) {
-  using E = typename S::elem;
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const E e = ((const E *)src)[t];
-  dst[t] = (T)(is_f ? S::dec_f(e, w) : S::dec_g(e));
-}
-template <class S, typename T>
-__global__ __launch_bounds__(256) void encode_plane(const T *__restrict__ src, void *dst, uint32_t n, float w, int is_f) {
+__global__ __launch_bounds__(256) void decode_plane(const void *src, T *__restrict__ dst, uint32_t n, float w, int is_f,
+                                                    const float *__restrict__ th) {
   using E = typename S::elem;
   using R = typename S::real;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  ((E *)dst)[t] = is_f ? S::enc_f(R(src[t]), w) : S::enc_g(R(src[t]));
+  const E e = ((const E *)src)[t];
+  dst[t] = (T)(is_f ? S::dec_f(e, w) : S::dec_g(e, w, th ? R(th[t]) : R(0)));
+}
+template <class S, typename T>
+__global__ __launch_bounds__(256) void encode_plane(const T *__restrict__ src, void *dst, uint32_t n, float w, int is_f,
+                                                    const float *__restrict__ th) {
+  using E = typename S::elem;
+  using R = typename S::real;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  ((E *)dst)[t] = is_f ? S::enc_f(R(src[t]), w) : S::enc_g(R(src[t]), w, th ? R(th[t]) : R(0));
 }
 
 // ---- launchers ------------------------------------------------------------------------------------------------
@@ -688,6 +736,13 @@ uint32_t content_slots(const Grid &g, int block) {
 }
 
 int storage_bytes(int storage) { return storage == 0 ? 4 : (storage == 3 ? 8 : 2); }
+bool storage_theta_ref(int storage) { return storage == 1 || storage == 2; }
+size_t theta_offset(int storage, size_t N) {
+  return storage_theta_ref(storage) ? (7 * N * (size_t)storage_bytes(storage) + 255) / 256 * 256 : 0;
+}
+size_t g_lattice_bytes(int storage, size_t N) {
+  return storage_theta_ref(storage) ? theta_offset(storage, N) + 4 * N : 7 * N * (size_t)storage_bytes(storage);
+}
 
 #define AHT_DISPATCH(storage, CALL)                   \
   switch (storage) {                                  \
@@ -705,10 +760,11 @@ static void launch_step_kb(const Grid &g, const void *fA, void *fB, const void *
   const uint32_t nk = (uint32_t)(g.nz - g.kmin + KB - 1) / KB;
   const dim3 grid(bx * nk);
   const Acc a = acc ? *acc : Acc{nullptr, nullptr, nullptr};
-  if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
-  else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
-  else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
-  else hipLaunchKernelGGL((step_kernel<S, false, false, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx);
+  const float ib = 1.0f / (float)bx;
+  if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx, ib);
+  else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx, ib);
+  else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx, ib);
+  else hipLaunchKernelGGL((step_kernel<S, false, false, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx, ib);
 }
 
 hipError_t launch_step(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB, const Acc *acc,
@@ -770,21 +826,21 @@ hipError_t launch_sum(const double *x, uint32_t n, double *out, hipStream_t st) 
 }
 
 hipError_t launch_decode(int storage, const void *src, void *dst, int dst_f64, uint32_t n, float w, int is_f,
-                         hipStream_t st) {
+                         const float *th, hipStream_t st) {
   const dim3 grid((n + 255) / 256);
   AHT_DISPATCH(storage, {
-    if (dst_f64) hipLaunchKernelGGL((decode_plane<S, double>), grid, dim3(256), 0, st, src, (double *)dst, n, w, is_f);
-    else hipLaunchKernelGGL((decode_plane<S, float>), grid, dim3(256), 0, st, src, (float *)dst, n, w, is_f);
+    if (dst_f64) hipLaunchKernelGGL((decode_plane<S, double>), grid, dim3(256), 0, st, src, (double *)dst, n, w, is_f, th);
+    else hipLaunchKernelGGL((decode_plane<S, float>), grid, dim3(256), 0, st, src, (float *)dst, n, w, is_f, th);
   })
   return hipGetLastError();
 }
 
 hipError_t launch_encode(int storage, const void *src, int src_f64, void *dst, uint32_t n, float w, int is_f,
-                         hipStream_t st) {
+                         const float *th, hipStream_t st) {
   const dim3 grid((n + 255) / 256);
   AHT_DISPATCH(storage, {
-    if (src_f64) hipLaunchKernelGGL((encode_plane<S, double>), grid, dim3(256), 0, st, (const double *)src, dst, n, w, is_f);
-    else hipLaunchKernelGGL((encode_plane<S, float>), grid, dim3(256), 0, st, (const float *)src, dst, n, w, is_f);
+    if (src_f64) hipLaunchKernelGGL((encode_plane<S, double>), grid, dim3(256), 0, st, (const double *)src, dst, n, w, is_f, th);
+    else hipLaunchKernelGGL((encode_plane<S, float>), grid, dim3(256), 0, st, (const float *)src, dst, n, w, is_f, th);
   })
   return hipGetLastError();
 }

@@ -1,23 +1,32 @@
 // Population storage formats. The arithmetic type `real` is what the kernels compute in; `elem` is what sits in HBM.
 //
 //   F32  : f and g stored raw in f32 (the reference's layout: read_state/write_state exchange it bit for bit).
-//   F16  : f stored as (f_q - w_q) * 2^10, g as g_q * 2^10, IEEE binary16; f32 arithmetic. Storing the deviation
-//          from the rest weight keeps the mantissa for the part that changes; the power-of-two scale (exact) keeps
-//          deviations down to ~6e-8 out of the subnormal range (binary16 normals start at 6.1e-5) and leaves
-//          |f - w| <= 64 and |g_q| <= 64 representable.
-//   BF16 : the same deviations in bfloat16 (round to nearest even), f32 arithmetic.
+//   F16  : f stored as (f_q - w_q) * 2^10 (deviation from the rest weight, rho_0 = 1) in IEEE binary16; the D3Q7
+//          lattice as the cell's theta in f32 plus the deviations (g_q - w_q theta) * 2^10 in binary16, i.e. theta_0
+//          taken per cell. Keeping theta itself in f32 keeps the slowly growing heat content out of binary16 rounding
+//          (stored as g_q * 2^10, the rounding of g alone biased the 10-minute mean warming by 1.5e-3 at 1200 x 1200
+//          x 96, while the f deviations contributed 8e-5). f32 arithmetic. The power-of-two scale (exact) keeps
+//          deviations down to ~6e-8 out of the subnormal range and leaves |deviation| <= 64 representable.
+//   BF16 : the same layout with bfloat16 deviations (round to nearest even), f32 arithmetic.
 //   F64  : f and g raw in f64 with f64 arithmetic: the FP64 twin used as the reference of reduced-precision errors.
+//
+// D3Q7 lattice of a theta-referenced format (theta_ref): [7][N] deviations, then (at byte offset theta_offset(N),
+// 256-aligned) [N] f32 theta = sum_q g_q of the stored post-collision populations.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-// cache-policy operand of the population buffer loads and stores (gfx950 encoding: 1 = sc0, 2 = nt, 16 = sc1);
-// a build option, 0 = default policy
+// Population loads and stores go through raw buffer descriptors whose base is the population plane with the pull
+// offset folded in (built from SGPRs per wavefront), so the only per-lane address is the cell's byte offset.
+// Cache-policy operand of those loads and stores (gfx950 encoding: 1 = sc0, 2 = nt, 16 = sc1), a build option. Stores
+// are non-temporal by default: the next lattice is written once and read one step later by other workgroups, so
+// keeping it out of the way of the pulled lines in L2 is worth ~6 % of the step time (measured at 600 x 600 x 64:
+// 0.970 -> 0.908 ms f32, 0.548 -> 0.511 ms f16). Non-temporal loads are slower (the pulls share lines).
 #ifndef AHT_LD_CPOL
 #define AHT_LD_CPOL 0
 #endif
 #ifndef AHT_ST_CPOL
-#define AHT_ST_CPOL 0
+#define AHT_ST_CPOL 2
 #endif
 
 namespace aht {
@@ -40,10 +49,11 @@ struct StoreF32 {
   using real = float;
   static constexpr int bytes = 4;
   static constexpr int id = 0;
+  static constexpr bool theta_ref = false;
   __device__ static real dec_f(elem e, float) { return e; }
   __device__ static elem enc_f(real v, float) { return v; }
-  __device__ static real dec_g(elem e) { return e; }
-  __device__ static elem enc_g(real v) { return v; }
+  __device__ static real dec_g(elem e, float, real) { return e; }
+  __device__ static elem enc_g(real v, float, real) { return v; }
   __device__ static elem ld_raw(const void *base, uint32_t boff) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc(base), boff, 0, AHT_LD_CPOL));
   }
@@ -57,10 +67,11 @@ struct StoreF16 {
   using real = float;
   static constexpr int bytes = 2;
   static constexpr int id = 1;
+  static constexpr bool theta_ref = true;
   __device__ static real dec_f(elem e, float w) { return (float)e * DEV_INV + w; }
   __device__ static elem enc_f(real v, float w) { return (elem)((v - w) * DEV_SCALE); }
-  __device__ static real dec_g(elem e) { return (float)e * DEV_INV; }
-  __device__ static elem enc_g(real v) { return (elem)(v * DEV_SCALE); }
+  __device__ static real dec_g(elem e, float w, real th) { return (float)e * DEV_INV + w * th; }
+  __device__ static elem enc_g(real v, float w, real th) { return (elem)((v - w * th) * DEV_SCALE); }
   __device__ static elem ld_raw(const void *base, uint32_t boff) {
     return __builtin_bit_cast(_Float16, (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc(base), boff, 0, AHT_LD_CPOL));
   }
@@ -78,10 +89,11 @@ struct StoreBF16 {
   using real = float;
   static constexpr int bytes = 2;
   static constexpr int id = 2;
+  static constexpr bool theta_ref = true;
   __device__ static real dec_f(elem e, float w) { return bf16_to_f32(e.bits) * DEV_INV + w; }
   __device__ static elem enc_f(real v, float w) { return bf16_t{f32_to_bf16((v - w) * DEV_SCALE)}; }
-  __device__ static real dec_g(elem e) { return bf16_to_f32(e.bits) * DEV_INV; }
-  __device__ static elem enc_g(real v) { return bf16_t{f32_to_bf16(v * DEV_SCALE)}; }
+  __device__ static real dec_g(elem e, float w, real th) { return bf16_to_f32(e.bits) * DEV_INV + w * th; }
+  __device__ static elem enc_g(real v, float w, real th) { return bf16_t{f32_to_bf16((v - w * th) * DEV_SCALE)}; }
   __device__ static elem ld_raw(const void *base, uint32_t boff) {
     return bf16_t{(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc(base), boff, 0, AHT_LD_CPOL)};
   }
@@ -95,10 +107,11 @@ struct StoreF64 {
   using real = double;
   static constexpr int bytes = 8;
   static constexpr int id = 3;
+  static constexpr bool theta_ref = false;
   __device__ static real dec_f(elem e, float) { return e; }
   __device__ static elem enc_f(real v, float) { return v; }
-  __device__ static real dec_g(elem e) { return e; }
-  __device__ static elem enc_g(real v) { return v; }
+  __device__ static real dec_g(elem e, float, real) { return e; }
+  __device__ static elem enc_g(real v, float, real) { return v; }
   __device__ static elem ld_raw(const void *base, uint32_t boff) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc(base), boff, 0, AHT_LD_CPOL));
   }
@@ -107,5 +120,13 @@ struct StoreF64 {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), rsrc(base), boff, 0, AHT_ST_CPOL);
   }
 };
+
+// f32 theta of a theta-referenced D3Q7 lattice at a byte offset from a base
+__device__ __forceinline__ float ld_theta(const void *base, uint32_t boff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc(base), boff, 0, AHT_LD_CPOL));
+}
+__device__ __forceinline__ void st_theta(void *base, uint32_t boff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rsrc(base), boff, 0, AHT_ST_CPOL);
+}
 
 }  // namespace aht

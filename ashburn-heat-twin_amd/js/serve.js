@@ -8,9 +8,17 @@
 // batch and free their memory, which admits the next ones. Each member steps exactly its own step count and averages
 // exactly its own window, so its result is bit-identical to running it alone (tests/test_gpu_serve.py).
 //
+// Iteration-level priority (policy 'short-first', the default): every launch of a chunk steps only some members, so
+// a member that has little work left (cells x remaining steps <= --short-work) steps in every launch while members
+// with much more work step in the first 1/--long-share of the chunk's launches only, as long as any short member is
+// active. Small scenarios that arrive behind a 600 x 600 x 64 one then finish in about their own compute time instead
+// of one big step per launch, and the big one resumes its full rate as soon as they leave. Policy 'fifo' steps every
+// member in every launch.
+//
 // The baseline (`mode: 'sequential'`) runs the same requests one at a time with the plain per-solver path.
 // Usage: node serve.js --rate 5 --requests 24 [--mode batch|sequential] [--spinup 2200] [--mean 600] [--seed 1]
-//        [--chunk 50] [--budget-gb 200] [--slots 64] [--out file.json]
+//        [--chunk 50] [--budget-gb 200] [--slots 64] [--policy short-first|fifo] [--short-work 3e10]
+//        [--long-share 8] [--out file.json]
 
 const fs = require('fs');
 const { Scenario } = require('./scenario');
@@ -83,6 +91,24 @@ function createSolver(A, job, o) {
   return h;
 }
 
+/** launches of the next chunk: members [{cells, total, spin, done}] -> n launches, per-member step counts ns and the
+ *  first averaging launch af (relative to the chunk). Under 'short-first', while any member has at most shortWork
+ *  cell-steps left, members with more step only in the first ceil(chunk / longShare) launches. */
+function planChunk(members, chunk, policy, shortWork, longShare) {
+  const m = members.length, ns = new Int32Array(m), af = new Int32Array(m);
+  const work = members.map(a => a.cells * (a.total - a.done));
+  const prio = policy === 'short-first' && work.some(w => w <= shortWork);
+  let n = 0;
+  for (let k = 0; k < m; k++) {
+    const a = members[k];
+    ns[k] = Math.min(chunk, a.total - a.done);
+    if (prio && work[k] > shortWork) ns[k] = Math.min(ns[k], Math.ceil(chunk / longShare));
+    af[k] = Math.max(0, a.spin - a.done);
+    n = Math.max(n, ns[k]);
+  }
+  return { n, ns, af };
+}
+
 const sleep = ms => new Promise(r => setTimeout(r, ms));
 // progress on stderr at most every 15 s (long runs must not look idle)
 function progress(o, mode) {
@@ -114,6 +140,7 @@ async function serveBatched(reqs, o) {
   const B = A.batchCreate(o.device || 0, o.storage || 0);
   A.batchSetGrid(B, o.grid || 0, o.useGraph === false ? 0 : 1);
   const budget = (o.budgetGB || 200) * 1e9, slots = o.slots || 64, chunk = o.chunk || 50;
+  const shortWork = o.shortWork || 3e10, longShare = o.longShare || 8;
   const future = reqs.slice().sort((a, b) => a.arrival - b.arrival);
   const pending = [], active = [], results = [];
   let used = 0, peak = 0, busy = 0;
@@ -133,7 +160,7 @@ async function serveBatched(reqs, o) {
       job.keepSurf = !!o.keepSurf;
       const h = createSolver(A, job, o);
       A.batchAdd(B, h);
-      active.push({ r, job, h, done: 0, fp });
+      active.push({ r, job, h, done: 0, fp, cells: r.grid[0] * r.grid[1] * r.grid[2] });
       used += fp;
       peak = Math.max(peak, used);
     }
@@ -141,16 +168,10 @@ async function serveBatched(reqs, o) {
       if (future.length) await sleep(Math.max(0, (future[0].arrival - now()) * 1e3));
       continue;
     }
-    const m = active.length, ns = new Int32Array(m), af = new Int32Array(m);
-    let n = 0;
-    for (let k = 0; k < m; k++) {
-      const a = active[k];
-      ns[k] = Math.min(chunk, a.job.total - a.done);
-      af[k] = Math.max(0, a.job.spin - a.done);
-      n = Math.max(n, ns[k]);
-    }
+    const { n, ns, af } = planChunk(active.map(a => ({ cells: a.cells, total: a.job.total, spin: a.job.spin, done: a.done })),
+      chunk, o.policy || 'short-first', shortWork, longShare);
     busy += await A.batchStep(B, n, ns, af);
-    for (let k = 0; k < m; k++) active[k].done += ns[k];
+    for (let k = 0; k < active.length; k++) active[k].done += ns[k];
     for (let k = active.length - 1; k >= 0; k--) {
       const a = active[k];
       if (a.done >= a.job.total) {
@@ -166,7 +187,11 @@ async function serveBatched(reqs, o) {
   const grid = A.batchGrid(B);
   A.batchDestroy(B);
   results.sort((a, b) => a.id - b.id);
-  return { summary: Object.assign(summary('batch', o, results, 0, t1, peak, busy), { persistent_grid: grid }), results };
+  return {
+    summary: Object.assign(summary('batch', o, results, 0, t1, peak, busy),
+      { persistent_grid: grid, policy: o.policy || 'short-first' }),
+    results,
+  };
 }
 
 /** the baseline: one scenario at a time with the plain per-solver path */
@@ -223,6 +248,7 @@ async function main() {
     rate: +(a.rate || 5), requests: +(a.requests || 24), seed: +(a.seed || 1), chunk: +(a.chunk || 50),
     spinupS: +(a.spinup || 2200), meanS: +(a.mean || 600), budgetGB: +(a['budget-gb'] || 200), slots: +(a.slots || 64),
     storage: +(a.storage || 0), grid: +(a.grid || 0), useGraph: a['no-graph'] ? false : true,
+    policy: a.policy || 'short-first', shortWork: +(a['short-work'] || 3e10), longShare: +(a['long-share'] || 8),
   };
   if (a.grids) o.grids = JSON.parse(a.grids);
   const reqs = makeRequests(o);
@@ -236,6 +262,6 @@ async function main() {
   if (a.out) fs.writeFileSync(a.out, JSON.stringify(out));
 }
 
-module.exports = { makeRequests, buildScenario, serveBatched, serveSequential };
+module.exports = { makeRequests, buildScenario, planChunk, serveBatched, serveSequential };
 
 if (require.main === module) main().catch(e => { console.error(e && e.stack || e); process.exit(1); });

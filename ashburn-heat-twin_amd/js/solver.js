@@ -29,7 +29,8 @@ const STORAGE = { f32: 0, f16: 1, bf16: 2 };
 
 /** createLiveSolver's counterpart (js/live/solver.js:44-101). opts: dx, u10Max, heightM, base (static10 directory),
  *  synthetic (generator spec), nz, alpha, nsp, nsw, outflowClip, existingFlux, met, sideInflowDeg, metReinit, dirTol,
- *  allowAnyDx, onProgress, device (HIP ordinal), storage ('f32' | 'f16' | 'bf16'), block */
+ *  allowAnyDx, onProgress, device (HIP ordinal), storage ('f32' | 'f16' | 'bf16'), block, autotune (tuning steps
+ *  per launch-shape candidate, run once after the flow is initialised; 0 or absent = default shape) */
 async function createSolver(opts) {
   const o = Object.assign({ dx: 30, u10Max: 8, heightM: 720, onProgress: () => {}, device: 0 }, opts || {});
   const prog = (t, f) => { try { o.onProgress(t, f); } catch (e) { /* UI errors ignored */ } };
@@ -39,6 +40,7 @@ async function createSolver(opts) {
   const s = new Solver(A, sc, o, prog);
   try {
     s._init();
+    if (o.autotune) await s.autotune(+o.autotune);
   } catch (e) {
     s.destroy();
     throw (e instanceof Error ? e : new Error(String(e)));
@@ -190,6 +192,15 @@ class Solver {
       return { simTime, surf, u, v, vol, volMax: vm, nzv: this.nzv, thetaMax, heatMW, steps };
     });
   }
+
+  /** launch-shape autotuner of this grid (no counterpart in the reference): times every block x levels x tile-window
+   *  candidate for `steps` steps, keeps the fastest and resolves to it; the state and simTime do not change */
+  autotune(steps) {
+    return this._enqueue(() => this._A.autotune(this._h, steps === undefined ? 4 : steps));
+  }
+
+  /** the launch shape in use, {block, levels, window} */
+  get launch() { this._check(); return this._A.getLaunch(this._h); }
 
   /** steps the model and returns MLUPS over all nx*ny*nz cells, device time from HIP events */
   benchmark(nSteps) {

@@ -28,7 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, 'ashburn-heat-twin_amd')
 METRIC = 'MLUPS & % HBM roofline at 600×600×64; scenarios/hour + p50/p99 latency'
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL = {0: 208, 1: 104, 2: 104, 3: 416}
+BYTES_PER_CELL = {0: 208, 1: 112, 2: 112, 3: 416}
 STORAGE = ['f32', 'f16', 'bf16', 'f64']
 # published reference number for this metric and grid: numba.cuda prototype step at 600x600x64, 10 m, RTX 4090
 # (README.md:37 of the reference, data/manifest.json:9468)

@@ -32,7 +32,8 @@ extern "C" {
 
 /* population storage formats */
 #define AHT_STORE_F32 0   /* f32 populations (208 B per fluid-cell update)                            */
-#define AHT_STORE_F16 1   /* f16 deviations (f_q - w_q)*2^10, theta_q*2^10 (104 B per update), f32 math */
+#define AHT_STORE_F16 1   /* f16 deviations (f_q - w_q)*2^10; theta in f32 + f16 (g_q - w_q theta)*2^10;
+                             112 B per update, f32 math                                                */
 #define AHT_STORE_BF16 2  /* bf16 deviations, same layout as F16                                      */
 #define AHT_STORE_F64 3   /* f64 populations and f64 arithmetic: the FP64 twin (416 B per update)     */
 
@@ -146,8 +147,10 @@ int aht_batch_size(const aht_batch *b);
  * and replayed while membership, n and the grid stay the same. *ms (may be NULL) = event time of the chunk. */
 int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t *acc_from, double *ms);
 int aht_batch_sync(aht_batch *b);
-/* persistent grid size: blocks > 0 fixes it, 0 = online tuning over 1, 2, 4, 8 blocks per CU on the first chunks of
- * each membership (every candidate computes the same result); use_graph 0 launches without graph capture */
+/* grid size of the batched step: blocks > 0 fixes it (clamped to the tile count: one block per tile), 0 = online
+ * tuning over 8 and 32 blocks per CU (persistent, looping over tiles) and one block per tile, one candidate per chunk,
+ * timed per tile-step, again whenever the batch's tile count has moved by more than 2x since the last tuning (every
+ * candidate computes the same result); use_graph 0 launches without graph capture */
 int aht_batch_set_grid(aht_batch *b, int32_t blocks, int32_t use_graph);
 int aht_batch_grid(const aht_batch *b);              /* chosen grid (0 while still tuning)                 */
 
@@ -155,7 +158,7 @@ int aht_batch_grid(const aht_batch *b);              /* chosen grid (0 while sti
 typedef struct {
   int32_t block;         /* threads per block: 64, 128, 192 or 256                                        */
   int32_t levels;        /* cells per thread along z: 1, 2 or 4                                           */
-  int32_t window;        /* tile order: 1 linear, W in 2..4096 = runs of W consecutive tiles per XCD,
+  int32_t window;        /* tile order: 1 linear, W = 2, 4, ..., 4096 = runs of W consecutive tiles per XCD,
                             0 = one contiguous slab of tiles per XCD                                      */
   int32_t reserved;
 } aht_launch;

@@ -11,7 +11,8 @@ fluid-volume heat content — in float64. From those it records what the GPU tes
   * the heat budget of the mean fields in lattice units (the C-ABI's aht_budget definitions) and in MW / MJ;
   * a fingerprint of the domain (column sums) so the test knows it rebuilt the identical inputs.
 
-Usage: python tests/golden/make_long_reference.py --case c2s [--threads 6] [--ckpt DIR] [--ckpt-every 100]
+Usage: python tests/golden/make_long_reference.py --case c1                (the committed record)
+       python tests/golden/make_long_reference.py --case c2s --every 64 [--threads 6] [--ckpt DIR] [--ckpt-every 100]
 The 600 x 600 x 64 case takes about an hour on 8 cores; --ckpt resumes an interrupted run.
 """
 import argparse

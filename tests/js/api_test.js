@@ -56,6 +56,15 @@ async function main() {
   const n2 = s.flowReinits;
   out.flowReinits = { afterSmallTurn: n1, afterBigTurn: n2 };
   out.mlups = await s.benchmark(200);
+  const tBefore = s.simTime, rBefore = await s.read();
+  out.autotune = await s.autotune(2);
+  const rAfter = await s.read();
+  let same = rBefore.surf.length === rAfter.surf.length;
+  for (let k = 0; same && k < rBefore.surf.length; k++) {
+    same = rBefore.surf[k] === rAfter.surf[k] && rBefore.u[k] === rAfter.u[k];
+  }
+  out.autotuneKeepsState = same && s.simTime === tBefore;
+  out.launch = s.launch;
   s.destroy();
   out.afterDestroy = await expectReject(Promise.resolve().then(() => s.step(1)));
   out.readAfterDestroy = await expectReject(Promise.resolve().then(() => s.read()));

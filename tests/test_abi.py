@@ -69,3 +69,32 @@ def test_create_without_gpu_fails_cleanly():
 def test_library_links_no_torch():
     out = subprocess.run(['ldd', os.path.join(PKG, 'lib', 'libaht.so')], capture_output=True, text=True).stdout
     assert 'torch' not in out and 'amdhip64' in out
+
+
+def _create_rc(m, nx, ny, nz, storage):
+    L = m.lib()
+    h = ctypes.c_void_p()
+    rc = L.aht_create(ctypes.byref(m.Dims(nx, ny, nz, storage, 0, 0)), ctypes.byref(h))
+    if h.value:
+        L.aht_destroy(h)
+    return rc
+
+
+def test_size_rules_are_checked_before_any_device_work():
+    """argument rules of aht_create, checked without a GPU (only the rejected shapes are tried where one is present):
+    nz <= 1022 (10-bit column fields); 32-bit byte offsets within a population plane and within a theta-referenced
+    D3Q7 lattice (18 B per cell for f16 / bf16)"""
+    m = load_pkg()
+    assert _create_rc(m, 8, 8, 1023, 0) == m.AHT_EINVAL
+    assert _create_rc(m, 2000, 2000, 60, m.STORE_F16) == m.AHT_EINVAL       # 18 * 240 M > 2^32
+    assert _create_rc(m, 2000, 2000, 60, m.STORE_BF16) == m.AHT_EINVAL
+    assert _create_rc(m, 1, 8, 8, 0) == m.AHT_EINVAL
+    try:
+        import torch  # noqa: F401  (only to know whether a GPU is present)
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if not has_gpu:
+        # accepted shapes get as far as the device lookup: 1200 x 1200 x 96 in f64 (plane 1.1 GB) and f16 (2.5 GB)
+        assert _create_rc(m, 1200, 1200, 96, m.STORE_F64) == m.AHT_EDEVICE
+        assert _create_rc(m, 1200, 1200, 96, m.STORE_F16) == m.AHT_EDEVICE

@@ -37,3 +37,6 @@ def test_api_sequence_matches_reference_record():
     assert out['afterDestroy'] == K['afterDestroy']
     assert out['readAfterDestroy'] == 'rejected: live solver: destroyed'
     assert out['mlups'] > 1000
+    # the launch-shape autotuner (no counterpart in the reference) leaves the simulation where it was
+    assert out['autotune']['block'] in (64, 128, 256) and out['autotune']['levels'] in (1, 2, 4)
+    assert out['autotuneKeepsState'] and out['launch'] == out['autotune']

@@ -55,3 +55,23 @@ def test_campus_rasterisation_conserves_area():
     assert abs(r['area'] - 300 * 200) < 1e-6
     assert abs(r['heat'] - 60e6) < 1e-3
     assert r['info'][0]['cells'] == 11 * 7          # columns 66..76, rows 107..113
+
+
+def test_serving_chunk_plan():
+    """the serving loop's iteration-level plan (js/serve.js planChunk): exact per-member step counts, averaging start
+    relative to the chunk, and the short-first rule (members with far more work left yield most launches)"""
+    code = """
+const { planChunk } = require('./ashburn-heat-twin_amd/js/serve.js');
+const mem = [{ cells: 320000, total: 4400, spin: 3400, done: 3390 },      // small, 1010 steps left
+             { cells: 23040000, total: 26000, spin: 20000, done: 100 },   // large
+             { cells: 1920000, total: 8800, spin: 6600, done: 8780 }];    // small, 20 steps left
+const r = planChunk(mem, 50, 'short-first', 3e10, 8);
+const f = planChunk(mem, 50, 'fifo', 3e10, 8);
+const big = planChunk([mem[1]], 50, 'short-first', 3e10, 8);
+console.log(JSON.stringify({ r: { n: r.n, ns: Array.from(r.ns), af: Array.from(r.af) },
+  f: { n: f.n, ns: Array.from(f.ns) }, big: Array.from(big.ns) }));
+"""
+    out = node_eval(code)
+    assert out['r'] == {'n': 50, 'ns': [50, 7, 20], 'af': [10, 19900, 0]}
+    assert out['f'] == {'n': 50, 'ns': [50, 50, 20]}
+    assert out['big'] == [50]                  # alone, a large member steps every launch
