@@ -18,7 +18,7 @@
 // The baseline (`mode: 'sequential'`) runs the same requests one at a time with the plain per-solver path.
 // Usage: node serve.js --rate 5 --requests 24 [--mode batch|sequential] [--spinup 2200] [--mean 600] [--seed 1]
 //        [--chunk 50] [--budget-gb 200] [--slots 64] [--policy short-first|fifo] [--short-work 3e10]
-//        [--long-share 8] [--out file.json]
+//        [--long-share 8] [--grid blocks (0 = online tuning; default one block per tile)] [--out file.json]
 
 const fs = require('fs');
 const { Scenario } = require('./scenario');
@@ -27,6 +27,10 @@ const { loadAddon } = require('./solver');
 const P = require('./physics');
 
 const STEP_ACC = 1;
+// batch grid: one workgroup per tile (the aht batch clamps the grid to the tile count). Measured at 600 x 600 x 64
+// plus four 200 x 200 x 48 domains, it beat the persistent grids (8 and 32 blocks per CU) for every mix; --grid 0
+// lets the batch's online tuner choose.
+const ONE_BLOCK_PER_TILE = 1 << 30;
 
 /** seeded request list: Poisson arrivals, grid mix, wind, campus */
 function makeRequests(o) {
@@ -138,7 +142,7 @@ function summary(mode, o, results, t0, t1, peakBytes, busyMs) {
 async function serveBatched(reqs, o) {
   const A = loadAddon();
   const B = A.batchCreate(o.device || 0, o.storage || 0);
-  A.batchSetGrid(B, o.grid || 0, o.useGraph === false ? 0 : 1);
+  A.batchSetGrid(B, o.grid === undefined ? ONE_BLOCK_PER_TILE : o.grid, o.useGraph === false ? 0 : 1);
   const budget = (o.budgetGB || 200) * 1e9, slots = o.slots || 64, chunk = o.chunk || 50;
   const shortWork = o.shortWork || 3e10, longShare = o.longShare || 8;
   const future = reqs.slice().sort((a, b) => a.arrival - b.arrival);
@@ -247,7 +251,8 @@ async function main() {
   const o = {
     rate: +(a.rate || 5), requests: +(a.requests || 24), seed: +(a.seed || 1), chunk: +(a.chunk || 50),
     spinupS: +(a.spinup || 2200), meanS: +(a.mean || 600), budgetGB: +(a['budget-gb'] || 200), slots: +(a.slots || 64),
-    storage: +(a.storage || 0), grid: +(a.grid || 0), useGraph: a['no-graph'] ? false : true,
+    storage: +(a.storage || 0), grid: a.grid === undefined ? ONE_BLOCK_PER_TILE : +a.grid,
+    useGraph: a['no-graph'] ? false : true,
     policy: a.policy || 'short-first', shortWork: +(a['short-work'] || 3e10), longShare: +(a['long-share'] || 8),
   };
   if (a.grids) o.grids = JSON.parse(a.grids);

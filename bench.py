@@ -63,6 +63,25 @@ def traffic_from_profiles(kernel_tag):
         return None
 
 
+def serving(rate):
+    """the serving half of the metric (scenarios/hour, p50/p99 latency): measured live by the Node serving loop when
+    rate > 0 (16 requests, batch and sequential, a few minutes), else the committed record of profiles/serving.json"""
+    if rate > 0:
+        out = os.path.join(tempfile.mkdtemp(prefix='aht_serve_'), 'serve.json')
+        subprocess.run(['node', os.path.join(PKG, 'js', 'serve.js'), '--rate', str(rate), '--requests', '16',
+                        '--mode', 'batch,sequential', '--out', out], check=True, stdout=subprocess.DEVNULL)
+        d = json.load(open(out))
+        subprocess.run(['rm', '-rf', os.path.dirname(out)])
+        return {'measured_in_this_run': True, 'rate_req_s': rate,
+                'batch': d['batch']['summary'], 'sequential': d['sequential']['summary']}
+    p = os.path.join(ROOT, 'profiles', 'serving.json')
+    if not os.path.exists(p):
+        return None
+    d = json.load(open(p))['summaries']
+    return {'measured_in_this_run': False, 'source': 'profiles/serving_c5.md', 'rate_req_s': 5,
+            'batch': d.get('batch_r5'), 'sequential': d.get('sequential_r5')}
+
+
 def cpu_baseline(dom, seconds):
     sys.path.insert(0, ROOT)
     from oracle import oracle as O
@@ -95,6 +114,8 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--dump', default=None, help='existing domain dump directory')
+    ap.add_argument('--serve', type=float, default=0.0,
+                    help='also measure the serving loop at this arrival rate (req/s); 0 = report the committed record')
     a = ap.parse_args()
     if a.gpus != 1:
         print(f'bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); the multi-GPU design is '
@@ -160,6 +181,7 @@ def main():
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic},
         'cpu_baseline': cpu,
+        'serving': serving(a.serve),
     }
     print(json.dumps(out), flush=True)
 
