@@ -29,6 +29,9 @@ struct aht_solver {
   size_t gbytes = 0;          // bytes of one D3Q7 lattice (theta-referenced formats add the f32 theta plane)
   void *f[2] = {nullptr, nullptr}, *gq[2] = {nullptr, nullptr};
   void *stage = nullptr;      // one population plane in f64, for state conversion
+  aht::Grid *gdev = nullptr;  // device copy of g read by the step kernel
+  aht::Grid *gpin = nullptr;  // its pinned staging copy
+  bool gdirty = true;         // g changed since the last upload
   uint32_t *colk = nullptr;
   float2 *colcq = nullptr;
   uint32_t *ktop = nullptr, *oldk = nullptr;
@@ -186,6 +189,8 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   HIP_TRY(s, hipMalloc(&s->vol, ((N + 3) / 4) * sizeof(uint32_t)));
   HIP_TRY(s, hipMalloc(&s->tmax, sizeof(uint32_t)));
   s->g.colk = s->colk; s->g.colcq = s->colcq; s->g.ktop = s->ktop; s->g.prof = s->prof;
+  HIP_TRY(s, hipMalloc(&s->gdev, sizeof(aht::Grid)));
+  HIP_TRY(s, hipHostMalloc((void **)&s->gpin, sizeof(aht::Grid), hipHostMallocDefault));
   rc = aht_mean_reset(s);
   if (rc) return rc;
   HIP_TRY(s, hipStreamSynchronize(s->st));
@@ -199,10 +204,11 @@ void aht_destroy(aht_solver *s) {
     (void)hipSetDevice(s->device);
     (void)hipStreamSynchronize(s->st);
   }
-  void *bufs[] = {s->f[0], s->f[1], s->gq[0], s->gq[1], s->stage, s->colk, s->colcq, s->ktop, s->oldk, s->prof, s->acc_surf, s->acc_face,
+  void *bufs[] = {s->f[0], s->f[1], s->gq[0], s->gq[1], s->stage, s->gdev, s->colk, s->colcq, s->ktop, s->oldk, s->prof, s->acc_surf, s->acc_face,
                   s->acc_content, s->red_out, s->stat_part, s->out4, s->surf3, s->vol, s->tmax};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  if (s->gpin) (void)hipHostFree(s->gpin);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->after_batch) (void)hipEventDestroy(s->after_batch);
@@ -244,6 +250,7 @@ int aht_set_columns(aht_solver *s, const uint32_t *kt, const uint32_t *ktop, con
   HIP_TRY(s, hipMemcpyAsync(s->ktop, ktop, NXY * sizeof(uint32_t), hipMemcpyHostToDevice, s->st));
   HIP_TRY(s, hipStreamSynchronize(s->st));
   s->g.kmin = kmin;
+  s->gdirty = true;
 
   s->h_q.assign(q, q + NXY);
   s->has_cols = true;
@@ -253,11 +260,14 @@ int aht_set_columns(aht_solver *s, const uint32_t *kt, const uint32_t *ktop, con
 int aht_set_params(aht_solver *s, const aht_params *p, const float *prof) {
   if (!s || !p || !prof) return s ? fail(s, AHT_EINVAL, "null params") : AHT_EINVAL;
   if (p->nsp < 1 || p->nsw < 1) return fail(s, AHT_EINVAL, "nsp and nsw must be >= 1");
+  if (!(p->prt > 0.0f)) return fail(s, AHT_EINVAL, "prt must be > 0");
   const float *pf = &p->tau0;
   for (int k = 0; k < 8; k++)
     if (!std::isfinite(pf[k])) return fail(s, AHT_EINVAL, "non-finite parameter %d", k);
   static_assert(sizeof(aht::Params) == sizeof(aht_params), "parameter block layout");
   memcpy(&s->g.P, p, sizeof(aht_params));
+  s->gdirty = true;
+  s->g.rp = aht::ParamRecip{(float)(1.0 / (double)p->prt), (float)(1.0 / (double)p->nsp), (float)(1.0 / (double)p->nsw)};
   int rc = set_device(s);
   if (rc) return rc;
   HIP_TRY(s, hipMemcpyAsync(s->prof, prof, s->g.nz * sizeof(float4), hipMemcpyHostToDevice, s->st));
@@ -295,11 +305,23 @@ int aht_reinit_cells(aht_solver *s, const uint32_t *old_ktop) {
   return AHT_OK;
 }
 
+// the step kernel's device copy of the grid follows s->g; uploaded (rarely: columns, parameters, launch shape) after
+// the stream has drained, so no launch still reads the old copy and the pinned staging copy is free to overwrite
+static int upload_grid(aht_solver *s) {
+  if (!s->gdirty) return AHT_OK;
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  *s->gpin = s->g;
+  HIP_TRY(s, hipMemcpyAsync(s->gdev, s->gpin, sizeof(aht::Grid), hipMemcpyHostToDevice, s->st));
+  s->gdirty = false;
+  return AHT_OK;
+}
+
 int aht_step(aht_solver *s, int32_t n, uint32_t flags) {
   int rc = check_ready(s);
   if (rc) return rc;
   if (n < 0) return fail(s, AHT_EINVAL, "negative step count");
   if ((rc = set_device(s))) return rc;
+  if ((rc = upload_grid(s))) return rc;
   const bool acc = flags & AHT_STEP_ACC, out = flags & AHT_STEP_OUT;
   if (out && !s->out4) HIP_TRY(s, hipMalloc(&s->out4, 4 * (size_t)s->g.N * sizeof(float)));
   aht::Acc a{s->acc_surf, s->acc_face, s->acc_content};
@@ -307,8 +329,8 @@ int aht_step(aht_solver *s, int32_t n, uint32_t flags) {
     const int p = s->parity;
     float *o4 = (out && it == n - 1) ? s->out4 : nullptr;
     if (o4) HIP_TRY(s, hipMemsetAsync(o4, 0, 4 * (size_t)s->g.N * sizeof(float), s->st));
-    HIP_TRY(s, aht::launch_step(s->storage, s->g, s->f[p], s->f[1 - p], s->gq[p], s->gq[1 - p], acc ? &a : nullptr, o4,
-                                s->block, s->levels, s->st));
+    HIP_TRY(s, aht::launch_step(s->storage, s->g, s->gdev, s->f[p], s->f[1 - p], s->gq[p], s->gq[1 - p],
+                                acc ? &a : nullptr, o4, s->block, s->levels, s->st));
     s->parity ^= 1;
     s->steps++;
     if (acc) s->n_acc++;
@@ -802,6 +824,7 @@ void apply_launch(aht_solver *s, const aht_launch &l) {
   s->block = l.block;
   s->levels = l.levels;
   s->g.win = (uint32_t)l.window;
+  s->gdirty = true;
 }
 }  // namespace
 

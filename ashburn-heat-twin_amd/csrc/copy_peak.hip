@@ -10,9 +10,11 @@
 // cells); `lattice_pull_nt` is the pull with non-temporal stores (as the step stores), `lattice16_pull_nt` the same
 // for the 16-bit formats: 26 arrays of 2-byte elements plus an f32 theta array pulled at the 7 D3Q7 offsets. They
 // give the rates attainable with 52+ concurrent streams, a tighter bound for the step than the plain copy.
+// `lattice_aosoa_pull_nt` is the f32 pull pattern on a blocked layout [cell / 64][q][cell % 64] (26 populations of
+// 64 consecutive cells side by side), the bound of a layout in which a wavefront's streams are one contiguous region.
 // Usage: aht_copy_peak [bytes] [reps] [variant]
 //   (variant: all | grid | flat | unroll4 | unroll4_xcd | unroll4_nt | lattice | lattice_pull | lattice_aa_even
-//    | lattice_aa_odd | lattice_pull_nt | lattice16_pull_nt)
+//    | lattice_aa_odd | lattice_pull_nt | lattice16_pull_nt | lattice_aosoa_pull_nt)
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -129,6 +131,22 @@ __global__ __launch_bounds__(256) void copy_lattice_aa(float *__restrict__ a, ui
   }
 }
 
+// blocked layout: element (q, n) at ((n / 64) * 26 + q) * 64 + n % 64
+__global__ __launch_bounds__(256) void copy_lattice_aosoa(const float *__restrict__ a, float *__restrict__ b, uint32_t N,
+                                                          int nx, int nxy) {
+  const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  float v[26];
+#pragma unroll
+  for (int q = 0; q < 26; q++) {
+    int64_t src = (int64_t)n - (LCX[q] + LCY[q] * nx + LCZ[q] * nxy);
+    src = src < 0 ? 0 : (src >= (int64_t)N ? (int64_t)N - 1 : src);
+    v[q] = a[((size_t)(src >> 6) * 26 + q) * 64 + (src & 63)];
+  }
+#pragma unroll
+  for (int q = 0; q < 26; q++) __builtin_nontemporal_store(v[q], &b[((size_t)(n >> 6) * 26 + q) * 64 + (n & 63)]);
+}
+
 #define CK(x)                                                                     \
   do {                                                                            \
     hipError_t e = (x);                                                           \
@@ -153,13 +171,14 @@ int main(int argc, char **argv) {
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
-  const char *names[11] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
-                           "lattice_aa_even", "lattice_aa_odd", "lattice_pull_nt", "lattice16_pull_nt"};
+  const char *names[12] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
+                           "lattice_aa_even", "lattice_aa_odd", "lattice_pull_nt", "lattice16_pull_nt",
+                           "lattice_aosoa_pull_nt"};
   const int lnx = 600, lnxy = 600 * 600;
   const uint32_t lN = 600u * 600u * 64u;   // lattice variants: 26 * lN floats each way (2.4 GB)
   double best = 0;
   const char *best_name = "";
-  for (int v = 0; v < 11; v++) {
+  for (int v = 0; v < 12; v++) {
     if (strcmp(which, "all") && strcmp(which, names[v])) continue;
     if (v >= 5 && n * 16 < (size_t)26 * lN * 4) continue;   // the lattice needs 2.4 GB per buffer
     auto launch = [&]() {
@@ -175,6 +194,7 @@ int main(int argc, char **argv) {
         case 8: hipLaunchKernelGGL((copy_lattice_aa<true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
         case 9: hipLaunchKernelGGL((copy_lattice<true, true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 10: hipLaunchKernelGGL(copy_lattice16, dim3((lN + 255) / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy); break;
+        case 11: hipLaunchKernelGGL(copy_lattice_aosoa, dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
       }
     };
     launch();

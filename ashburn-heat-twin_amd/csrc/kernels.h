@@ -22,6 +22,7 @@ struct Grid {
   const uint32_t *ktop;
   const float4 *prof;  // [nz] (u_in, v_in, kb, 0)
   Params P;
+  ParamRecip rp;       // 1/prt, 1/nsp, 1/nsw
 };
 
 // FP64 time-mean accumulators (AHT_STEP_ACC)
@@ -60,8 +61,11 @@ size_t g_lattice_bytes(int storage, size_t N);          // bytes of one D3Q7 lat
 // kernels are templated on the storage format; the launchers dispatch on the AHT_STORE_* id
 // levels = cells per thread along z (1, 2 or 4): the tile is (block of columns, chunk of `levels` levels);
 // g.win sets the order in which blocks take tiles
-hipError_t launch_step(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB, const Acc *acc,
-                       float *out4, int block, int levels, hipStream_t st);
+// gd = device copy of g: the kernel reads the grid through a pointer, so the compiler loads fields where they are used
+// (as a by-value argument the grid was pinned in SGPRs, which spilled into VGPR lanes: ~120 v_readlane/v_writelane
+// per cell of the f32 fast path)
+hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const void *fA, void *fB, const void *gA, void *gB,
+                       const Acc *acc, float *out4, int block, int levels, hipStream_t st);
 // one step of every batch member (launch index `it` of a chunk) on a persistent grid of `grid` blocks of 256
 hipError_t launch_batch_step(int storage, const Member *mem, int nmem, uint32_t total_tiles, int it, int grid,
                              hipStream_t st);

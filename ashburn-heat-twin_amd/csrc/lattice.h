@@ -44,6 +44,19 @@ struct Params {
   int inflow_w, inflow_e, inflow_s, inflow_n, nsp, nsw, oclip, pad;
 };
 
+// reciprocals of the uniform parameters that the step divides by (host-computed once per parameter set)
+struct ParamRecip {
+  float prt, nsp, nsw;
+};
+
+// x / d for a parameter d with 1/d given: a multiply in f32 (within an ulp of the division, and no per-lane IEEE
+// division sequence), the division itself in the FP64 twin (kept at the f64 oracle's rounding)
+template <typename R>
+__device__ __forceinline__ R div_param(R x, float d, float inv_d) {
+  if constexpr (sizeof(R) == 8) return x / R(d);
+  else return x * inv_d;
+}
+
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
@@ -97,8 +110,8 @@ struct Forces {
 // buoyancy, limited log-law drag at the first cell of building-free columns, top sponge, force-corrected velocity
 // (js/live/kernel.js:279-306). DRAG / SPONGE = false drop branches the caller has proven inactive.
 template <bool DRAG, bool SPONGE, typename R>
-__device__ __forceinline__ Forces<R> forces(const Params &P, int NZ, int k, int kg, int ktc, R cd, R uI, R vI,
-                                            const Moments<R> &m) {
+__device__ __forceinline__ Forces<R> forces(const Params &P, const ParamRecip &rp, int NZ, int k, int kg, int ktc, R cd,
+                                            R uI, R vI, const Moments<R> &m) {
   Forces<R> o;
   const R rho = m.rho, ir = rcp(rho);
   o.inv_rho = ir;
@@ -114,7 +127,7 @@ __device__ __forceinline__ Forces<R> forces(const Params &P, int NZ, int k, int 
   }
   R sg = R(0);
   if (SPONGE && k >= NZ - P.nsp) {
-    const R aa = (R(k - (NZ - P.nsp)) + R(1)) / R(P.nsp);
+    const R aa = div_param(R(k - (NZ - P.nsp)) + R(1), (float)P.nsp, rp.nsp);
     sg = R(P.sigmax) * aa * aa;
     const R rs = rho * sg;
     Fx += rs * (uI - m.jx * ir);
@@ -136,8 +149,8 @@ struct Relax {
 };
 
 template <typename R>
-__device__ __forceinline__ Relax<R> relax(const Params &P, int NX, int NY, int i, int j, R kb, const Moments<R> &m,
-                                          const Forces<R> &F) {
+__device__ __forceinline__ Relax<R> relax(const Params &P, const ParamRecip &rp, int NX, int NY, int i, int j, R kb,
+                                          const Moments<R> &m, const Forces<R> &F) {
   Relax<R> r;
   const R rho = m.rho, ux = F.ux, uy = F.uy, uz = F.uz;
   const R third = R(1.0 / 3.0);
@@ -159,12 +172,12 @@ __device__ __forceinline__ Relax<R> relax(const Params &P, int NX, int NY, int i
   if (P.inflow_s == 0) dmin = min(dmin, j);
   if (P.inflow_n == 0) dmin = min(dmin, NY - 1 - j);
   if (dmin < P.nsw) {
-    const R rr = (R(P.nsw) - R(dmin)) / R(P.nsw);
+    const R rr = div_param(R(P.nsw) - R(dmin), (float)P.nsw, rp.nsw);
     tb += R(0.15) * rr * rr;
   }
   const R tau = R(0.5) * (tb + fsqrt(tb * tb + R(P.cs2smag) * Pn * F.inv_rho));
   const R nusgs = (tau - tb) * third;
-  const R kap = nusgs / R(P.prt) + kb + R(P.kfloor);   // P.prt is uniform: one division per wave
+  const R kap = div_param(nusgs, P.prt, rp.prt) + kb + R(P.kfloor);
   const R itau = rcp(tau);
   const R om = R(1) - itau, fpre = R(1) - R(0.5) * itau;
   r.omg = rcp(R(0.5) + R(4) * kap);

@@ -361,8 +361,8 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const void *fA, 
     R f[19], gg[7];
     stream_cell<S>(g, fA, gA, n, i, j, k, kg, fast, R(pr.x), R(pr.y), f, gg);
     const Moments<R> m = moments(f, gg);
-    const Forces<R> F = forces<true, true>(g.P, g.nz, k, kg, ktc, R(cq.x), R(pr.x), R(pr.y), m);
-    const Relax<R> Rx = relax(g.P, g.nx, g.ny, i, j, R(pr.z), m, F);
+    const Forces<R> F = forces<true, true>(g.P, g.rp, g.nz, k, kg, ktc, R(cq.x), R(pr.x), R(pr.y), m);
+    const Relax<R> Rx = relax(g.P, g.rp, g.nx, g.ny, i, j, R(pr.z), m, F);
     const uint32_t boff = n * (uint32_t)S::bytes;
     S::st_raw(fB, boff, S::enc_f(collide_rest(Rx), w19<0>()));
     StorePairs<1>::template run<S>(fB, N, boff, m, F, Rx);
@@ -431,8 +431,10 @@ __device__ __forceinline__ void step_cell(const Grid &g, const void *fA, void *f
 // one block per (column block, KB-level chunk) tile, numbered column block fastest, then level chunk, taken in the
 // order g.win selects; a thread walks its column through the KB levels (column word and index math once per column)
 template <class S, bool ACC, bool OUT, int KB>
-__global__ __launch_bounds__(256) void step_kernel(Grid g, const void *fA, void *fB, const void *gA, void *gB, Acc acc,
-                                                   float *__restrict__ out4, uint32_t bx, float inv_bx) {
+__global__ __launch_bounds__(256) void step_kernel(const Grid *__restrict__ gp, const void *fA, void *fB, const void *gA,
+                                                   void *gB, Acc acc, float *__restrict__ out4, uint32_t bx,
+                                                   float inv_bx) {
+  const Grid &g = *gp;
   const uint32_t t = tile_of_block(blockIdx.x, gridDim.x, g.win);
   uint32_t cb;
   const uint32_t kk = div_by(t, bx, inv_bx, cb);
@@ -506,7 +508,7 @@ __global__ __launch_bounds__(256) void surface_kernel(Grid g, const void *fA, co
   R f[19], gg[7];
   stream_cell<S>(g, fA, gA, n, i, j, k, kg, false, R(pr.x), R(pr.y), f, gg);
   const Moments<R> m = moments(f, gg);
-  const Forces<R> F = forces<true, true>(g.P, g.nz, k, kg, ktc, R(g.colcq[c].x), R(pr.x), R(pr.y), m);
+  const Forces<R> F = forces<true, true>(g.P, g.rp, g.nz, k, kg, ktc, R(g.colcq[c].x), R(pr.x), R(pr.y), m);
   out3[o] = (float)m.th;
   out3[g.NXY + o] = (float)(F.ux * R(conv));
   out3[2 * g.NXY + o] = (float)(F.uy * R(conv));
@@ -754,26 +756,26 @@ size_t g_lattice_bytes(int storage, size_t N) {
   }
 
 template <class S, int KB>
-static void launch_step_kb(const Grid &g, const void *fA, void *fB, const void *gA, void *gB, const Acc *acc,
-                           float *out4, int block, hipStream_t st) {
+static void launch_step_kb(const Grid &g, const Grid *gd, const void *fA, void *fB, const void *gA, void *gB,
+                           const Acc *acc, float *out4, int block, hipStream_t st) {
   const uint32_t bx = (g.NXY + block - 1) / block;
   const uint32_t nk = (uint32_t)(g.nz - g.kmin + KB - 1) / KB;
   const dim3 grid(bx * nk);
   const Acc a = acc ? *acc : Acc{nullptr, nullptr, nullptr};
   const float ib = 1.0f / (float)bx;
-  if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx, ib);
-  else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx, ib);
-  else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx, ib);
-  else hipLaunchKernelGGL((step_kernel<S, false, false, KB>), grid, dim3(block), 0, st, g, fA, fB, gA, gB, a, out4, bx, ib);
+  if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true, KB>), grid, dim3(block), 0, st, gd, fA, fB, gA, gB, a, out4, bx, ib);
+  else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false, KB>), grid, dim3(block), 0, st, gd, fA, fB, gA, gB, a, out4, bx, ib);
+  else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true, KB>), grid, dim3(block), 0, st, gd, fA, fB, gA, gB, a, out4, bx, ib);
+  else hipLaunchKernelGGL((step_kernel<S, false, false, KB>), grid, dim3(block), 0, st, gd, fA, fB, gA, gB, a, out4, bx, ib);
 }
 
-hipError_t launch_step(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB, const Acc *acc,
-                       float *out4, int block, int levels, hipStream_t st) {
+hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const void *fA, void *fB, const void *gA, void *gB,
+                       const Acc *acc, float *out4, int block, int levels, hipStream_t st) {
   if (g.kmin >= g.nz) return hipSuccess;
   AHT_DISPATCH(storage, {
-    if (levels == 4) (launch_step_kb<S, 4>)(g, fA, fB, gA, gB, acc, out4, block, st);
-    else if (levels == 2) (launch_step_kb<S, 2>)(g, fA, fB, gA, gB, acc, out4, block, st);
-    else (launch_step_kb<S, 1>)(g, fA, fB, gA, gB, acc, out4, block, st);
+    if (levels == 4) (launch_step_kb<S, 4>)(g, gd, fA, fB, gA, gB, acc, out4, block, st);
+    else if (levels == 2) (launch_step_kb<S, 2>)(g, gd, fA, fB, gA, gB, acc, out4, block, st);
+    else (launch_step_kb<S, 1>)(g, gd, fA, fB, gA, gB, acc, out4, block, st);
   })
   return hipGetLastError();
 }
