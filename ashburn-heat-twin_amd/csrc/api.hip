@@ -16,6 +16,12 @@
 
 struct aht_batch;
 
+// what the step kernel reads through pointers: the grid and the stream tables of both lattice parities
+struct DevGrid {
+  aht::Grid g;
+  aht::Streams s[2];   // s[p]: the step that reads lattice pair p and writes pair 1 - p
+};
+
 struct aht_solver {
   aht_dims dims{};
   int device = 0;
@@ -29,8 +35,8 @@ struct aht_solver {
   size_t gbytes = 0;          // bytes of one D3Q7 lattice (theta-referenced formats add the f32 theta plane)
   void *f[2] = {nullptr, nullptr}, *gq[2] = {nullptr, nullptr};
   void *stage = nullptr;      // one population plane in f64, for state conversion
-  aht::Grid *gdev = nullptr;  // device copy of g read by the step kernel
-  aht::Grid *gpin = nullptr;  // its pinned staging copy
+  DevGrid *gdev = nullptr;    // device copy of g and of its stream tables, read by the step kernel
+  DevGrid *gpin = nullptr;    // its pinned staging copy
   bool gdirty = true;         // g changed since the last upload
   uint32_t *colk = nullptr;
   float2 *colcq = nullptr;
@@ -189,8 +195,8 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   HIP_TRY(s, hipMalloc(&s->vol, ((N + 3) / 4) * sizeof(uint32_t)));
   HIP_TRY(s, hipMalloc(&s->tmax, sizeof(uint32_t)));
   s->g.colk = s->colk; s->g.colcq = s->colcq; s->g.ktop = s->ktop; s->g.prof = s->prof;
-  HIP_TRY(s, hipMalloc(&s->gdev, sizeof(aht::Grid)));
-  HIP_TRY(s, hipHostMalloc((void **)&s->gpin, sizeof(aht::Grid), hipHostMallocDefault));
+  HIP_TRY(s, hipMalloc(&s->gdev, sizeof(DevGrid)));
+  HIP_TRY(s, hipHostMalloc((void **)&s->gpin, sizeof(DevGrid), hipHostMallocDefault));
   rc = aht_mean_reset(s);
   if (rc) return rc;
   HIP_TRY(s, hipStreamSynchronize(s->st));
@@ -310,8 +316,10 @@ int aht_reinit_cells(aht_solver *s, const uint32_t *old_ktop) {
 static int upload_grid(aht_solver *s) {
   if (!s->gdirty) return AHT_OK;
   HIP_TRY(s, hipStreamSynchronize(s->st));
-  *s->gpin = s->g;
-  HIP_TRY(s, hipMemcpyAsync(s->gdev, s->gpin, sizeof(aht::Grid), hipMemcpyHostToDevice, s->st));
+  s->gpin->g = s->g;
+  for (int p = 0; p < 2; p++)
+    s->gpin->s[p] = aht::make_streams(s->storage, s->g, s->f[p], s->f[1 - p], s->gq[p], s->gq[1 - p]);
+  HIP_TRY(s, hipMemcpyAsync(s->gdev, s->gpin, sizeof(DevGrid), hipMemcpyHostToDevice, s->st));
   s->gdirty = false;
   return AHT_OK;
 }
@@ -329,8 +337,8 @@ int aht_step(aht_solver *s, int32_t n, uint32_t flags) {
     const int p = s->parity;
     float *o4 = (out && it == n - 1) ? s->out4 : nullptr;
     if (o4) HIP_TRY(s, hipMemsetAsync(o4, 0, 4 * (size_t)s->g.N * sizeof(float), s->st));
-    HIP_TRY(s, aht::launch_step(s->storage, s->g, s->gdev, s->f[p], s->f[1 - p], s->gq[p], s->gq[1 - p],
-                                acc ? &a : nullptr, o4, s->block, s->levels, s->st));
+    HIP_TRY(s, aht::launch_step(s->storage, s->g, &s->gdev->g, &s->gdev->s[p], s->f[p], s->f[1 - p], s->gq[p],
+                                s->gq[1 - p], acc ? &a : nullptr, o4, s->block, s->levels, s->st));
     s->parity ^= 1;
     s->steps++;
     if (acc) s->n_acc++;

@@ -32,6 +32,15 @@ struct Acc {
   double *content;  // one slot per wavefront of the step grid: sum of theta over its fluid cells
 };
 
+// Stream bases of the single-domain fast path, host-built once per lattice parity (the step that reads lattice pair A
+// and writes pair B): rd = the base of every read stream with its pull offset folded in (population plane q shifted
+// back by c_q . (1, nx, nx ny) cells), wr = every written plane. A wavefront loads these 64-bit bases through the
+// scalar cache instead of forming 60 64-bit addresses with scalar multiplies and adds.
+struct Streams {
+  uint64_t rd[33];   // f_q (0..18), g_q (19..25), the theta plane pulled along D3Q7 velocity q (26..32, theta_ref)
+  uint64_t wr[27];   // f_q planes (0..18), g_q planes (19..25), the theta plane (26, theta_ref)
+};
+
 constexpr int STEP_BLOCK = 256;
 constexpr uint32_t NUM_XCD = 8;   // MI355X: workgroups are dispatched round-robin over 8 XCDs, each with its own L2
 
@@ -63,9 +72,10 @@ size_t g_lattice_bytes(int storage, size_t N);          // bytes of one D3Q7 lat
 // g.win sets the order in which blocks take tiles
 // gd = device copy of g: the kernel reads the grid through a pointer, so the compiler loads fields where they are used
 // (as a by-value argument the grid was pinned in SGPRs, which spilled into VGPR lanes: ~120 v_readlane/v_writelane
-// per cell of the f32 fast path)
-hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const void *fA, void *fB, const void *gA, void *gB,
-                       const Acc *acc, float *out4, int block, int levels, hipStream_t st);
+// per cell of the f32 fast path); sd = device copy of make_streams(storage, g, fA, fB, gA, gB)
+hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const Streams *sd, const void *fA, void *fB,
+                       const void *gA, void *gB, const Acc *acc, float *out4, int block, int levels, hipStream_t st);
+Streams make_streams(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB);
 // one step of every batch member (launch index `it` of a chunk) on a persistent grid of `grid` blocks of 256
 hipError_t launch_batch_step(int storage, const Member *mem, int nmem, uint32_t total_tiles, int it, int grid,
                              hipStream_t st);

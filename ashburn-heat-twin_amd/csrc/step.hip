@@ -101,18 +101,28 @@ __device__ __forceinline__ typename S::real ldg(const Grid &g, const void *gA, i
   else return S::dec_g(e, 0.0f, typename S::real(0));
 }
 
+// Stream bases. TAB = true (the single-domain step) reads them from the host-built Streams table; TAB = false
+// (batched launches, the surface read) forms them from the lattice pointers.
+template <class S, bool TAB>
+__device__ __forceinline__ void *plane_base(const Streams *sp, int slot, void *lat, size_t N, int q) {
+  if constexpr (TAB) return (void *)sp->wr[slot];
+  else return plane<S>(lat, N, q);
+}
+
 // the seven post-collision theta populations p of cell n: raw, or theta = sum_q p_q in f32 plus the deviations
-template <class S>
-__device__ __forceinline__ void store_g(const Grid &g, void *gB, uint32_t n, const typename S::real (&p)[7]) {
+template <class S, bool TAB>
+__device__ __forceinline__ void store_g(const Grid &g, const Streams *sp, void *gB, uint32_t n,
+                                        const typename S::real (&p)[7]) {
   using R = typename S::real;
   const uint32_t boff = n * (uint32_t)S::bytes;
   R th = R(0);
   if constexpr (S::theta_ref) {
     th = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + p[6]);
-    st_theta(gB, g.th_off + n * 4u, th);
+    if constexpr (TAB) st_theta((void *)sp->wr[26], n * 4u, th);
+    else st_theta(gB, g.th_off + n * 4u, th);
   }
 #pragma unroll
-  for (int q = 0; q < 7; q++) S::st_raw(plane<S>(gB, g.N, q), boff, S::enc_g(p[q], w7q(q), th));
+  for (int q = 0; q < 7; q++) S::st_raw(plane_base<S, TAB>(sp, 19 + q, gB, g.N, q), boff, S::enc_g(p[q], w7q(q), th));
 }
 
 template <int q>
@@ -270,49 +280,58 @@ struct Pull7<7> {
 // fast path: all 26 sources are fluid interior cells; plain shifted loads through per-population descriptors
 template <int q>
 struct FastPull {
-  template <class S>
+  template <class S, bool TAB>
   __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
-                                             const void *fA, const void *gA, uint32_t n) {
+                                             const Streams *sp, const void *fA, const void *gA, uint32_t n) {
     const uint32_t boff = n * (uint32_t)S::bytes;
-    const int off = CX[q] + CY[q] * g.nx + CZ[q] * (int)g.NXY;
-    f[q] = S::dec_f(S::ld_raw(shifted<S>(fA, g.N, q, off), boff), w19<q>());
+    const void *bf;
+    if constexpr (TAB) bf = (const void *)sp->rd[q];
+    else bf = shifted<S>(fA, g.N, q, CX[q] + CY[q] * g.nx + CZ[q] * (int)g.NXY);
+    f[q] = S::dec_f(S::ld_raw(bf, boff), w19<q>());
     if constexpr (q < 7) {
       const int o7 = CX7[q] + CY7[q] * g.nx + CZ7[q] * (int)g.NXY;
-      const typename S::elem e = S::ld_raw(shifted<S>(gA, g.N, q, o7), boff);
+      const void *bg;
+      if constexpr (TAB) bg = (const void *)sp->rd[19 + q];
+      else bg = shifted<S>(gA, g.N, q, o7);
+      const typename S::elem e = S::ld_raw(bg, boff);
       if constexpr (S::theta_ref) {
-        const void *thb = (const void *)((uintptr_t)gA + g.th_off - (intptr_t)o7 * 4);   // theta plane, pulled
+        const void *thb;                                                  // theta plane, pulled
+        if constexpr (TAB) thb = (const void *)sp->rd[26 + q];
+        else thb = (const void *)((uintptr_t)gA + g.th_off - (intptr_t)o7 * 4);
         gg[q] = S::dec_g(e, w7q(q), ld_theta(thb, n * 4u));
       } else {
         gg[q] = S::dec_g(e, 0.0f, typename S::real(0));
       }
     }
-    FastPull<q + 1>::template run<S>(f, gg, g, fA, gA, n);
+    FastPull<q + 1>::template run<S, TAB>(f, gg, g, sp, fA, gA, n);
   }
 };
 template <>
 struct FastPull<19> {
-  template <class S>
+  template <class S, bool TAB>
   __device__ __forceinline__ static void run(typename S::real (&)[19], typename S::real (&)[7], const Grid &,
-                                             const void *, const void *, uint32_t) {}
+                                             const Streams *, const void *, const void *, uint32_t) {}
 };
 
 template <int q>
 struct StorePairs {
-  template <class S>
-  __device__ __forceinline__ static void run(void *fB, size_t N, uint32_t boff, const Moments<typename S::real> &m,
-                                             const Forces<typename S::real> &F, const Relax<typename S::real> &R) {
+  template <class S, bool TAB>
+  __device__ __forceinline__ static void run(const Streams *sp, void *fB, size_t N, uint32_t boff,
+                                             const Moments<typename S::real> &m, const Forces<typename S::real> &F,
+                                             const Relax<typename S::real> &R) {
     typename S::real a, b;
     collide_pair<q>(m, F, R, a, b);
-    S::st_raw(plane<S>(fB, N, q), boff, S::enc_f(a, w19<q>()));
-    S::st_raw(plane<S>(fB, N, q + 1), boff, S::enc_f(b, w19<q + 1>()));
-    StorePairs<q + 2>::template run<S>(fB, N, boff, m, F, R);
+    S::st_raw(plane_base<S, TAB>(sp, q, fB, N, q), boff, S::enc_f(a, w19<q>()));
+    S::st_raw(plane_base<S, TAB>(sp, q + 1, fB, N, q + 1), boff, S::enc_f(b, w19<q + 1>()));
+    StorePairs<q + 2>::template run<S, TAB>(sp, fB, N, boff, m, F, R);
   }
 };
 template <>
 struct StorePairs<19> {
-  template <class S>
-  __device__ __forceinline__ static void run(void *, size_t, uint32_t, const Moments<typename S::real> &,
-                                             const Forces<typename S::real> &, const Relax<typename S::real> &) {}
+  template <class S, bool TAB>
+  __device__ __forceinline__ static void run(const Streams *, void *, size_t, uint32_t,
+                                             const Moments<typename S::real> &, const Forces<typename S::real> &,
+                                             const Relax<typename S::real> &) {}
 };
 
 __device__ __forceinline__ void load_neighbours(const Grid &g, int i, int j, int kg, int (&kn)[3][3]) {
@@ -325,13 +344,13 @@ __device__ __forceinline__ void load_neighbours(const Grid &g, int i, int j, int
 }
 
 // streaming of one fluid cell: the fast or the boundary path
-template <class S>
-__device__ __forceinline__ void stream_cell(const Grid &g, const void *fA, const void *gA, uint32_t n, int i, int j,
-                                            int k, int kg, bool fast, typename S::real uI, typename S::real vI,
-                                            typename S::real (&f)[19], typename S::real (&gg)[7]) {
+template <class S, bool TAB>
+__device__ __forceinline__ void stream_cell(const Grid &g, const Streams *sp, const void *fA, const void *gA,
+                                            uint32_t n, int i, int j, int k, int kg, bool fast, typename S::real uI,
+                                            typename S::real vI, typename S::real (&f)[19], typename S::real (&gg)[7]) {
   using E = typename S::elem;
   if (fast) {
-    FastPull<0>::template run<S>(f, gg, g, fA, gA, n);
+    FastPull<0>::template run<S, TAB>(f, gg, g, sp, fA, gA, n);
   } else {
     int kn[3][3];
     load_neighbours(g, i, j, kg, kn);
@@ -343,9 +362,10 @@ __device__ __forceinline__ void stream_cell(const Grid &g, const void *fA, const
 
 // one cell of the fused step (genStepWGSL, js/live/kernel.js:395-431) with FP64 accumulation (ACC) and debug outputs
 // (OUT). Every thread of the block must call it (the ACC heat-content sum is a wavefront reduction).
-template <class S, bool ACC, bool OUT>
-__device__ __forceinline__ double step_cell_body(const Grid &g, const void *fA, void *fB, const void *gA, void *gB,
-                                                 const Acc &acc, float *__restrict__ out4, int c, int k, uint32_t ck) {
+template <class S, bool ACC, bool OUT, bool TAB>
+__device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *sp, const void *fA, void *fB,
+                                                 const void *gA, void *gB, const Acc &acc, float *__restrict__ out4,
+                                                 int c, int k, uint32_t ck) {
   using R = typename S::real;
   const int ktc = col_kt(ck), kg = col_ktop(ck);
   const bool fluid = c < (int)g.NXY && k >= kg && k < g.nz;
@@ -359,13 +379,13 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const void *fA, 
     const bool fast = k >= col_kfast(ck) && k < g.nz - 1 && i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
     const float2 cq = (k == kg) ? g.colcq[c] : make_float2(0.0f, 0.0f);   // drag and source act at k == ktop only
     R f[19], gg[7];
-    stream_cell<S>(g, fA, gA, n, i, j, k, kg, fast, R(pr.x), R(pr.y), f, gg);
+    stream_cell<S, TAB>(g, sp, fA, gA, n, i, j, k, kg, fast, R(pr.x), R(pr.y), f, gg);
     const Moments<R> m = moments(f, gg);
     const Forces<R> F = forces<true, true>(g.P, g.rp, g.nz, k, kg, ktc, R(cq.x), R(pr.x), R(pr.y), m);
     const Relax<R> Rx = relax(g.P, g.rp, g.nx, g.ny, i, j, R(pr.z), m, F);
     const uint32_t boff = n * (uint32_t)S::bytes;
     S::st_raw(fB, boff, S::enc_f(collide_rest(Rx), w19<0>()));
-    StorePairs<1>::template run<S>(fB, N, boff, m, F, Rx);
+    StorePairs<1>::template run<S, TAB>(sp, fB, N, boff, m, F, Rx);
     // D3Q7: g^post = a g + b (1 + 4 c.u) + w S (1 - sg)
     const R Sq = (k == kg) ? R(cq.y) * R(g.P.srcscale) : R(0);
     const Scalar<R> sc = scalar_coeffs(m.th, Rx.omg, Sq, F.sg);
@@ -373,7 +393,7 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const void *fA, 
     const R gp[7] = {sc.a * gg[0] + sc.cs0,          sc.a * gg[1] + (b1 + b4 * F.ux), sc.a * gg[2] + (b1 - b4 * F.ux),
                      sc.a * gg[3] + (b1 + b4 * F.uy), sc.a * gg[4] + (b1 - b4 * F.uy), sc.a * gg[5] + (b1 + b4 * F.uz),
                      sc.a * gg[6] + (b1 - b4 * F.uz)};
-    store_g<S>(g, gB, n, gp);
+    store_g<S, TAB>(g, sp, gB, n, gp);
     if (OUT) {
       out4[n] = (float)m.th;
       out4[N + n] = (float)F.ux;
@@ -424,16 +444,16 @@ template <class S, bool ACC, bool OUT>
 __device__ __forceinline__ void step_cell(const Grid &g, const void *fA, void *fB, const void *gA, void *gB,
                                           const Acc &acc, float *__restrict__ out4, int c, int k, uint32_t slot) {
   const uint32_t ck = g.colk[c < (int)g.NXY ? c : 0];
-  const double th = step_cell_body<S, ACC, OUT>(g, fA, fB, gA, gB, acc, out4, c, k, ck);
+  const double th = step_cell_body<S, ACC, OUT, false>(g, nullptr, fA, fB, gA, gB, acc, out4, c, k, ck);
   if (ACC) content_add(acc, slot, th);
 }
 
 // one block per (column block, KB-level chunk) tile, numbered column block fastest, then level chunk, taken in the
 // order g.win selects; a thread walks its column through the KB levels (column word and index math once per column)
 template <class S, bool ACC, bool OUT, int KB>
-__global__ __launch_bounds__(256) void step_kernel(const Grid *__restrict__ gp, const void *fA, void *fB, const void *gA,
-                                                   void *gB, Acc acc, float *__restrict__ out4, uint32_t bx,
-                                                   float inv_bx) {
+__global__ __launch_bounds__(256) void step_kernel(const Grid *__restrict__ gp, const Streams *__restrict__ sp,
+                                                   const void *fA, void *fB, const void *gA, void *gB, Acc acc,
+                                                   float *__restrict__ out4, uint32_t bx, float inv_bx) {
   const Grid &g = *gp;
   const uint32_t t = tile_of_block(blockIdx.x, gridDim.x, g.win);
   uint32_t cb;
@@ -442,7 +462,8 @@ __global__ __launch_bounds__(256) void step_kernel(const Grid *__restrict__ gp, 
   const int k0 = g.kmin + (int)kk * KB;
   const uint32_t ck = g.colk[c < (int)g.NXY ? c : 0];
   double th = 0.0;
-  for (int u = 0; u < KB; u++) th += step_cell_body<S, ACC, OUT>(g, fA, fB, gA, gB, acc, out4, c, k0 + u, ck);
+  for (int u = 0; u < KB; u++)
+    th += step_cell_body<S, ACC, OUT, true>(g, sp, fA, fB, gA, gB, acc, out4, c, k0 + u, ck);
   if (ACC) content_add(acc, t * (blockDim.x >> 6) + (threadIdx.x >> 6), th);
 }
 
@@ -506,7 +527,7 @@ __global__ __launch_bounds__(256) void surface_kernel(Grid g, const void *fA, co
   const uint32_t n = (uint32_t)k * g.NXY + (uint32_t)c;
   const float4 pr = g.prof[max(k - ktc, 0)];
   R f[19], gg[7];
-  stream_cell<S>(g, fA, gA, n, i, j, k, kg, false, R(pr.x), R(pr.y), f, gg);
+  stream_cell<S, false>(g, nullptr, fA, gA, n, i, j, k, kg, false, R(pr.x), R(pr.y), f, gg);
   const Moments<R> m = moments(f, gg);
   const Forces<R> F = forces<true, true>(g.P, g.rp, g.nz, k, kg, ktc, R(g.colcq[c].x), R(pr.x), R(pr.y), m);
   out3[o] = (float)m.th;
@@ -756,28 +777,49 @@ size_t g_lattice_bytes(int storage, size_t N) {
   }
 
 template <class S, int KB>
-static void launch_step_kb(const Grid &g, const Grid *gd, const void *fA, void *fB, const void *gA, void *gB,
-                           const Acc *acc, float *out4, int block, hipStream_t st) {
+static void launch_step_kb(const Grid &g, const Grid *gd, const Streams *sd, const void *fA, void *fB, const void *gA,
+                           void *gB, const Acc *acc, float *out4, int block, hipStream_t st) {
   const uint32_t bx = (g.NXY + block - 1) / block;
   const uint32_t nk = (uint32_t)(g.nz - g.kmin + KB - 1) / KB;
   const dim3 grid(bx * nk);
   const Acc a = acc ? *acc : Acc{nullptr, nullptr, nullptr};
   const float ib = 1.0f / (float)bx;
-  if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true, KB>), grid, dim3(block), 0, st, gd, fA, fB, gA, gB, a, out4, bx, ib);
-  else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false, KB>), grid, dim3(block), 0, st, gd, fA, fB, gA, gB, a, out4, bx, ib);
-  else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true, KB>), grid, dim3(block), 0, st, gd, fA, fB, gA, gB, a, out4, bx, ib);
-  else hipLaunchKernelGGL((step_kernel<S, false, false, KB>), grid, dim3(block), 0, st, gd, fA, fB, gA, gB, a, out4, bx, ib);
+  if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib);
+  else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib);
+  else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib);
+  else hipLaunchKernelGGL((step_kernel<S, false, false, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib);
 }
 
-hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const void *fA, void *fB, const void *gA, void *gB,
-                       const Acc *acc, float *out4, int block, int levels, hipStream_t st) {
+hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const Streams *sd, const void *fA, void *fB,
+                       const void *gA, void *gB, const Acc *acc, float *out4, int block, int levels, hipStream_t st) {
   if (g.kmin >= g.nz) return hipSuccess;
   AHT_DISPATCH(storage, {
-    if (levels == 4) (launch_step_kb<S, 4>)(g, gd, fA, fB, gA, gB, acc, out4, block, st);
-    else if (levels == 2) (launch_step_kb<S, 2>)(g, gd, fA, fB, gA, gB, acc, out4, block, st);
-    else (launch_step_kb<S, 1>)(g, gd, fA, fB, gA, gB, acc, out4, block, st);
+    if (levels == 4) (launch_step_kb<S, 4>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, st);
+    else if (levels == 2) (launch_step_kb<S, 2>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, st);
+    else (launch_step_kb<S, 1>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, st);
   })
   return hipGetLastError();
+}
+
+// the bases FastPull / StorePairs / store_g form on the device with TAB = false, for one lattice parity
+Streams make_streams(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB) {
+  static const int cx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+  static const int cy[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+  static const int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+  const int64_t b = storage_bytes(storage), N = g.N;
+  Streams s{};
+  for (int q = 0; q < 19; q++) {
+    const int64_t off = cx[q] + (int64_t)cy[q] * g.nx + (int64_t)cz[q] * g.NXY;   // D3Q7 q < 7 shares c_q
+    s.rd[q] = (uint64_t)((uintptr_t)fA + (q * N - off) * b);
+    s.wr[q] = (uint64_t)((uintptr_t)fB + q * N * b);
+    if (q < 7) {
+      s.rd[19 + q] = (uint64_t)((uintptr_t)gA + (q * N - off) * b);
+      s.wr[19 + q] = (uint64_t)((uintptr_t)gB + q * N * b);
+      if (storage_theta_ref(storage)) s.rd[26 + q] = (uint64_t)((uintptr_t)gA + g.th_off - off * 4);
+    }
+  }
+  if (storage_theta_ref(storage)) s.wr[26] = (uint64_t)((uintptr_t)gB + g.th_off);
+  return s;
 }
 
 hipError_t launch_batch_step(int storage, const Member *mem, int nmem, uint32_t total_tiles, int it, int grid,
