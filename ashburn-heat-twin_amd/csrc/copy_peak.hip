@@ -12,9 +12,12 @@
 // give the rates attainable with 52+ concurrent streams, a tighter bound for the step than the plain copy.
 // `lattice_aosoa_pull_nt` is the f32 pull pattern on a blocked layout [cell / 64][q][cell % 64] (26 populations of
 // 64 consecutive cells side by side), the bound of a layout in which a wavefront's streams are one contiguous region.
+// `lattice_pull_nt_x2` / `_x4` move 2 / 4 consecutive cells per thread (8- / 16-byte accesses, the pulled loads
+// dword-aligned), the bound of a step that widens its per-lane accesses.
 // Usage: aht_copy_peak [bytes] [reps] [variant]
 //   (variant: all | grid | flat | unroll4 | unroll4_xcd | unroll4_nt | lattice | lattice_pull | lattice_aa_even
-//    | lattice_aa_odd | lattice_pull_nt | lattice16_pull_nt | lattice_aosoa_pull_nt)
+//    | lattice_aa_odd | lattice_pull_nt | lattice16_pull_nt | lattice_aosoa_pull_nt | lattice_pull_nt_x2
+//    | lattice_pull_nt_x4)
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -147,6 +150,25 @@ __global__ __launch_bounds__(256) void copy_lattice_aosoa(const float *__restric
   for (int q = 0; q < 26; q++) __builtin_nontemporal_store(v[q], &b[((size_t)(n >> 6) * 26 + q) * 64 + (n & 63)]);
 }
 
+// W consecutive cells per thread (N a multiple of W): W-wide loads at the pulled position (clamped), W-wide
+// non-temporal stores
+template <int W>
+__global__ __launch_bounds__(256) void copy_lattice_wide(const float *__restrict__ a, float *__restrict__ b, uint32_t N,
+                                                         int nx, int nxy) {
+  typedef float vw __attribute__((ext_vector_type(W)));
+  const uint32_t n = (blockIdx.x * 256 + threadIdx.x) * W;
+  if (n >= N) return;
+  vw v[26];
+#pragma unroll
+  for (int q = 0; q < 26; q++) {
+    int64_t src = (int64_t)n - (LCX[q] + LCY[q] * nx + LCZ[q] * nxy);
+    src = src < 0 ? 0 : (src > (int64_t)N - W ? (int64_t)N - W : src);
+    v[q] = *(const vw *)&a[(size_t)q * N + (size_t)src];
+  }
+#pragma unroll
+  for (int q = 0; q < 26; q++) __builtin_nontemporal_store(v[q], (vw *)&b[(size_t)q * N + n]);
+}
+
 #define CK(x)                                                                     \
   do {                                                                            \
     hipError_t e = (x);                                                           \
@@ -171,14 +193,14 @@ int main(int argc, char **argv) {
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
-  const char *names[12] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
+  const char *names[14] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
                            "lattice_aa_even", "lattice_aa_odd", "lattice_pull_nt", "lattice16_pull_nt",
-                           "lattice_aosoa_pull_nt"};
+                           "lattice_aosoa_pull_nt", "lattice_pull_nt_x2", "lattice_pull_nt_x4"};
   const int lnx = 600, lnxy = 600 * 600;
   const uint32_t lN = 600u * 600u * 64u;   // lattice variants: 26 * lN floats each way (2.4 GB)
   double best = 0;
   const char *best_name = "";
-  for (int v = 0; v < 12; v++) {
+  for (int v = 0; v < 14; v++) {
     if (strcmp(which, "all") && strcmp(which, names[v])) continue;
     if (v >= 5 && n * 16 < (size_t)26 * lN * 4) continue;   // the lattice needs 2.4 GB per buffer
     auto launch = [&]() {
@@ -195,6 +217,8 @@ int main(int argc, char **argv) {
         case 9: hipLaunchKernelGGL((copy_lattice<true, true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 10: hipLaunchKernelGGL(copy_lattice16, dim3((lN + 255) / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy); break;
         case 11: hipLaunchKernelGGL(copy_lattice_aosoa, dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
+        case 12: hipLaunchKernelGGL(copy_lattice_wide<2>, dim3((lN / 2 + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
+        case 13: hipLaunchKernelGGL(copy_lattice_wide<4>, dim3((lN / 4 + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
       }
     };
     launch();
