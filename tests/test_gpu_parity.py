@@ -139,3 +139,28 @@ def test_p30a_reference_run_on_gpu(pkg, O, ashburn30):
         fl = o.fluid_mask()
         assert rel_l2(g[:, fl], o.g[:, fl]) <= 1e-5
         assert rel_l2(f[:, fl], o.f[:, fl]) <= 1e-5
+
+
+# the reference's short validation runs whose settings are fully recorded (js/live/README.md:284; the p20 run is the
+# same command at dx = 20 with campuses A and B): campus rooftop sources, backflow-clipped outlets, default outflow
+# sponge, `pre` steps then `steps` steps from rest, then read(). Bars: the recorded WebGPU (f32) and numba (f64
+# arithmetic) surface maxima differ from each other by ~7e-6 relative after these 1,800-2,300 steps; the GPU must
+# agree with both to 2e-5, and with the surface mean and mean wind speed to 2e-5.
+@pytest.mark.parametrize('tag,flags', [('p30b', ['--dx', 30, '--nsw', 5, '--oclip', 1, '--campus', 'A']),
+                                       ('p20', ['--dx', 20, '--nsw', 8, '--oclip', 1, '--campus', 'AB'])])
+def test_reference_short_runs_on_gpu(tmp_path, pkg, tag, flags):
+    K = load_summary()['short'][tag]
+    dom = dump_domain(tmp_path, *flags)
+    assert dom['info']['nFluid'] == K['info']['nFluid'] and dom['dt'] == K['info']['dt']
+    with pkg.Solver.from_dump(dom) as s:
+        s.step(K['pre'] + K['steps'])
+        th, u, v = s.read_surface(dom['conv'])
+        st = s.stats()
+    wind = np.hypot(u.astype(np.float64), v.astype(np.float64)).mean()
+    print(tag, 'surface max %.7f mean %.8f wind %.7f' % (th.max(), th.astype(np.float64).mean(), wind),
+          'records', K['webgpu'], K['numba'])
+    assert st['non_finite'] == 0
+    assert abs(th.max() - K['webgpu']['thetaMax']) <= 2e-5 * K['webgpu']['thetaMax']
+    assert abs(th.max() - K['numba']['theta_max']) <= 2e-5 * K['numba']['theta_max']
+    assert abs(th.astype(np.float64).mean() - K['webgpu']['surfMean']) <= 2e-5 * K['webgpu']['surfMean']
+    assert abs(wind - K['webgpu']['windMean']) <= 2e-5 * K['webgpu']['windMean']

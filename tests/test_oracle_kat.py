@@ -8,6 +8,8 @@ These run on the CPU only. They pin the checker the GPU parity tests rely on:
   * the p30a run: 300 steps from a cold start on the Ashburn 30 m grid, read() surface max / mean / wind mean
     (the f32 oracle against the WebGPU record, the f64-arithmetic oracle against the numba record).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -76,3 +78,20 @@ def test_p30a_f64_oracle_matches_numba_record(ashburn30, O):
     o.step(K['steps'])
     th, _, _ = o.surface(ashburn30['conv'])
     assert abs(th.max() - K['numba']['theta_max']) < 1e-6 * K['numba']['theta_max']
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(not os.environ.get('AHT_SLOW'), reason='2,300 oracle steps on the 30 m grid: ~4 min on 8 cores '
+                                                          '(set AHT_SLOW=1); the GPU runs the same case every time')
+def test_p30b_f32_oracle_matches_webgpu_record(tmp_path, O):
+    """p30b: campus A, backflow-clipped outlets, outflow sponge 5, 2,000 + 300 steps (js/live/README.md:284)"""
+    K = load_summary()['short']['p30b']
+    dom = dump_domain(tmp_path, '--dx', 30, '--nsw', 5, '--oclip', 1, '--campus', 'A')
+    o = O.Oracle(dom, 'f32')
+    o.init_flow()
+    o.step(K['pre'] + K['steps'])
+    th, u, v = o.surface(dom['conv'])
+    wind = np.hypot(u.astype(np.float64), v.astype(np.float64)).mean()
+    assert abs(th.max() - K['webgpu']['thetaMax']) < 1e-5 * K['webgpu']['thetaMax']
+    assert abs(th.astype(np.float64).mean() - K['webgpu']['surfMean']) < 1e-5 * K['webgpu']['surfMean']
+    assert abs(wind - K['webgpu']['windMean']) < 1e-5 * K['webgpu']['windMean']
