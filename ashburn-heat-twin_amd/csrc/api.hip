@@ -16,6 +16,31 @@
 
 struct aht_batch;
 
+namespace {
+// Neighbour word of a column for the near-solid cells of the fast path: for each of the 8 lateral neighbours
+// (dx, dy) in {-1, 0, 1}^2 minus (0, 0), nibble n = (dy + 1) * 3 + (dx + 1) (minus one past the centre) holds
+// e = ktop(neighbour) - ktop(column) + 1, so that the source (i + dx, j + dy, k - c_z) of a cell at height
+// r = k - ktop above its column top is solid iff r - c_z + 1 < e (e <= 0 never is: stored as 0). Returns false when
+// some e exceeds 15 (a neighbour more than 14 levels taller): such a column's near-solid cells take the boundary
+// path. Neighbours are clamped at the faces as the boundary path clamps them.
+bool pack_neighbours(const uint32_t *ktop, int nx, int ny, int i, int j, uint32_t &word) {
+  const int kc = (int)ktop[(size_t)j * nx + i];
+  word = 0;
+  bool ok = true;
+  int slot = 0;
+  for (int dy = -1; dy <= 1; dy++)
+    for (int dx = -1; dx <= 1; dx++) {
+      if (dx == 0 && dy == 0) continue;
+      const int a = std::min(std::max(i + dx, 0), nx - 1), b = std::min(std::max(j + dy, 0), ny - 1);
+      const int e = std::max((int)ktop[(size_t)b * nx + a] - kc + 1, 0);
+      if (e > 15) ok = false;
+      word |= (uint32_t)std::min(e, 15) << (4 * slot);
+      slot++;
+    }
+  return ok;
+}
+}  // namespace
+
 // what the step kernel reads through pointers: the grid and the stream tables of both lattice parities
 struct DevGrid {
   aht::Grid g;
@@ -33,12 +58,13 @@ struct aht_solver {
   int storage = AHT_STORE_F32;
   int eb = 4;                 // bytes per stored population element
   size_t gbytes = 0;          // bytes of one D3Q7 lattice (theta-referenced formats add the f32 theta plane)
-  void *f[2] = {nullptr, nullptr}, *gq[2] = {nullptr, nullptr};
+  void *f[2] = {nullptr, nullptr}, *gq[2] = {nullptr, nullptr};   // parts of `lattices`
+  void *lattices = nullptr;
   void *stage = nullptr;      // one population plane in f64, for state conversion
   DevGrid *gdev = nullptr;    // device copy of g and of its stream tables, read by the step kernel
   DevGrid *gpin = nullptr;    // its pinned staging copy
   bool gdirty = true;         // g changed since the last upload
-  uint32_t *colk = nullptr;
+  uint32_t *colk = nullptr, *colnb = nullptr;
   float2 *colcq = nullptr;
   uint32_t *ktop = nullptr, *oldk = nullptr;
   float4 *prof = nullptr;
@@ -174,13 +200,19 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   g.inv_nx = 1.0f / (float)d.nx;
   g.win = 1;
   g.th_off = (uint32_t)aht::theta_offset(d.storage, (size_t)N);
-  for (int p = 0; p < 2; p++) {
-    HIP_TRY(s, hipMalloc(&s->f[p], 19 * N * s->eb));
-    HIP_TRY(s, hipMalloc(&s->gq[p], s->gbytes));
-    HIP_TRY(s, hipMemsetAsync(s->f[p], 0, 19 * N * s->eb, s->st));
-    HIP_TRY(s, hipMemsetAsync(s->gq[p], 0, s->gbytes, s->st));
+  {
+    // both lattice pairs in one allocation (2 MiB-aligned parts): one large mapping keeps the 52+ streams of a step
+    // inside few translation entries (the f16 step's L1 TLB misses went from 4 M per launch to none at 600x600x64)
+    const size_t A = (size_t)2 << 20;
+    const size_t fb = (19 * N * s->eb + A - 1) / A * A, gb = (s->gbytes + A - 1) / A * A;
+    char *base = nullptr;
+    HIP_TRY(s, hipMalloc((void **)&base, 2 * (fb + gb)));
+    s->lattices = base;
+    s->f[0] = base; s->gq[0] = base + fb; s->f[1] = base + fb + gb; s->gq[1] = base + 2 * fb + gb;
+    HIP_TRY(s, hipMemsetAsync(base, 0, 2 * (fb + gb), s->st));
   }
   HIP_TRY(s, hipMalloc(&s->colk, NXY * sizeof(uint32_t)));
+  HIP_TRY(s, hipMalloc(&s->colnb, NXY * sizeof(uint32_t)));
   HIP_TRY(s, hipMalloc(&s->colcq, NXY * sizeof(float2)));
   HIP_TRY(s, hipMalloc(&s->ktop, NXY * sizeof(uint32_t)));
   HIP_TRY(s, hipMalloc(&s->oldk, NXY * sizeof(uint32_t)));
@@ -194,7 +226,7 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   HIP_TRY(s, hipMalloc(&s->surf3, 3 * (size_t)NXY * sizeof(float)));
   HIP_TRY(s, hipMalloc(&s->vol, ((N + 3) / 4) * sizeof(uint32_t)));
   HIP_TRY(s, hipMalloc(&s->tmax, sizeof(uint32_t)));
-  s->g.colk = s->colk; s->g.colcq = s->colcq; s->g.ktop = s->ktop; s->g.prof = s->prof;
+  s->g.colk = s->colk; s->g.colnb = s->colnb; s->g.colcq = s->colcq; s->g.ktop = s->ktop; s->g.prof = s->prof;
   HIP_TRY(s, hipMalloc(&s->gdev, sizeof(DevGrid)));
   HIP_TRY(s, hipHostMalloc((void **)&s->gpin, sizeof(DevGrid), hipHostMallocDefault));
   rc = aht_mean_reset(s);
@@ -210,7 +242,7 @@ void aht_destroy(aht_solver *s) {
     (void)hipSetDevice(s->device);
     (void)hipStreamSynchronize(s->st);
   }
-  void *bufs[] = {s->f[0], s->f[1], s->gq[0], s->gq[1], s->stage, s->gdev, s->colk, s->colcq, s->ktop, s->oldk, s->prof, s->acc_surf, s->acc_face,
+  void *bufs[] = {s->lattices, s->stage, s->gdev, s->colk, s->colnb, s->colcq, s->ktop, s->oldk, s->prof, s->acc_surf, s->acc_face,
                   s->acc_content, s->red_out, s->stat_part, s->out4, s->surf3, s->vol, s->tmax};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -228,7 +260,7 @@ int aht_set_columns(aht_solver *s, const uint32_t *kt, const uint32_t *ktop, con
   const uint32_t NXY = s->g.NXY;
   const int nz = s->g.nz;
   const int nx = s->g.nx, ny = s->g.ny;
-  std::vector<uint32_t> ckv(NXY);
+  std::vector<uint32_t> ckv(NXY), nbv(NXY);
   std::vector<float2> cqv(NXY);
   int kmin = nz;
   for (uint32_t c = 0; c < NXY; c++) {
@@ -245,13 +277,15 @@ int aht_set_columns(aht_solver *s, const uint32_t *kt, const uint32_t *ktop, con
       for (int b = std::max(j - 1, 0); b <= std::min(j + 1, ny - 1); b++)
         for (int a = std::max(i - 1, 0); a <= std::min(i + 1, nx - 1); a++) m = std::max(m, ktop[(size_t)b * nx + a]);
       const uint32_t kfast = std::min<uint32_t>(m + 1, 1023u);
-      ckv[c] = kt[c] | (ktop[c] << 10) | (kfast << 20);
+      const bool near = pack_neighbours(ktop, nx, ny, i, j, nbv[c]);
+      ckv[c] = kt[c] | (ktop[c] << 10) | (kfast << 20) | ((uint32_t)near << 30);
       cqv[c] = make_float2(cd[c], q[c]);
     }
   }
   int rc = set_device(s);
   if (rc) return rc;
   HIP_TRY(s, hipMemcpyAsync(s->colk, ckv.data(), NXY * sizeof(uint32_t), hipMemcpyHostToDevice, s->st));
+  HIP_TRY(s, hipMemcpyAsync(s->colnb, nbv.data(), NXY * sizeof(uint32_t), hipMemcpyHostToDevice, s->st));
   HIP_TRY(s, hipMemcpyAsync(s->colcq, cqv.data(), NXY * sizeof(float2), hipMemcpyHostToDevice, s->st));
   HIP_TRY(s, hipMemcpyAsync(s->ktop, ktop, NXY * sizeof(uint32_t), hipMemcpyHostToDevice, s->st));
   HIP_TRY(s, hipStreamSynchronize(s->st));

@@ -17,7 +17,9 @@ struct Grid {
   uint32_t win;        // tile order: 1 = linear, W > 1 = windows of W consecutive tiles per XCD, 0 = one slab per XCD
   uint32_t th_off;     // theta-referenced D3Q7 lattices: byte offset of the f32 theta plane (0 otherwise)
   uint32_t pad;
-  const uint32_t *colk; // per column: kt | ktop << 10 | kfast << 20 (read by every cell), solver orientation
+  const uint32_t *colk; // per column: kt | ktop << 10 | kfast << 20 | near << 30 (read by every cell), solver orientation
+  const uint32_t *colnb; // per column: ktop of the 8 lateral neighbours, 4 bits each (see pack_neighbours in api.hip),
+                         // read by near-solid cells only
   const float2 *colcq;  // per column: (cd, q), read only at the first fluid cell
   const uint32_t *ktop;
   const float4 *prof;  // [nz] (u_in, v_in, kb, 0)

@@ -7,12 +7,15 @@
 // the lowest fluid level of the grid, so all-solid planes are never launched. Solid cells above that plane exit.
 //
 // Two streaming paths feed one collision:
-//   * fast path: interior cells (not on a lateral face, below the lid) at or above the column's kfast level, the first
-//     level where every source cell in the 3 x 3 x 3 neighbourhood is fluid (host-computed). All 26 pulls are plain
-//     loads through buffer descriptors: one per population with the pull offset folded into the scalar base, so the
-//     only per-lane address is the cell's byte offset. Drag and the rooftop source cannot apply there (k > ktop).
+//   * fast path: interior cells (not on a lateral face, below the lid). All 26 pulls are plain loads through buffer
+//     descriptors: one per population with the pull offset folded into the scalar base, so the only per-lane address
+//     is the cell's byte offset. At or above the column's kfast level (the first level where every source cell in the
+//     3 x 3 x 3 neighbourhood is fluid, host-computed) that is all; below it (near-solid cells) the populations whose
+//     source is solid are then replaced by the bounce-back value, from a per-column word with the neighbours' ktop.
+//     A wavefront mixing cells next to terrain or roofs with free cells thus runs one pull sequence, not two.
 //   * boundary path: every branch of js/live/kernel.js:159-243 per population (lid first, then lateral faces, then the
-//     interior pull with half-way bounce-back from solid sources), with the 3 x 3 neighbour ktop values loaded once.
+//     interior pull with half-way bounce-back from solid sources), with the 3 x 3 neighbour ktop values loaded once;
+//     for face and lid cells, and near-solid cells of columns whose neighbour word cannot hold the heights.
 #include <algorithm>
 
 #include "kernels.h"
@@ -80,10 +83,12 @@ __device__ __forceinline__ uint32_t div_by(uint32_t t, uint32_t d, float inv_d, 
   return q;
 }
 
-// packed column word: kt | ktop << 10 | kfast << 20 (nz < 1023)
+// packed column word: kt | ktop << 10 | kfast << 20 | near << 30 (nz < 1023); near: the column's neighbour word
+// (Grid::colnb) is exact, so its near-solid interior cells may take the fast path with the bounce-back fix-up
 __device__ __forceinline__ int col_kt(uint32_t x) { return (int)(x & 1023u); }
 __device__ __forceinline__ int col_ktop(uint32_t x) { return (int)((x >> 10) & 1023u); }
-__device__ __forceinline__ int col_kfast(uint32_t x) { return (int)(x >> 20); }
+__device__ __forceinline__ int col_kfast(uint32_t x) { return (int)((x >> 20) & 1023u); }
+__device__ __forceinline__ bool col_near(uint32_t x) { return (x >> 30) & 1u; }
 
 // element access of population qq at cell index idx, decoded with weight w19<q> (deviation formats)
 template <class S, int q>
@@ -313,6 +318,45 @@ struct FastPull<19> {
                                              const Streams *, const void *, const void *, uint32_t) {}
 };
 
+// Near-solid interior cells (ktop <= k < kfast, not on a face or the lid) share the fast path's pulls, which for a
+// solid source cell read a value that is then replaced: population q whose source (i - cx, j - cy, k - cz) is solid
+// takes the own cell's opp(q) instead (half-way bounce-back, js/live/kernel.js:209-210 and :240-241), exactly the
+// value the boundary path loads. r = k - ktop; nb = the column's neighbour word (ktop of the neighbour minus ktop of
+// the column plus one, per nibble). Only lanes that need a replacement load it.
+template <int cx, int cy, int cz>
+__device__ __forceinline__ bool source_solid(int r, uint32_t nb) {
+  if constexpr (cx == 0 && cy == 0) {
+    return r - cz < 0;
+  } else {
+    constexpr int dx = -cx, dy = -cy, n9 = (dy + 1) * 3 + (dx + 1), slot = n9 < 4 ? n9 : n9 - 1;
+    return r - cz + 1 < (int)((nb >> (4 * slot)) & 15u);
+  }
+}
+
+template <int q>
+struct BounceFix {
+  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
+                                             const void *fA, const void *gA, uint32_t n, int r, uint32_t nb) {
+    const uint32_t boff = n * (uint32_t)S::bytes;
+    if (source_solid<CX[q], CY[q], CZ[q]>(r, nb)) f[q] = S::dec_f(S::ld_raw(plane<S>((void *)fA, g.N, OPP[q]), boff), w19<q>());
+    if constexpr (q < 7) {
+      if (source_solid<CX7[q], CY7[q], CZ7[q]>(r, nb)) {
+        const typename S::elem e = S::ld_raw(plane<S>((void *)gA, g.N, OPP7[q]), boff);
+        if constexpr (S::theta_ref) gg[q] = S::dec_g(e, w7q(q), ld_theta(gA, g.th_off + n * 4u));
+        else gg[q] = S::dec_g(e, 0.0f, typename S::real(0));
+      }
+    }
+    BounceFix<q + 1>::template run<S>(f, gg, g, fA, gA, n, r, nb);
+  }
+};
+template <>
+struct BounceFix<19> {
+  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&)[19], typename S::real (&)[7], const Grid &,
+                                             const void *, const void *, uint32_t, int, uint32_t) {}
+};
+
 template <int q>
 struct StorePairs {
   template <class S, bool TAB>
@@ -346,11 +390,13 @@ __device__ __forceinline__ void load_neighbours(const Grid &g, int i, int j, int
 // streaming of one fluid cell: the fast or the boundary path
 template <class S, bool TAB>
 __device__ __forceinline__ void stream_cell(const Grid &g, const Streams *sp, const void *fA, const void *gA,
-                                            uint32_t n, int i, int j, int k, int kg, bool fast, typename S::real uI,
-                                            typename S::real vI, typename S::real (&f)[19], typename S::real (&gg)[7]) {
+                                            uint32_t n, int c, int i, int j, int k, int kg, int kfast, bool fast,
+                                            typename S::real uI, typename S::real vI, typename S::real (&f)[19],
+                                            typename S::real (&gg)[7]) {
   using E = typename S::elem;
   if (fast) {
     FastPull<0>::template run<S, TAB>(f, gg, g, sp, fA, gA, n);
+    if (k < kfast) BounceFix<1>::template run<S>(f, gg, g, fA, gA, n, k - kg, g.colnb[c]);
   } else {
     int kn[3][3];
     load_neighbours(g, i, j, kg, kn);
@@ -376,10 +422,11 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *s
     int i, j;
     split_column(g, c, i, j);
     const float4 pr = g.prof[max(k - ktc, 0)];
-    const bool fast = k >= col_kfast(ck) && k < g.nz - 1 && i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
+    const int kfast = col_kfast(ck);
+    const bool fast = (k >= kfast || col_near(ck)) && k < g.nz - 1 && i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
     const float2 cq = (k == kg) ? g.colcq[c] : make_float2(0.0f, 0.0f);   // drag and source act at k == ktop only
     R f[19], gg[7];
-    stream_cell<S, TAB>(g, sp, fA, gA, n, i, j, k, kg, fast, R(pr.x), R(pr.y), f, gg);
+    stream_cell<S, TAB>(g, sp, fA, gA, n, c, i, j, k, kg, kfast, fast, R(pr.x), R(pr.y), f, gg);
     const Moments<R> m = moments(f, gg);
     const Forces<R> F = forces<true, true>(g.P, g.rp, g.nz, k, kg, ktc, R(cq.x), R(pr.x), R(pr.y), m);
     const Relax<R> Rx = relax(g.P, g.rp, g.nx, g.ny, i, j, R(pr.z), m, F);
@@ -527,7 +574,7 @@ __global__ __launch_bounds__(256) void surface_kernel(Grid g, const void *fA, co
   const uint32_t n = (uint32_t)k * g.NXY + (uint32_t)c;
   const float4 pr = g.prof[max(k - ktc, 0)];
   R f[19], gg[7];
-  stream_cell<S, false>(g, nullptr, fA, gA, n, i, j, k, kg, false, R(pr.x), R(pr.y), f, gg);
+  stream_cell<S, false>(g, nullptr, fA, gA, n, c, i, j, k, kg, 0, false, R(pr.x), R(pr.y), f, gg);
   const Moments<R> m = moments(f, gg);
   const Forces<R> F = forces<true, true>(g.P, g.rp, g.nz, k, kg, ktc, R(g.colcq[c].x), R(pr.x), R(pr.y), m);
   out3[o] = (float)m.th;

@@ -118,8 +118,9 @@ def main():
                     help='also measure the serving loop at this arrival rate (req/s); 0 = report the committed record')
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f'bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); the multi-GPU design is '
-              f'described in DESIGN.md and not run', file=sys.stderr)
+        print(f'bench.py: one GPU only (asked for --gpus {a.gpus}): scenarios are independent per GPU and a single '
+              f'domain would be cut into y-slabs with a one-plane halo exchange per step, which this build does not '
+              f'run', file=sys.stderr)
         sys.exit(2)
     m = load_pkg()
     tmp = None
