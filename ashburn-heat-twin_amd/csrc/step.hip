@@ -184,15 +184,37 @@ __device__ __forceinline__ Outflow<typename S::real> outflow_prelude(const Grid 
   return o;
 }
 
+// population q pulled from outside the lateral faces (js/live/kernel.js:186-205): inflow equilibrium with rho = 1, or
+// the pressure outlet's non-equilibrium extrapolation towards rho = 1 (optionally backflow-clipped). Shared by the
+// boundary path and the fast path's face fix-up, so both give the same bits.
+template <class S, int q>
+__device__ __forceinline__ typename S::real lateral19(const Grid &g, const typename S::elem *__restrict__ fA, size_t N,
+                                                      uint32_t n, int i, int j, typename S::real uI,
+                                                      typename S::real vI, const Outflow<typename S::real> &ob) {
+  using R = typename S::real;
+  constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
+  const Params &P = g.P;
+  const int NX = g.nx, NY = g.ny;
+  const bool inflow = (cx == 1 && i == 0 && P.inflow_w != 0) || (cx == -1 && i == NX - 1 && P.inflow_e != 0) ||
+                      (cy == 1 && j == 0 && P.inflow_s != 0) || (cy == -1 && j == NY - 1 && P.inflow_n != 0);
+  const R w = R(w19<q>());
+  if (inflow) return w * epoly(cdot<cx, cy, 0>(uI, vI, R(0)), uI * uI + vI * vI);   // equilibrium, rho = 1
+  const R cu = cdot<cx, cy, cz>(ob.uxn, ob.uyn, ob.uzn);
+  const R own = ldf<S, q>(fA, N, q, n);
+  if (P.oclip != 0) {
+    const R cuc = cdot<cx, cy, cz>(ob.uxc, ob.uyc, ob.uzn);
+    return own + w * (epoly(cuc, ob.usqc) - ob.rn * epoly(cu, ob.usqn));
+  }
+  return own + w * (R(1) - ob.rn) * epoly(cu, ob.usqn);
+}
+
 // post-stream population q of cell (i, j, k), every boundary branch (js/live/kernel.js:159-217)
 template <class S, int q>
 __device__ __forceinline__ typename S::real pull19(const Grid &g, const typename S::elem *__restrict__ fA, size_t N,
                                                    uint32_t n, int i, int j, int k, const int (&kn)[3][3],
                                                    typename S::real uI, typename S::real vI,
                                                    const Outflow<typename S::real> &ob) {
-  using R = typename S::real;
   constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
-  const Params &P = g.P;
   const int NX = g.nx, NY = g.ny;
   if (cz == -1 && k == g.nz - 1) {
     // free-slip lid: the mirrored population of the lateral neighbour at the same level
@@ -204,20 +226,7 @@ __device__ __forceinline__ typename S::real pull19(const Grid &g, const typename
   }
   const bool outx = (cx == 1 && i == 0) || (cx == -1 && i == NX - 1);
   const bool outy = (cy == 1 && j == 0) || (cy == -1 && j == NY - 1);
-  if (outx || outy) {
-    const bool inflow = (cx == 1 && i == 0 && P.inflow_w != 0) || (cx == -1 && i == NX - 1 && P.inflow_e != 0) ||
-                        (cy == 1 && j == 0 && P.inflow_s != 0) || (cy == -1 && j == NY - 1 && P.inflow_n != 0);
-    const R w = R(w19<q>());
-    if (inflow) return w * epoly(cdot<cx, cy, 0>(uI, vI, R(0)), uI * uI + vI * vI);   // equilibrium, rho = 1
-    // pressure outlet: non-equilibrium extrapolation towards rho = 1, optionally backflow-clipped
-    const R cu = cdot<cx, cy, cz>(ob.uxn, ob.uyn, ob.uzn);
-    const R own = ldf<S, q>(fA, N, q, n);
-    if (P.oclip != 0) {
-      const R cuc = cdot<cx, cy, cz>(ob.uxc, ob.uyc, ob.uzn);
-      return own + w * (epoly(cuc, ob.usqc) - ob.rn * epoly(cu, ob.usqn));
-    }
-    return own + w * (R(1) - ob.rn) * epoly(cu, ob.usqn);
-  }
+  if (outx || outy) return lateral19<S, q>(g, fA, N, n, i, j, uI, vI, ob);
   // interior: half-way bounce-back when the source cell is solid
   const int sk = k - cz;
   if (sk < kn[1 - cx][1 - cy]) return ldf<S, q>(fA, N, OPP[q], n);
@@ -225,21 +234,27 @@ __device__ __forceinline__ typename S::real pull19(const Grid &g, const typename
   return ldf<S, q>(fA, N, q, (uint32_t)((int)n - off));
 }
 
+// theta population q pulled from outside the lateral faces (js/live/kernel.js:231-236): 0 at an inflow face, the own
+// cell's value at an outlet
+template <class S, int q>
+__device__ __forceinline__ typename S::real lateral7(const Grid &g, const void *gA, uint32_t n, int i, int j) {
+  using R = typename S::real;
+  constexpr int cx = CX7[q], cy = CY7[q];
+  const Params &P = g.P;
+  const bool inflow = (cx == 1 && i == 0 && P.inflow_w != 0) || (cx == -1 && i == g.nx - 1 && P.inflow_e != 0) ||
+                      (cy == 1 && j == 0 && P.inflow_s != 0) || (cy == -1 && j == g.ny - 1 && P.inflow_n != 0);
+  return inflow ? R(0) : ldg<S>(g, gA, q, n);
+}
+
 // post-stream theta population q (js/live/kernel.js:219-243)
 template <class S, int q>
 __device__ __forceinline__ typename S::real pull7(const Grid &g, const void *gA, uint32_t n, int i, int j, int k,
                                                   const int (&kn)[3][3]) {
-  using R = typename S::real;
   constexpr int cx = CX7[q], cy = CY7[q], cz = CZ7[q];
-  const Params &P = g.P;
   if (cz == -1 && k == g.nz - 1) return ldg<S>(g, gA, q, n);   // zero-gradient top
   const bool outx = (cx == 1 && i == 0) || (cx == -1 && i == g.nx - 1);
   const bool outy = (cy == 1 && j == 0) || (cy == -1 && j == g.ny - 1);
-  if (outx || outy) {
-    const bool inflow = (cx == 1 && i == 0 && P.inflow_w != 0) || (cx == -1 && i == g.nx - 1 && P.inflow_e != 0) ||
-                        (cy == 1 && j == 0 && P.inflow_s != 0) || (cy == -1 && j == g.ny - 1 && P.inflow_n != 0);
-    return inflow ? R(0) : ldg<S>(g, gA, q, n);
-  }
+  if (outx || outy) return lateral7<S, q>(g, gA, n, i, j);
   const int sk = k - cz;
   if (sk < kn[1 - cx][1 - cy]) return ldg<S>(g, gA, OPP7[q], n);
   const int off = cx + cy * g.nx + cz * (int)g.NXY;
@@ -357,6 +372,38 @@ struct BounceFix<19> {
                                              const void *, const void *, uint32_t, int, uint32_t) {}
 };
 
+// Face cells (below the lid) also share the fast path's pulls: populations whose source lies outside the lateral
+// faces are then replaced by the inflow / outlet value of the boundary path (lateral19, lateral7). The pulled values
+// they replace come from in-bounds cells of the wrapped row or the neighbouring plane (k >= 1) and are discarded.
+template <int q>
+struct FaceFix {
+  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
+                                             const void *fA, const void *gA, uint32_t n, int i, int j,
+                                             typename S::real uI, typename S::real vI,
+                                             const Outflow<typename S::real> &ob) {
+    constexpr int cx = CX[q], cy = CY[q];
+    if constexpr (cx != 0 || cy != 0) {
+      const bool out = (cx == 1 && i == 0) || (cx == -1 && i == g.nx - 1) || (cy == 1 && j == 0) ||
+                       (cy == -1 && j == g.ny - 1);
+      if (out) f[q] = lateral19<S, q>(g, (const typename S::elem *)fA, g.N, n, i, j, uI, vI, ob);
+    }
+    if constexpr (q < 7 && (CX7[q] != 0 || CY7[q] != 0)) {
+      const bool out = (CX7[q] == 1 && i == 0) || (CX7[q] == -1 && i == g.nx - 1) || (CY7[q] == 1 && j == 0) ||
+                       (CY7[q] == -1 && j == g.ny - 1);
+      if (out) gg[q] = lateral7<S, q>(g, gA, n, i, j);
+    }
+    FaceFix<q + 1>::template run<S>(f, gg, g, fA, gA, n, i, j, uI, vI, ob);
+  }
+};
+template <>
+struct FaceFix<19> {
+  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&)[19], typename S::real (&)[7], const Grid &,
+                                             const void *, const void *, uint32_t, int, int, typename S::real,
+                                             typename S::real, const Outflow<typename S::real> &) {}
+};
+
 template <int q>
 struct StorePairs {
   template <class S, bool TAB>
@@ -397,6 +444,10 @@ __device__ __forceinline__ void stream_cell(const Grid &g, const Streams *sp, co
   if (fast) {
     FastPull<0>::template run<S, TAB>(f, gg, g, sp, fA, gA, n);
     if (k < kfast) BounceFix<1>::template run<S>(f, gg, g, fA, gA, n, k - kg, g.colnb[c]);
+    if (i == 0 || i == g.nx - 1 || j == 0 || j == g.ny - 1) {
+      const Outflow<typename S::real> ob = outflow_prelude<S>(g, (const E *)fA, g.N, n, i, j);
+      FaceFix<1>::template run<S>(f, gg, g, fA, gA, n, i, j, uI, vI, ob);
+    }
   } else {
     int kn[3][3];
     load_neighbours(g, i, j, kg, kn);
@@ -423,7 +474,7 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *s
     split_column(g, c, i, j);
     const float4 pr = g.prof[max(k - ktc, 0)];
     const int kfast = col_kfast(ck);
-    const bool fast = (k >= kfast || col_near(ck)) && k < g.nz - 1 && i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
+    const bool fast = (k >= kfast || col_near(ck)) && k < g.nz - 1;
     const float2 cq = (k == kg) ? g.colcq[c] : make_float2(0.0f, 0.0f);   // drag and source act at k == ktop only
     R f[19], gg[7];
     stream_cell<S, TAB>(g, sp, fA, gA, n, c, i, j, k, kg, kfast, fast, R(pr.x), R(pr.y), f, gg);
