@@ -884,21 +884,41 @@ int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best) {
   if (eg != hipSuccess) { (void)hipFree(bf); return fail(s, AHT_ENOMEM, "autotune backup: %s", hipGetErrorString(eg)); }
   HIP_TRY(s, hipMemcpyAsync(bf, s->f[p0], fb, hipMemcpyDeviceToDevice, s->st));
   HIP_TRY(s, hipMemcpyAsync(bg, s->gq[p0], gb, hipMemcpyDeviceToDevice, s->st));
+  // every shape over `steps` timed steps (after one untimed step that also uploads the changed grid), then the four
+  // fastest again twice over 3 x `steps`, best of the two: neighbouring shapes differ by ~1 %, about the noise of a
+  // single short timing
   const int cb[3] = {64, 128, 256}, cl[3] = {1, 2, 4}, cw[5] = {1, 2, 4, 16, 0};
-  double best_ms = 1e30;
+  struct Cand {
+    aht_launch l;
+    double ms;
+  };
+  std::vector<Cand> cand;
   aht_launch keep{s->block, s->levels, (int32_t)s->g.win, 0};
-  for (int b : cb) {
-    for (int l : cl) {
+  auto time_shape = [&](const aht_launch &l, int32_t n, double &ms) {
+    apply_launch(s, l);
+    int r = aht_step(s, 1, 0);
+    if (!r) r = aht_benchmark(s, n, &ms);
+    return r;
+  };
+  for (int b : cb)
+    for (int l : cl)
       for (int w : cw) {
-        apply_launch(s, aht_launch{b, l, w, 0});
-        double ms = 0;
-        rc = aht_benchmark(s, steps, &ms);
         if (rc) break;
-        if (ms < best_ms) { best_ms = ms; keep = aht_launch{b, l, w, 0}; }
+        double ms = 0;
+        rc = time_shape(aht_launch{b, l, w, 0}, steps, ms);
+        cand.push_back(Cand{aht_launch{b, l, w, 0}, ms / steps});
       }
-      if (rc) break;
-    }
-    if (rc) break;
+  if (!rc) {
+    std::sort(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+    cand.resize(std::min<size_t>(cand.size(), 4));
+    for (Cand &c : cand) c.ms = 1e30;
+    for (int round = 0; round < 2 && !rc; round++)
+      for (Cand &c : cand) {
+        double ms = 0;
+        if ((rc = time_shape(c.l, 3 * steps, ms))) break;
+        c.ms = std::min(c.ms, ms / (3 * steps));
+      }
+    if (!rc) keep = std::min_element(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; })->l;
   }
   apply_launch(s, keep);
   // restore the state and the counters: tuning leaves the simulation where it was
