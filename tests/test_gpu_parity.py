@@ -66,6 +66,31 @@ def test_tiny_domain_steps_match_oracle(tmp_path, pkg, O, wdir, oclip, nsw):
         assert rel_l2(np.stack([u_g, v_g]), np.stack([u_o, v_o])) <= 1e-5
 
 
+def test_tall_towers_match_oracle(tmp_path, pkg, O):
+    """columns next to towers more than 14 levels taller than themselves (one inland, one on the west face): their
+    near-solid cells take the boundary path (the per-column neighbour word holds heights up to 14 levels), the towers'
+    own columns and the rest the fast path with the bounce-back / face fix-ups"""
+    dom = tiny(tmp_path, 250, 1, 5, shape='24,20,24,7')
+    nx, ny, nz = dom['nx'], dom['ny'], dom['nz']
+    kt, ktop = dom['kt'].reshape(ny, nx), dom['ktop'].reshape(ny, nx).copy()
+    ktop[8:10, 10:13] = np.minimum(kt[8:10, 10:13] + 18, nz - 2)
+    ktop[3:6, 0:2] = np.minimum(kt[3:6, 0:2] + 17, nz - 2)
+    assert (ktop[8:10, 10:13] - ktop[7, 10:13].max()).min() > 14
+    dom['ktop'] = ktop.reshape(-1).astype(dom['ktop'].dtype)
+    o, s = pair(pkg, O, dom)
+    with s:
+        ro = o.step(1, out=True)
+        s.step(1, out=True)
+        go = s.read_out()
+        fl = o.fluid_mask()
+        for c in range(3):
+            assert rel_l2(go[c, fl], ro[c, fl]) <= 1e-6
+        assert_state_close(s, o, 1e-6)
+        o.step(49)
+        s.step(49)
+        assert_state_close(s, o, 1e-5)
+
+
 def test_single_step_theta_from_developed_state(tmp_path, pkg, O):
     """north_star bar: one step from an identical developed state, relative L2 of theta <= 1e-5"""
     dom = tiny(tmp_path, 210, 1, 5)
