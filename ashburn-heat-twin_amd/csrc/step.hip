@@ -333,11 +333,13 @@ struct FastPull<19> {
                                              const Streams *, const void *, const void *, uint32_t) {}
 };
 
-// Near-solid interior cells (ktop <= k < kfast, not on a face or the lid) share the fast path's pulls, which for a
-// solid source cell read a value that is then replaced: population q whose source (i - cx, j - cy, k - cz) is solid
-// takes the own cell's opp(q) instead (half-way bounce-back, js/live/kernel.js:209-210 and :240-241), exactly the
-// value the boundary path loads. r = k - ktop; nb = the column's neighbour word (ktop of the neighbour minus ktop of
-// the column plus one, per nibble). Only lanes that need a replacement load it.
+// Near-solid cells below the lid (ktop <= k < kfast) share the fast path's pulls, which for a solid source cell read
+// a value that is then replaced: population q whose source (i - cx, j - cy, k - cz) is solid takes the own cell's
+// opp(q) instead (half-way bounce-back, js/live/kernel.js:209-210 and :240-241), exactly the value the boundary path
+// loads. r = k - ktop; nb = the column's neighbour word (ktop of the neighbour minus ktop of the column plus one, per
+// nibble; clamped at the faces like the boundary path's neighbours). Only lanes that need a replacement load it. On
+// a face, populations pulled from outside the domain are replaced again afterwards (FaceFix), as the boundary path
+// tests the faces before solidity.
 template <int cx, int cy, int cz>
 __device__ __forceinline__ bool source_solid(int r, uint32_t nb) {
   if constexpr (cx == 0 && cy == 0) {
@@ -354,7 +356,8 @@ struct BounceFix {
   __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
                                              const void *fA, const void *gA, uint32_t n, int r, uint32_t nb) {
     const uint32_t boff = n * (uint32_t)S::bytes;
-    if (source_solid<CX[q], CY[q], CZ[q]>(r, nb)) f[q] = S::dec_f(S::ld_raw(plane<S>((void *)fA, g.N, OPP[q]), boff), w19<q>());
+    if (source_solid<CX[q], CY[q], CZ[q]>(r, nb))
+      f[q] = S::dec_f(S::ld_raw(plane<S>((void *)fA, g.N, OPP[q]), boff), w19<q>());
     if constexpr (q < 7) {
       if (source_solid<CX7[q], CY7[q], CZ7[q]>(r, nb)) {
         const typename S::elem e = S::ld_raw(plane<S>((void *)gA, g.N, OPP7[q]), boff);
