@@ -57,6 +57,7 @@ struct aht_solver {
   int levels = 1;             // cells per thread along z
   int storage = AHT_STORE_F32;
   int eb = 4;                 // bytes per stored population element
+  size_t fbytes = 0;          // bytes of one D3Q19 lattice (whole blocks in the blocked layout)
   size_t gbytes = 0;          // bytes of one D3Q7 lattice (theta-referenced formats add the f32 theta plane)
   void *f[2] = {nullptr, nullptr}, *gq[2] = {nullptr, nullptr};   // parts of `lattices`
   void *lattices = nullptr;
@@ -153,7 +154,10 @@ int64_t aht_footprint_bytes(const aht_dims *d) {
   if (!d || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return -1;
   const int64_t N = (int64_t)d->nx * d->ny * d->nz, NXY = (int64_t)d->nx * d->ny;
   const int eb = d->storage == AHT_STORE_F32 ? 4 : (d->storage == AHT_STORE_F64 ? 8 : 2);
-  const int64_t pop = 2 * (19 * N * eb + (int64_t)aht::g_lattice_bytes(d->storage, (size_t)N));
+  const int lb = aht::population_layout(d->storage, (size_t)N);
+  const int64_t pop = 2 * (int64_t)(aht::f_lattice_bytes(d->storage, lb, (size_t)N) +
+                                    aht::g_lattice_bytes(d->storage, lb, (size_t)N));
+  (void)eb;
   const int64_t cols = NXY * (4 + 8 + 4 + 4) + (int64_t)d->nz * 16;
   const int64_t acc = 3 * NXY * 8 + (int64_t)face_total(d->nx, d->ny, d->nz) * 8 + (int64_t)max_content_slots(*d) * 8;
   const int64_t misc = ((N + 255) / 256) * 64 + 3 * NXY * 4 + (N + 3) / 4 * 4 / 1 + 4096;
@@ -169,7 +173,7 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   if (d.storage < AHT_STORE_F32 || d.storage > AHT_STORE_F64) return AHT_EINVAL;
   // byte offsets within one population plane and within a D3Q7 lattice (theta plane) stay 32-bit
   if (N * (uint64_t)aht::storage_bytes(d.storage) >= (1ull << 32)) return AHT_EINVAL;
-  if (aht::storage_theta_ref(d.storage) && (uint64_t)aht::g_lattice_bytes(d.storage, (size_t)N) >= (1ull << 32))
+  if (aht::storage_theta_ref(d.storage) && (uint64_t)aht::g_lattice_bytes(d.storage, 0, (size_t)N) >= (1ull << 32))
     return AHT_EINVAL;
   const int block = d.block ? d.block : aht::STEP_BLOCK;
   if (block < 64 || block > 256 || block % 64) return AHT_EINVAL;
@@ -177,7 +181,9 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   s->dims = d;
   s->storage = d.storage;
   s->eb = aht::storage_bytes(d.storage);
-  s->gbytes = aht::g_lattice_bytes(d.storage, (size_t)N);
+  s->g.lb = aht::population_layout(d.storage, (size_t)N);
+  s->fbytes = aht::f_lattice_bytes(d.storage, s->g.lb, (size_t)N);
+  s->gbytes = aht::g_lattice_bytes(d.storage, s->g.lb, (size_t)N);
   s->device = d.device;
   s->block = block;
   int ndev = 0;
@@ -204,7 +210,7 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
     // both lattice pairs in one allocation (2 MiB-aligned parts): one large mapping keeps the 52+ streams of a step
     // inside few translation entries (the f16 step's L1 TLB misses went from 4 M per launch to none at 600x600x64)
     const size_t A = (size_t)2 << 20;
-    const size_t fb = (19 * N * s->eb + A - 1) / A * A, gb = (s->gbytes + A - 1) / A * A;
+    const size_t fb = (s->fbytes + A - 1) / A * A, gb = (s->gbytes + A - 1) / A * A;
     char *base = nullptr;
     HIP_TRY(s, hipMalloc((void **)&base, 2 * (fb + gb)));
     s->lattices = base;
@@ -543,8 +549,8 @@ static int state_io(aht_solver *s, void *f, void *g, int f64, int write) {
   const size_t N = s->g.N;
   const int p = s->parity;
   const size_t hb = f64 ? 8 : 4;
-  // f32 storage and f32 host format: plain copies
-  if (s->storage == AHT_STORE_F32 && !f64) {
+  // f32 storage in the reference's [q][cell] layout and f32 host format: plain copies
+  if (s->storage == AHT_STORE_F32 && s->g.lb == 0 && !f64) {
     if (write) {
       if (f) HIP_TRY(s, hipMemcpyAsync(s->f[p], f, 19 * N * 4, hipMemcpyHostToDevice, s->st));
       if (g) HIP_TRY(s, hipMemcpyAsync(s->gq[p], g, 7 * N * 4, hipMemcpyHostToDevice, s->st));
@@ -580,12 +586,11 @@ static int state_io(aht_solver *s, void *f, void *g, int f64, int write) {
     const int nq = is_f ? 19 : 7;
     for (int q = 0; q < nq; q++) {
       const float w = is_f ? W19[q] : W7[q];
-      char *plane = dev + (size_t)q * N * s->eb;
       if (write) {
         HIP_TRY(s, hipMemcpyAsync(s->stage, host + (size_t)q * N * hb, N * hb, hipMemcpyHostToDevice, s->st));
-        HIP_TRY(s, aht::launch_encode(s->storage, s->stage, f64, plane, (uint32_t)N, w, is_f, th, s->st));
+        HIP_TRY(s, aht::launch_encode(s->storage, s->g.lb, s->stage, f64, dev, q, (uint32_t)N, w, is_f, th, s->st));
       } else {
-        HIP_TRY(s, aht::launch_decode(s->storage, plane, s->stage, f64, (uint32_t)N, w, is_f, th, s->st));
+        HIP_TRY(s, aht::launch_decode(s->storage, s->g.lb, dev, q, s->stage, f64, (uint32_t)N, w, is_f, th, s->st));
         HIP_TRY(s, hipMemcpyAsync(host + (size_t)q * N * hb, s->stage, N * hb, hipMemcpyDeviceToHost, s->st));
       }
     }
@@ -696,6 +701,9 @@ int aht_batch_add(aht_batch *b, aht_solver *s) {
                  b->device, b->storage);
   if (!s->has_cols || !s->has_params) return bfail(b, AHT_ESTATE, "solver needs columns and parameters first");
   if (s->block != 256) return bfail(b, AHT_EINVAL, "batch members use 256-thread blocks");
+  if (!b->mem.empty() && b->mem[0]->g.lb != s->g.lb)
+    return bfail(b, AHT_EINVAL, "batch members share one population layout (f32 lattices of 4 GiB and more use the "
+                                "[q][cell] layout, smaller ones the blocked one)");
   b->mem.push_back(s);
   s->batch = b;
   b->version++;
@@ -792,7 +800,8 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
       hipGraph_t graph;
       HIP_TRYB(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
       for (int it = 0; it < n; it++) {
-        const hipError_t e = aht::launch_batch_step(b->storage, b->dmem, (int)nm, b->total, it, grid, b->st);
+        const hipError_t e = aht::launch_batch_step(b->storage, b->mem[0]->g.lb, b->dmem, (int)nm, b->total, it, grid,
+                                                    b->st);
         if (e != hipSuccess) {
           hipGraph_t dummy;
           (void)hipStreamEndCapture(b->st, &dummy);
@@ -809,7 +818,7 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
   } else {
     HIP_TRYB(b, hipEventRecord(b->ev0, b->st));
     for (int it = 0; it < n; it++)
-      HIP_TRYB(b, aht::launch_batch_step(b->storage, b->dmem, (int)nm, b->total, it, grid, b->st));
+      HIP_TRYB(b, aht::launch_batch_step(b->storage, b->mem[0]->g.lb, b->dmem, (int)nm, b->total, it, grid, b->st));
   }
   HIP_TRYB(b, hipEventRecord(b->ev1, b->st));
   // member bookkeeping; their next own-stream work waits for this chunk
@@ -875,7 +884,7 @@ int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best) {
   if (rc) return rc;
   if (steps < 1) return fail(s, AHT_EINVAL, "autotune needs steps >= 1");
   if ((rc = set_device(s))) return rc;
-  const size_t N = s->g.N, fb = 19 * N * s->eb, gb = s->gbytes;
+  const size_t fb = s->fbytes, gb = s->gbytes;
   const int p0 = s->parity;
   const int64_t steps0 = s->steps, nacc0 = s->n_acc;
   void *bf = nullptr, *bg = nullptr;

@@ -16,7 +16,7 @@ struct Grid {
   float inv_nx;        // 1 / nx, for the column -> (i, j) split
   uint32_t win;        // tile order: 1 = linear, W > 1 = windows of W consecutive tiles per XCD, 0 = one slab per XCD
   uint32_t th_off;     // theta-referenced D3Q7 lattices: byte offset of the f32 theta plane (0 otherwise)
-  uint32_t pad;
+  int32_t lb;          // population layout: 0 = [q][cell]; LB > 0 = blocked [cell >> LB][q][cell & (2^LB - 1)]
   const uint32_t *colk; // per column: kt | ktop << 10 | kfast << 20 | near << 30 (read by every cell), solver orientation
   const uint32_t *colnb; // per column: ktop of the 8 lateral neighbours, 4 bits each (see pack_neighbours in api.hip),
                          // read by near-solid cells only
@@ -67,7 +67,10 @@ uint32_t content_slots(const Grid &g, int block);       // wavefront slots of th
 int storage_bytes(int storage);                         // bytes per population element of an AHT_STORE_* format
 bool storage_theta_ref(int storage);                    // D3Q7 lattice = f32 theta + deviations (f16, bf16)
 size_t theta_offset(int storage, size_t N);             // byte offset of the theta plane in a D3Q7 lattice (or 0)
-size_t g_lattice_bytes(int storage, size_t N);          // bytes of one D3Q7 lattice
+int population_layout(int storage, size_t N);           // Grid::lb for a lattice of N cells in this format
+size_t lattice_cells(int lb, size_t N);                 // cells per population, rounded up to whole blocks
+size_t f_lattice_bytes(int storage, int lb, size_t N);  // bytes of one D3Q19 lattice
+size_t g_lattice_bytes(int storage, int lb, size_t N);  // bytes of one D3Q7 lattice
 
 // kernels are templated on the storage format; the launchers dispatch on the AHT_STORE_* id
 // levels = cells per thread along z (1, 2 or 4): the tile is (block of columns, chunk of `levels` levels);
@@ -79,7 +82,7 @@ hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const Streams
                        const void *gA, void *gB, const Acc *acc, float *out4, int block, int levels, hipStream_t st);
 Streams make_streams(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB);
 // one step of every batch member (launch index `it` of a chunk) on a persistent grid of `grid` blocks of 256
-hipError_t launch_batch_step(int storage, const Member *mem, int nmem, uint32_t total_tiles, int it, int grid,
+hipError_t launch_batch_step(int storage, int lb, const Member *mem, int nmem, uint32_t total_tiles, int it, int grid,
                              hipStream_t st);
 hipError_t launch_init(int storage, const Grid &g, void *f0, void *f1, hipStream_t st);
 hipError_t launch_reinit(int storage, const Grid &g, const uint32_t *old_ktop, void *f, void *gq, hipStream_t st);
@@ -93,12 +96,12 @@ hipError_t launch_stats(int storage, const Grid &g, const void *fA, const void *
 uint32_t stats_partials(const Grid &g);
 // fixed-order sum of n doubles into *out (one block)
 hipError_t launch_sum(const double *x, uint32_t n, double *out, hipStream_t st);
-// one population plane of n cells between storage and f32 (or f64) host-format buffers; w = rest weight for f planes
-// g planes of theta-referenced formats decode / encode against the lattice's theta plane `th` (w = the plane's D3Q7
-// weight); otherwise th is unused
-hipError_t launch_decode(int storage, const void *src, void *dst, int dst_f64, uint32_t n, float w, int is_f,
-                         const float *th, hipStream_t st);
-hipError_t launch_encode(int storage, const void *src, int src_f64, void *dst, uint32_t n, float w, int is_f,
-                         const float *th, hipStream_t st);
+// population q of a lattice of n cells (layout lb) between storage and an f32 (or f64) host-format plane; w = rest
+// weight for f populations; g populations of theta-referenced formats decode / encode against the lattice's theta
+// plane `th` (w = the population's D3Q7 weight); otherwise th is unused
+hipError_t launch_decode(int storage, int lb, const void *lat, int q, void *dst, int dst_f64, uint32_t n, float w,
+                         int is_f, const float *th, hipStream_t st);
+hipError_t launch_encode(int storage, int lb, const void *src, int src_f64, void *lat, int q, uint32_t n, float w,
+                         int is_f, const float *th, hipStream_t st);
 
 }  // namespace aht

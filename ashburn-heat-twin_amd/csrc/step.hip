@@ -28,14 +28,16 @@ struct Outflow {
   R rn, uxn, uyn, uzn, usqn, uxc, uyc, usqc;
 };
 
-// base address of population q shifted back by `off` cells (the pull offset), as a plain integer
+// base address of population q shifted back by `off` cells (the pull offset), as a plain integer ([q][cell] layout)
 template <class S>
 __device__ __forceinline__ const void *shifted(const void *a, size_t N, int q, int off) {
+  static_assert(S::LB == 0, "a shifted base exists only in the [q][cell] layout");
   return (const void *)((uintptr_t)a + (uintptr_t)((int64_t)q * (int64_t)N - off) * (uintptr_t)S::bytes);
 }
-template <class S>
+// first element of population q of a lattice with Q populations (cell offsets from there: cell_off<S, Q>)
+template <class S, int Q>
 __device__ __forceinline__ void *plane(void *a, size_t N, int q) {
-  return (void *)((uintptr_t)a + (uintptr_t)q * N * (uintptr_t)S::bytes);
+  return (void *)((uintptr_t)a + plane_off<S>(Q, N, q));
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -93,7 +95,7 @@ __device__ __forceinline__ bool col_near(uint32_t x) { return (x >> 30) & 1u; }
 // element access of population qq at cell index idx, decoded with weight w19<q> (deviation formats)
 template <class S, int q>
 __device__ __forceinline__ typename S::real ldf(const typename S::elem *__restrict__ f, size_t N, int qq, uint32_t idx) {
-  return S::dec_f(f[(size_t)qq * N + idx], w19<q>());
+  return S::dec_f(f[lat_index<S>(19, N, qq, idx)], w19<q>());
 }
 __device__ __forceinline__ float w7q(int q) { return q == 0 ? 0.25f : 0.125f; }
 
@@ -101,17 +103,17 @@ __device__ __forceinline__ float w7q(int q) { return q == 0 ? 0.25f : 0.125f; }
 template <class S>
 __device__ __forceinline__ typename S::real ldg(const Grid &g, const void *gA, int qq, uint32_t idx) {
   using E = typename S::elem;
-  const E e = ((const E *)gA)[(size_t)qq * g.N + idx];
+  const E e = ((const E *)gA)[lat_index<S>(7, g.N, qq, idx)];
   if constexpr (S::theta_ref) return S::dec_g(e, w7q(qq), ld_theta(gA, g.th_off + idx * 4u));
   else return S::dec_g(e, 0.0f, typename S::real(0));
 }
 
 // Stream bases. TAB = true (the single-domain step) reads them from the host-built Streams table; TAB = false
 // (batched launches, the surface read) forms them from the lattice pointers.
-template <class S, bool TAB>
+template <class S, bool TAB, int Q>
 __device__ __forceinline__ void *plane_base(const Streams *sp, int slot, void *lat, size_t N, int q) {
   if constexpr (TAB) return (void *)sp->wr[slot];
-  else return plane<S>(lat, N, q);
+  else return plane<S, Q>(lat, N, q);
 }
 
 // the seven post-collision theta populations p of cell n: raw, or theta = sum_q p_q in f32 plus the deviations
@@ -119,7 +121,7 @@ template <class S, bool TAB>
 __device__ __forceinline__ void store_g(const Grid &g, const Streams *sp, void *gB, uint32_t n,
                                         const typename S::real (&p)[7]) {
   using R = typename S::real;
-  const uint32_t boff = n * (uint32_t)S::bytes;
+  const uint32_t boff = cell_off<S, 7>(n);
   R th = R(0);
   if constexpr (S::theta_ref) {
     th = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + p[6]);
@@ -127,7 +129,8 @@ __device__ __forceinline__ void store_g(const Grid &g, const Streams *sp, void *
     else st_theta(gB, g.th_off + n * 4u, th);
   }
 #pragma unroll
-  for (int q = 0; q < 7; q++) S::st_raw(plane_base<S, TAB>(sp, 19 + q, gB, g.N, q), boff, S::enc_g(p[q], w7q(q), th));
+  for (int q = 0; q < 7; q++)
+    S::st_raw(plane_base<S, TAB, 7>(sp, 19 + q, gB, g.N, q), boff, S::enc_g(p[q], w7q(q), th));
 }
 
 template <int q>
@@ -303,24 +306,42 @@ struct FastPull {
   template <class S, bool TAB>
   __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
                                              const Streams *sp, const void *fA, const void *gA, uint32_t n) {
-    const uint32_t boff = n * (uint32_t)S::bytes;
-    const void *bf;
-    if constexpr (TAB) bf = (const void *)sp->rd[q];
-    else bf = shifted<S>(fA, g.N, q, CX[q] + CY[q] * g.nx + CZ[q] * (int)g.NXY);
-    f[q] = S::dec_f(S::ld_raw(bf, boff), w19<q>());
-    if constexpr (q < 7) {
-      const int o7 = CX7[q] + CY7[q] * g.nx + CZ7[q] * (int)g.NXY;
-      const void *bg;
-      if constexpr (TAB) bg = (const void *)sp->rd[19 + q];
-      else bg = shifted<S>(gA, g.N, q, o7);
-      const typename S::elem e = S::ld_raw(bg, boff);
-      if constexpr (S::theta_ref) {
-        const void *thb;                                                  // theta plane, pulled
-        if constexpr (TAB) thb = (const void *)sp->rd[26 + q];
-        else thb = (const void *)((uintptr_t)gA + g.th_off - (intptr_t)o7 * 4);
-        gg[q] = S::dec_g(e, w7q(q), ld_theta(thb, n * 4u));
-      } else {
-        gg[q] = S::dec_g(e, 0.0f, typename S::real(0));
+    if constexpr (S::LB != 0) {
+      // blocked layout: unshifted population bases, per-lane offsets of the pulled cell (clamped into the lattice: a
+      // face cell's out-of-domain pulls are replaced afterwards, and the block arithmetic needs a cell in range)
+      static_assert(!S::theta_ref, "the blocked layout is an f32 / f64 layout");
+      const int off = CX[q] + CY[q] * g.nx + CZ[q] * (int)g.NXY;
+      const uint32_t src = (uint32_t)min(max((int)n - off, 0), (int)g.N - 1);
+      const void *bf;
+      if constexpr (TAB) bf = (const void *)sp->rd[q];
+      else bf = plane<S, 19>((void *)fA, g.N, q);
+      f[q] = S::dec_f(S::ld_raw(bf, cell_off<S, 19>(src)), w19<q>());
+      if constexpr (q < 7) {                                    // D3Q7 velocity q = D3Q19 velocity q
+        const void *bg;
+        if constexpr (TAB) bg = (const void *)sp->rd[19 + q];
+        else bg = plane<S, 7>((void *)gA, g.N, q);
+        gg[q] = S::dec_g(S::ld_raw(bg, cell_off<S, 7>(src)), 0.0f, typename S::real(0));
+      }
+    } else {   // [q][cell]: the pull offset is folded into each population's base
+      const uint32_t boff = n * (uint32_t)S::bytes;
+      const void *bf;
+      if constexpr (TAB) bf = (const void *)sp->rd[q];
+      else bf = shifted<S>(fA, g.N, q, CX[q] + CY[q] * g.nx + CZ[q] * (int)g.NXY);
+      f[q] = S::dec_f(S::ld_raw(bf, boff), w19<q>());
+      if constexpr (q < 7) {
+        const int o7 = CX7[q] + CY7[q] * g.nx + CZ7[q] * (int)g.NXY;
+        const void *bg;
+        if constexpr (TAB) bg = (const void *)sp->rd[19 + q];
+        else bg = shifted<S>(gA, g.N, q, o7);
+        const typename S::elem e = S::ld_raw(bg, boff);
+        if constexpr (S::theta_ref) {
+          const void *thb;                                                  // theta plane, pulled
+          if constexpr (TAB) thb = (const void *)sp->rd[26 + q];
+          else thb = (const void *)((uintptr_t)gA + g.th_off - (intptr_t)o7 * 4);
+          gg[q] = S::dec_g(e, w7q(q), ld_theta(thb, n * 4u));
+        } else {
+          gg[q] = S::dec_g(e, 0.0f, typename S::real(0));
+        }
       }
     }
     FastPull<q + 1>::template run<S, TAB>(f, gg, g, sp, fA, gA, n);
@@ -355,12 +376,11 @@ struct BounceFix {
   template <class S>
   __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
                                              const void *fA, const void *gA, uint32_t n, int r, uint32_t nb) {
-    const uint32_t boff = n * (uint32_t)S::bytes;
     if (source_solid<CX[q], CY[q], CZ[q]>(r, nb))
-      f[q] = S::dec_f(S::ld_raw(plane<S>((void *)fA, g.N, OPP[q]), boff), w19<q>());
+      f[q] = S::dec_f(S::ld_raw(plane<S, 19>((void *)fA, g.N, OPP[q]), cell_off<S, 19>(n)), w19<q>());
     if constexpr (q < 7) {
       if (source_solid<CX7[q], CY7[q], CZ7[q]>(r, nb)) {
-        const typename S::elem e = S::ld_raw(plane<S>((void *)gA, g.N, OPP7[q]), boff);
+        const typename S::elem e = S::ld_raw(plane<S, 7>((void *)gA, g.N, OPP7[q]), cell_off<S, 7>(n));
         if constexpr (S::theta_ref) gg[q] = S::dec_g(e, w7q(q), ld_theta(gA, g.th_off + n * 4u));
         else gg[q] = S::dec_g(e, 0.0f, typename S::real(0));
       }
@@ -415,8 +435,8 @@ struct StorePairs {
                                              const Relax<typename S::real> &R) {
     typename S::real a, b;
     collide_pair<q>(m, F, R, a, b);
-    S::st_raw(plane_base<S, TAB>(sp, q, fB, N, q), boff, S::enc_f(a, w19<q>()));
-    S::st_raw(plane_base<S, TAB>(sp, q + 1, fB, N, q + 1), boff, S::enc_f(b, w19<q + 1>()));
+    S::st_raw(plane_base<S, TAB, 19>(sp, q, fB, N, q), boff, S::enc_f(a, w19<q>()));
+    S::st_raw(plane_base<S, TAB, 19>(sp, q + 1, fB, N, q + 1), boff, S::enc_f(b, w19<q + 1>()));
     StorePairs<q + 2>::template run<S, TAB>(sp, fB, N, boff, m, F, R);
   }
 };
@@ -484,7 +504,7 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *s
     const Moments<R> m = moments(f, gg);
     const Forces<R> F = forces<true, true>(g.P, g.rp, g.nz, k, kg, ktc, R(cq.x), R(pr.x), R(pr.y), m);
     const Relax<R> Rx = relax(g.P, g.rp, g.nx, g.ny, i, j, R(pr.z), m, F);
-    const uint32_t boff = n * (uint32_t)S::bytes;
+    const uint32_t boff = cell_off<S, 19>(n);
     S::st_raw(fB, boff, S::enc_f(collide_rest(Rx), w19<0>()));
     StorePairs<1>::template run<S, TAB>(sp, fB, N, boff, m, F, Rx);
     // D3Q7: g^post = a g + b (1 + 4 c.u) + w S (1 - sg)
@@ -658,8 +678,8 @@ __global__ __launch_bounds__(256) void init_kernel(Grid g, void *f0, void *f1) {
     const float wf = q == 0 ? (float)(1.0 / 3.0) : (q < 7 ? (float)(1.0 / 18.0) : (float)(1.0 / 36.0));
     const R fq = R(wf) * (((R(1) + R(3) * cu) + R(4.5) * cu * cu) - R(1.5) * usq);
     const E e = S::enc_f(fq, wf);
-    ((E *)f0)[(size_t)q * N + n] = e;
-    ((E *)f1)[(size_t)q * N + n] = e;
+    ((E *)f0)[lat_index<S>(19, N, q, n)] = e;
+    ((E *)f1)[lat_index<S>(19, N, q, n)] = e;
   }
 }
 
@@ -677,10 +697,10 @@ __global__ __launch_bounds__(256) void reinit_kernel(Grid g, const uint32_t *__r
 #pragma unroll
     for (int q = 0; q < 19; q++) {
       const float wf = q == 0 ? (float)(1.0 / 3.0) : (q < 7 ? (float)(1.0 / 18.0) : (float)(1.0 / 36.0));
-      ((E *)f)[(size_t)q * N + n] = S::enc_f(R(wf), wf);
+      ((E *)f)[lat_index<S>(19, N, q, n)] = S::enc_f(R(wf), wf);
     }
 #pragma unroll
-    for (int q = 0; q < 7; q++) ((E *)gq)[(size_t)q * N + n] = S::enc_g(R(0), w7q(q), R(0));
+    for (int q = 0; q < 7; q++) ((E *)gq)[lat_index<S>(7, N, q, n)] = S::enc_g(R(0), w7q(q), R(0));
     if constexpr (S::theta_ref) ((float *)((char *)gq + g.th_off))[n] = 0.0f;
   }
 }
@@ -826,24 +846,25 @@ __global__ __launch_bounds__(256) void sum_f64(const double *__restrict__ x, uin
 
 // storage <-> f32 / f64 conversion of one population plane (read_state / write_state); w = the plane's weight (f: the
 // D3Q19 weight of its deviation; g: the D3Q7 weight); th = theta plane of a theta-referenced D3Q7 lattice
+// population q of a lattice (is_f: the 19 f populations, else the 7 theta populations) of n cells
 template <class S, typename T>
-__global__ __launch_bounds__(256) void decode_plane(const void *src, T *__restrict__ dst, uint32_t n, float w, int is_f,
-                                                    const float *__restrict__ th) {
+__global__ __launch_bounds__(256) void decode_plane(const void *lat, int q, T *__restrict__ dst, uint32_t n, float w,
+                                                    int is_f, const float *__restrict__ th) {
   using E = typename S::elem;
   using R = typename S::real;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const E e = ((const E *)src)[t];
+  const E e = ((const E *)lat)[lat_index<S>(is_f ? 19 : 7, n, q, t)];
   dst[t] = (T)(is_f ? S::dec_f(e, w) : S::dec_g(e, w, th ? R(th[t]) : R(0)));
 }
 template <class S, typename T>
-__global__ __launch_bounds__(256) void encode_plane(const T *__restrict__ src, void *dst, uint32_t n, float w, int is_f,
-                                                    const float *__restrict__ th) {
+__global__ __launch_bounds__(256) void encode_plane(const T *__restrict__ src, void *lat, int q, uint32_t n, float w,
+                                                    int is_f, const float *__restrict__ th) {
   using E = typename S::elem;
   using R = typename S::real;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  ((E *)dst)[t] = is_f ? S::enc_f(R(src[t]), w) : S::enc_g(R(src[t]), w, th ? R(th[t]) : R(0));
+  ((E *)lat)[lat_index<S>(is_f ? 19 : 7, n, q, t)] = is_f ? S::enc_f(R(src[t]), w) : S::enc_g(R(src[t]), w, th ? R(th[t]) : R(0));
 }
 
 // ---- launchers ------------------------------------------------------------------------------------------------
@@ -864,17 +885,26 @@ bool storage_theta_ref(int storage) { return storage == 1 || storage == 2; }
 size_t theta_offset(int storage, size_t N) {
   return storage_theta_ref(storage) ? (7 * N * (size_t)storage_bytes(storage) + 255) / 256 * 256 : 0;
 }
-size_t g_lattice_bytes(int storage, size_t N) {
-  return storage_theta_ref(storage) ? theta_offset(storage, N) + 4 * N : 7 * N * (size_t)storage_bytes(storage);
+size_t lattice_cells(int lb, size_t N) { return lb ? (N + ((size_t)1 << lb) - 1) >> lb << lb : N; }
+size_t f_lattice_bytes(int storage, int lb, size_t N) { return 19 * lattice_cells(lb, N) * (size_t)storage_bytes(storage); }
+size_t g_lattice_bytes(int storage, int lb, size_t N) {
+  return storage_theta_ref(storage) ? theta_offset(storage, N) + 4 * N
+                                    : 7 * lattice_cells(lb, N) * (size_t)storage_bytes(storage);
+}
+int population_layout(int storage, size_t N) {
+  // f32: blocked when every byte offset of the lattice fits 32 bits (cell_off), else [q][cell]
+  return storage == 0 && f_lattice_bytes(0, StoreF32B::LB, N) < ((size_t)1 << 32) ? StoreF32B::LB : 0;
 }
 
-#define AHT_DISPATCH(storage, CALL)                   \
-  switch (storage) {                                  \
-    case 0: { using S = StoreF32; CALL; } break;       \
-    case 1: { using S = StoreF16; CALL; } break;       \
-    case 2: { using S = StoreBF16; CALL; } break;      \
-    case 3: { using S = StoreF64; CALL; } break;       \
-    default: return hipErrorInvalidValue;             \
+#define AHT_DISPATCH(storage, lb, CALL)                                                \
+  switch (storage) {                                                                   \
+    case 0:                                                                            \
+      if (lb) { using S = StoreF32B; CALL; } else { using S = StoreF32; CALL; }        \
+      break;                                                                           \
+    case 1: { using S = StoreF16; CALL; } break;                                        \
+    case 2: { using S = StoreBF16; CALL; } break;                                       \
+    case 3: { using S = StoreF64; CALL; } break;                                        \
+    default: return hipErrorInvalidValue;                                              \
   }
 
 template <class S, int KB>
@@ -894,7 +924,7 @@ static void launch_step_kb(const Grid &g, const Grid *gd, const Streams *sd, con
 hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const Streams *sd, const void *fA, void *fB,
                        const void *gA, void *gB, const Acc *acc, float *out4, int block, int levels, hipStream_t st) {
   if (g.kmin >= g.nz) return hipSuccess;
-  AHT_DISPATCH(storage, {
+  AHT_DISPATCH(storage, g.lb, {
     if (levels == 4) (launch_step_kb<S, 4>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, st);
     else if (levels == 2) (launch_step_kb<S, 2>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, st);
     else (launch_step_kb<S, 1>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, st);
@@ -909,6 +939,17 @@ Streams make_streams(int storage, const Grid &g, const void *fA, void *fB, const
   static const int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
   const int64_t b = storage_bytes(storage), N = g.N;
   Streams s{};
+  if (g.lb) {   // blocked layout: unshifted population bases, the pull offset goes into the per-lane cell offset
+    for (int q = 0; q < 19; q++) {
+      s.rd[q] = (uint64_t)((uintptr_t)fA + ((size_t)q << g.lb) * b);
+      s.wr[q] = (uint64_t)((uintptr_t)fB + ((size_t)q << g.lb) * b);
+      if (q < 7) {
+        s.rd[19 + q] = (uint64_t)((uintptr_t)gA + ((size_t)q << g.lb) * b);
+        s.wr[19 + q] = (uint64_t)((uintptr_t)gB + ((size_t)q << g.lb) * b);
+      }
+    }
+    return s;
+  }
   for (int q = 0; q < 19; q++) {
     const int64_t off = cx[q] + (int64_t)cy[q] * g.nx + (int64_t)cz[q] * g.NXY;   // D3Q7 q < 7 shares c_q
     s.rd[q] = (uint64_t)((uintptr_t)fA + (q * N - off) * b);
@@ -923,35 +964,35 @@ Streams make_streams(int storage, const Grid &g, const void *fA, void *fB, const
   return s;
 }
 
-hipError_t launch_batch_step(int storage, const Member *mem, int nmem, uint32_t total_tiles, int it, int grid,
+hipError_t launch_batch_step(int storage, int lb, const Member *mem, int nmem, uint32_t total_tiles, int it, int grid,
                              hipStream_t st) {
   if (total_tiles == 0 || nmem == 0) return hipSuccess;
   const int G = (int)std::min<uint32_t>((uint32_t)grid, total_tiles);
-  AHT_DISPATCH(storage, hipLaunchKernelGGL(batch_step_kernel<S>, dim3(G), dim3(256), 0, st, mem, nmem, total_tiles, it))
+  AHT_DISPATCH(storage, lb, hipLaunchKernelGGL(batch_step_kernel<S>, dim3(G), dim3(256), 0, st, mem, nmem, total_tiles, it))
   return hipGetLastError();
 }
 
 hipError_t launch_init(int storage, const Grid &g, void *f0, void *f1, hipStream_t st) {
   const dim3 grid((g.NXY + 255) / 256, g.nz);
-  AHT_DISPATCH(storage, hipLaunchKernelGGL(init_kernel<S>, grid, dim3(256), 0, st, g, f0, f1))
+  AHT_DISPATCH(storage, g.lb, hipLaunchKernelGGL(init_kernel<S>, grid, dim3(256), 0, st, g, f0, f1))
   return hipGetLastError();
 }
 
 hipError_t launch_reinit(int storage, const Grid &g, const uint32_t *old_ktop, void *f, void *gq, hipStream_t st) {
-  AHT_DISPATCH(storage, hipLaunchKernelGGL(reinit_kernel<S>, dim3((g.NXY + 255) / 256), dim3(256), 0, st, g, old_ktop, f, gq))
+  AHT_DISPATCH(storage, g.lb, hipLaunchKernelGGL(reinit_kernel<S>, dim3((g.NXY + 255) / 256), dim3(256), 0, st, g, old_ktop, f, gq))
   return hipGetLastError();
 }
 
 hipError_t launch_surface(int storage, const Grid &g, const void *fA, const void *gA, float conv, float *out3,
                           hipStream_t st) {
-  AHT_DISPATCH(storage, hipLaunchKernelGGL(surface_kernel<S>, dim3((g.NXY + 255) / 256), dim3(256), 0, st, g, fA, gA, conv, out3))
+  AHT_DISPATCH(storage, g.lb, hipLaunchKernelGGL(surface_kernel<S>, dim3((g.NXY + 255) / 256), dim3(256), 0, st, g, fA, gA, conv, out3))
   return hipGetLastError();
 }
 
 hipError_t launch_volume(int storage, const Grid &g, const void *gA, int nzv, float volscale, uint32_t *vout,
                          uint32_t *tmax_bits, hipStream_t st) {
   const uint32_t nw = ((uint32_t)nzv * g.NXY + 3) / 4;
-  AHT_DISPATCH(storage, hipLaunchKernelGGL(volume_kernel<S>, dim3((nw + 255) / 256), dim3(256), 0, st, g, gA, nzv, volscale, vout, tmax_bits))
+  AHT_DISPATCH(storage, g.lb, hipLaunchKernelGGL(volume_kernel<S>, dim3((nw + 255) / 256), dim3(256), 0, st, g, gA, nzv, volscale, vout, tmax_bits))
   return hipGetLastError();
 }
 
@@ -960,7 +1001,7 @@ uint32_t stats_partials(const Grid &g) { return (g.N + 255) / 256; }
 hipError_t launch_stats(int storage, const Grid &g, const void *fA, const void *gA, double *part, double *out8,
                         hipStream_t st) {
   const uint32_t nb = stats_partials(g);
-  AHT_DISPATCH(storage, hipLaunchKernelGGL(stats_kernel<S>, dim3(nb), dim3(256), 0, st, g, fA, gA, part))
+  AHT_DISPATCH(storage, g.lb, hipLaunchKernelGGL(stats_kernel<S>, dim3(nb), dim3(256), 0, st, g, fA, gA, part))
   hipLaunchKernelGGL(stats_finish, dim3(1), dim3(256), 0, st, part, nb, out8);
   return hipGetLastError();
 }
@@ -970,22 +1011,22 @@ hipError_t launch_sum(const double *x, uint32_t n, double *out, hipStream_t st) 
   return hipGetLastError();
 }
 
-hipError_t launch_decode(int storage, const void *src, void *dst, int dst_f64, uint32_t n, float w, int is_f,
-                         const float *th, hipStream_t st) {
+hipError_t launch_decode(int storage, int lb, const void *lat, int q, void *dst, int dst_f64, uint32_t n, float w,
+                         int is_f, const float *th, hipStream_t st) {
   const dim3 grid((n + 255) / 256);
-  AHT_DISPATCH(storage, {
-    if (dst_f64) hipLaunchKernelGGL((decode_plane<S, double>), grid, dim3(256), 0, st, src, (double *)dst, n, w, is_f, th);
-    else hipLaunchKernelGGL((decode_plane<S, float>), grid, dim3(256), 0, st, src, (float *)dst, n, w, is_f, th);
+  AHT_DISPATCH(storage, lb, {
+    if (dst_f64) hipLaunchKernelGGL((decode_plane<S, double>), grid, dim3(256), 0, st, lat, q, (double *)dst, n, w, is_f, th);
+    else hipLaunchKernelGGL((decode_plane<S, float>), grid, dim3(256), 0, st, lat, q, (float *)dst, n, w, is_f, th);
   })
   return hipGetLastError();
 }
 
-hipError_t launch_encode(int storage, const void *src, int src_f64, void *dst, uint32_t n, float w, int is_f,
-                         const float *th, hipStream_t st) {
+hipError_t launch_encode(int storage, int lb, const void *src, int src_f64, void *lat, int q, uint32_t n, float w,
+                         int is_f, const float *th, hipStream_t st) {
   const dim3 grid((n + 255) / 256);
-  AHT_DISPATCH(storage, {
-    if (src_f64) hipLaunchKernelGGL((encode_plane<S, double>), grid, dim3(256), 0, st, (const double *)src, dst, n, w, is_f, th);
-    else hipLaunchKernelGGL((encode_plane<S, float>), grid, dim3(256), 0, st, (const float *)src, dst, n, w, is_f, th);
+  AHT_DISPATCH(storage, lb, {
+    if (src_f64) hipLaunchKernelGGL((encode_plane<S, double>), grid, dim3(256), 0, st, (const double *)src, lat, q, n, w, is_f, th);
+    else hipLaunchKernelGGL((encode_plane<S, float>), grid, dim3(256), 0, st, (const float *)src, lat, q, n, w, is_f, th);
   })
   return hipGetLastError();
 }
