@@ -11,8 +11,8 @@
 // Everything is templated on the arithmetic type R: float for the production kernels (f32, f16 and bf16 storage),
 // double for the FP64 twin that serves as the reference of the reduced-precision error statements.
 // The f32 result equals the reference expressions to f32 rounding (checked against the CPU oracle, which keeps the
-// generator's order, to relative L2 <= 1e-6 per step). Layout: populations [q][cell],
-// cell = (k*ny + j)*nx + i (js/live/kernel.js:17).
+// generator's order, to relative L2 <= 1e-6 per step). Cells: cell = (k*ny + j)*nx + i (js/live/kernel.js:17); the
+// population layouts in HBM are in storage.h.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

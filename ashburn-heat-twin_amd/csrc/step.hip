@@ -2,8 +2,9 @@
 // the storage format (storage.h: f32, f16 / bf16 deviations, f64 twin).
 //
 // Thread mapping: one thread per cell. The x-y plane is flattened into columns c = j*nx + i so that the 64 lanes of a
-// wavefront touch 64 consecutive cells of every [q][cell] stream (one contiguous access per stream per wavefront;
-// stores and unshifted loads are aligned whenever nx*ny is a multiple of 64). blockIdx.y is the level k, starting at
+// wavefront touch 64 consecutive cells of every population stream (one contiguous access per stream per wavefront in
+// the [q][cell] layout, one or two in the blocked f32 layout; stores and unshifted loads are aligned whenever nx*ny is
+// a multiple of 64). blockIdx.y is the level k, starting at
 // the lowest fluid level of the grid, so all-solid planes are never launched. Solid cells above that plane exit.
 //
 // Two streaming paths feed one collision:

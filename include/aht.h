@@ -5,8 +5,9 @@
  * The library owns every device buffer. Callers pass plain host pointers and sizes; there is no torch, no CPU path
  * and no fallback: every entry point fails with a negative AHT_E* code when the GPU or the kernels are unavailable.
  *
- * Layout (as the reference, js/live/kernel.js:17): populations [q][cell], cell = (k*ny + j)*nx + i, j = 0 at the south
- * edge; per-column arrays [j*nx + i] in that solver orientation; surface outputs north-up [row*nx + col], row 0 = north.
+ * Host arrays (as the reference, js/live/kernel.js:17): populations [q][cell], cell = (k*ny + j)*nx + i, j = 0 at the
+ * south edge; per-column arrays [j*nx + i] in that solver orientation; surface outputs north-up [row*nx + col], row 0 =
+ * north. (On the device, f32 lattices below 4 GiB are blocked by 1024 cells; aht_read_state / aht_write_state convert.)
  *
  * Threading: one host thread per solver at a time. Work is enqueued on the solver's own HIP stream; aht_step returns
  * once enqueued, every read returns after the data is on the host.
