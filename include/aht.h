@@ -140,7 +140,9 @@ typedef struct aht_batch aht_batch;
 int aht_batch_create(int32_t device, int32_t storage, aht_batch **out);
 void aht_batch_destroy(aht_batch *b);
 const char *aht_batch_last_error(const aht_batch *b);
-int aht_batch_add(aht_batch *b, aht_solver *s);       /* same device and storage, 256-thread blocks        */
+/* members: same device and storage, 256-thread blocks, and one population layout (f32 solvers below 4 GiB of
+ * lattice are blocked on the device, larger ones are not; a batch cannot mix them) */
+int aht_batch_add(aht_batch *b, aht_solver *s);
 int aht_batch_remove(aht_batch *b, aht_solver *s);
 int aht_batch_size(const aht_batch *b);
 /* one chunk of n launches: member d (in add order) steps nsteps[d] <= n times (NULL: n) and accumulates the FP64
