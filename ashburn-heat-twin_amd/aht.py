@@ -36,7 +36,8 @@ class AhtError(RuntimeError):
 
 class Dims(ctypes.Structure):
     _fields_ = [('nx', ctypes.c_int32), ('ny', ctypes.c_int32), ('nz', ctypes.c_int32), ('storage', ctypes.c_int32),
-                ('device', ctypes.c_int32), ('block', ctypes.c_int32), ('reserved', ctypes.c_int32 * 2)]
+                ('device', ctypes.c_int32), ('block', ctypes.c_int32), ('layout', ctypes.c_int32),
+                ('reserved', ctypes.c_int32)]
 
 
 class Launch(ctypes.Structure):
@@ -148,14 +149,14 @@ def load_dump(d):
 class Solver:
     """One device-resident domain (two f32 lattices of 19 + 7 populations)."""
 
-    def __init__(self, nx, ny, nz, storage=STORE_F32, device=0, block=0):
+    def __init__(self, nx, ny, nz, storage=STORE_F32, device=0, block=0, layout=0):
         L = lib()
         storage = STORAGE_NAMES.get(storage, storage)
         self.storage = storage
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.N = self.nx * self.ny * self.nz
         self.NXY = self.nx * self.ny
-        self.dims = Dims(self.nx, self.ny, self.nz, storage, device, block)
+        self.dims = Dims(self.nx, self.ny, self.nz, storage, device, block, layout)
         h = ctypes.c_void_p()
         rc = L.aht_create(ctypes.byref(self.dims), ctypes.byref(h))
         self.h = h

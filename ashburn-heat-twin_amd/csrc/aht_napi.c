@@ -129,7 +129,8 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   memset(&d, 0, sizeof d);
   if (!get_prop_i32(env, argv[0], "nx", &d.nx, 0) || !get_prop_i32(env, argv[0], "ny", &d.ny, 0) ||
       !get_prop_i32(env, argv[0], "nz", &d.nz, 0) || !get_prop_i32(env, argv[0], "storage", &d.storage, 0) ||
-      !get_prop_i32(env, argv[0], "device", &d.device, 0) || !get_prop_i32(env, argv[0], "block", &d.block, 0))
+      !get_prop_i32(env, argv[0], "device", &d.device, 0) || !get_prop_i32(env, argv[0], "block", &d.block, 0) ||
+      !get_prop_i32(env, argv[0], "layout", &d.layout, 0))
     return NULL;
   aht_solver *s = NULL;
   int rc = aht_create(&d, &s);
@@ -408,7 +409,8 @@ static napi_value js_footprint(napi_env env, napi_callback_info info) {
   aht_dims d;
   memset(&d, 0, sizeof d);
   if (!get_prop_i32(env, argv[0], "nx", &d.nx, 0) || !get_prop_i32(env, argv[0], "ny", &d.ny, 0) ||
-      !get_prop_i32(env, argv[0], "nz", &d.nz, 0) || !get_prop_i32(env, argv[0], "storage", &d.storage, 0))
+      !get_prop_i32(env, argv[0], "nz", &d.nz, 0) || !get_prop_i32(env, argv[0], "storage", &d.storage, 0) ||
+      !get_prop_i32(env, argv[0], "layout", &d.layout, 0))
     return NULL;
   napi_value r;
   napi_create_double(env, (double)aht_footprint_bytes(&d), &r);

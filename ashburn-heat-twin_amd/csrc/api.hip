@@ -154,7 +154,7 @@ int64_t aht_footprint_bytes(const aht_dims *d) {
   if (!d || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return -1;
   const int64_t N = (int64_t)d->nx * d->ny * d->nz, NXY = (int64_t)d->nx * d->ny;
   const int eb = d->storage == AHT_STORE_F32 ? 4 : (d->storage == AHT_STORE_F64 ? 8 : 2);
-  const int lb = aht::population_layout(d->storage, (size_t)N);
+  const int lb = d->layout == AHT_LAYOUT_QCELL ? 0 : aht::population_layout(d->storage, (size_t)N);
   const int64_t pop = 2 * (int64_t)(aht::f_lattice_bytes(d->storage, lb, (size_t)N) +
                                     aht::g_lattice_bytes(d->storage, lb, (size_t)N));
   (void)eb;
@@ -171,6 +171,7 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   if (d.nx < 2 || d.ny < 2 || d.nz < 2 || d.nz > 1022) return AHT_EINVAL;
   const uint64_t N = (uint64_t)d.nx * d.ny * d.nz;
   if (d.storage < AHT_STORE_F32 || d.storage > AHT_STORE_F64) return AHT_EINVAL;
+  if (d.layout != AHT_LAYOUT_AUTO && d.layout != AHT_LAYOUT_QCELL) return AHT_EINVAL;
   // byte offsets within one population plane and within a D3Q7 lattice (theta plane) stay 32-bit
   if (N * (uint64_t)aht::storage_bytes(d.storage) >= (1ull << 32)) return AHT_EINVAL;
   if (aht::storage_theta_ref(d.storage) && (uint64_t)aht::g_lattice_bytes(d.storage, 0, (size_t)N) >= (1ull << 32))
@@ -181,7 +182,7 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   s->dims = d;
   s->storage = d.storage;
   s->eb = aht::storage_bytes(d.storage);
-  s->g.lb = aht::population_layout(d.storage, (size_t)N);
+  s->g.lb = d.layout == AHT_LAYOUT_QCELL ? 0 : aht::population_layout(d.storage, (size_t)N);
   s->fbytes = aht::f_lattice_bytes(d.storage, s->g.lb, (size_t)N);
   s->gbytes = aht::g_lattice_bytes(d.storage, s->g.lb, (size_t)N);
   s->device = d.device;

@@ -49,8 +49,14 @@ typedef struct {
   int32_t storage;       /* AHT_STORE_*                                          */
   int32_t device;        /* HIP device ordinal                                   */
   int32_t block;         /* threads per block of the step kernel (0 = default)   */
-  int32_t reserved[2];
+  int32_t layout;        /* AHT_LAYOUT_*: population layout on the device        */
+  int32_t reserved;
 } aht_dims;
+
+/* population layout on the device (the state exchange is [q][cell] either way): AUTO blocks f32 lattices below 4 GiB
+ * by 1024 cells ([cell >> 10][q][cell & 1023], faster), QCELL keeps the reference's [q][cell] for every format */
+#define AHT_LAYOUT_AUTO 0
+#define AHT_LAYOUT_QCELL 1
 
 /* the scalar block of the reference's uniform Params (js/live/kernel.js:44-50; values js/live/solver.js:225-238) */
 typedef struct {

@@ -67,6 +67,31 @@ def test_batch_matches_alone_bitwise(tmp_path, pkg):
             s.close()
 
 
+def test_qcell_layout_matches_blocked_layout(tmp_path, pkg):
+    """f32 lattices below 4 GiB are blocked on the device unless the reference's [q][cell] layout is asked for (the
+    layout of larger f32 lattices): both give the same bits, state and FP64 means alike, through the whole step,
+    read-back and state-exchange path"""
+    dom = domains(tmp_path)[0]
+    out = []
+    for layout in (0, 1):
+        with pkg.Solver.from_dump(dom, layout=layout) as s:
+            s.step(25)
+            s.mean_reset()
+            s.step(30, acc=True)
+            f, g = s.read_state()
+            s3, b = s.read_mean()
+            th, u, v = s.read_surface(dom['conv'])
+            st = s.stats()
+            s.write_state(f, g)
+            s.step(5)
+            f2, g2 = s.read_state()
+        out.append((f, g, s3, b, th, u, v, st, f2, g2))
+    a, q = out
+    for x, y in zip(a[:3] + a[4:7] + a[8:], q[:3] + q[4:7] + q[8:]):
+        assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
+    assert a[3] == q[3] and a[7] == q[7]
+
+
 def test_members_join_and_leave_mid_run(tmp_path, pkg):
     """continuous batching: a member added after others have stepped, another removed early"""
     doms = domains(tmp_path)
