@@ -29,6 +29,17 @@ struct Outflow {
   R rn, uxn, uyn, uzn, usqn, uxc, uyc, usqc;
 };
 
+// element x of a column or profile table of the Grid. The tables live in device memory, but their pointers are read
+// from the Grid in device memory, so the compiler cannot prove them global and would emit flat loads. A pending flat
+// load counts against the LDS/scalar counter as well, which makes every later scalar load (the stream table) wait for
+// it and turns the waits of the pulled loads conservative (vmcnt(0) in the middle of the pulls); global loads do not.
+template <class T>
+__device__ __forceinline__ T ld_tab(const T *p, size_t x) {
+  static_assert(sizeof(T) % 4 == 0 && sizeof(T) <= 16, "table elements are one to four dwords");
+  typedef uint32_t W __attribute__((ext_vector_type(sizeof(T) / 4)));
+  return __builtin_bit_cast(T, ((const __attribute__((address_space(1))) W *)p)[x]);
+}
+
 // base address of population q shifted back by `off` cells (the pull offset), as a plain integer ([q][cell] layout)
 template <class S>
 __device__ __forceinline__ const void *shifted(const void *a, size_t N, int q, int off) {
@@ -455,7 +466,8 @@ __device__ __forceinline__ void load_neighbours(const Grid &g, int i, int j, int
 #pragma unroll
   for (int a = 0; a < 3; a++)
 #pragma unroll
-    for (int b = 0; b < 3; b++) kn[a][b] = (a == 1 && b == 1) ? kg : (int)g.ktop[ys[b] * g.nx + xs[a]];
+    for (int b = 0; b < 3; b++)
+      kn[a][b] = (a == 1 && b == 1) ? kg : (int)ld_tab(g.ktop, (size_t)(ys[b] * g.nx + xs[a]));
 }
 
 // streaming of one fluid cell: the fast or the boundary path
@@ -467,7 +479,7 @@ __device__ __forceinline__ void stream_cell(const Grid &g, const Streams *sp, co
   using E = typename S::elem;
   if (fast) {
     FastPull<0>::template run<S, TAB>(f, gg, g, sp, fA, gA, n);
-    if (k < kfast) BounceFix<1>::template run<S>(f, gg, g, fA, gA, n, k - kg, g.colnb[c]);
+    if (k < kfast) BounceFix<1>::template run<S>(f, gg, g, fA, gA, n, k - kg, ld_tab(g.colnb, (size_t)c));
     if (i == 0 || i == g.nx - 1 || j == 0 || j == g.ny - 1) {
       const Outflow<typename S::real> ob = outflow_prelude<S>(g, (const E *)fA, g.N, n, i, j);
       FaceFix<1>::template run<S>(f, gg, g, fA, gA, n, i, j, uI, vI, ob);
@@ -496,10 +508,11 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *s
     const uint32_t n = (uint32_t)k * g.NXY + (uint32_t)c;
     int i, j;
     split_column(g, c, i, j);
-    const float4 pr = g.prof[max(k - ktc, 0)];
+    const float4 pr = ld_tab(g.prof, (size_t)max(k - ktc, 0));
     const int kfast = col_kfast(ck);
     const bool fast = (k >= kfast || col_near(ck)) && k < g.nz - 1;
-    const float2 cq = (k == kg) ? g.colcq[c] : make_float2(0.0f, 0.0f);   // drag and source act at k == ktop only
+    // drag and source act at k == ktop only
+    const float2 cq = (k == kg) ? ld_tab(g.colcq, (size_t)c) : make_float2(0.0f, 0.0f);
     R f[19], gg[7];
     stream_cell<S, TAB>(g, sp, fA, gA, n, c, i, j, k, kg, kfast, fast, R(pr.x), R(pr.y), f, gg);
     const Moments<R> m = moments(f, gg);
@@ -565,7 +578,7 @@ __device__ __forceinline__ void content_add(const Acc &acc, uint32_t slot, doubl
 template <class S, bool ACC, bool OUT>
 __device__ __forceinline__ void step_cell(const Grid &g, const void *fA, void *fB, const void *gA, void *gB,
                                           const Acc &acc, float *__restrict__ out4, int c, int k, uint32_t slot) {
-  const uint32_t ck = g.colk[c < (int)g.NXY ? c : 0];
+  const uint32_t ck = ld_tab(g.colk, (size_t)(c < (int)g.NXY ? c : 0));
   const double th = step_cell_body<S, ACC, OUT, false>(g, nullptr, fA, fB, gA, gB, acc, out4, c, k, ck);
   if (ACC) content_add(acc, slot, th);
 }
@@ -582,7 +595,7 @@ __global__ __launch_bounds__(256) void step_kernel(const Grid *__restrict__ gp, 
   const uint32_t kk = div_by(t, bx, inv_bx, cb);
   const int c = (int)(cb * blockDim.x + threadIdx.x);
   const int k0 = g.kmin + (int)kk * KB;
-  const uint32_t ck = g.colk[c < (int)g.NXY ? c : 0];
+  const uint32_t ck = ld_tab(g.colk, (size_t)(c < (int)g.NXY ? c : 0));
   double th = 0.0;
   for (int u = 0; u < KB; u++)
     th += step_cell_body<S, ACC, OUT, true>(g, sp, fA, fB, gA, gB, acc, out4, c, k0 + u, ck);
