@@ -18,6 +18,7 @@
 //     interior pull with half-way bounce-back from solid sources), with the 3 x 3 neighbour ktop values loaded once;
 //     for face and lid cells, and near-solid cells of columns whose neighbour word cannot hold the heights.
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels.h"
 #include "storage.h"
@@ -383,29 +384,98 @@ __device__ __forceinline__ bool source_solid(int r, uint32_t nb) {
   }
 }
 
+// a loaded element in a register of its own (16-bit elements zero-extended to 32 bits, so that the compiler does not
+// pack two of them into one register, which would need the load to have completed)
+template <class E>
+using Slot = std::conditional_t<sizeof(E) == 2, uint32_t, E>;
+template <class E>
+__device__ __forceinline__ Slot<E> to_slot(E e) {
+  if constexpr (sizeof(E) == 2) return (uint32_t)__builtin_bit_cast(uint16_t, e);
+  else return e;
+}
+template <class E>
+__device__ __forceinline__ E from_slot(Slot<E> v) {
+  if constexpr (sizeof(E) == 2) return __builtin_bit_cast(E, (uint16_t)v);
+  else return v;
+}
+
 template <int q>
 struct BounceFix {
-  template <class S>
-  __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
-                                             const void *fA, const void *gA, uint32_t n, int r, uint32_t nb) {
-    if (source_solid<CX[q], CY[q], CZ[q]>(r, nb))
-      f[q] = S::dec_f(S::ld_raw(plane<S, 19>((void *)fA, g.N, OPP[q]), cell_off<S, 19>(n)), w19<q>());
+  // issue the loads of the replaced populations (lanes that need them only) before the first use of any of them, so
+  // that a wavefront with several solid sources waits for memory once instead of once per population
+  template <class S, class T>
+  __device__ __forceinline__ static void load(T (&ef)[19], T (&eg)[7], bool (&sf)[19], bool (&sg)[7], const Grid &g,
+                                              const void *fA, const void *gA, uint32_t n, int r, uint32_t nb) {
+    sf[q] = source_solid<CX[q], CY[q], CZ[q]>(r, nb);
+    if (sf[q]) ef[q] = to_slot(S::ld_raw(plane<S, 19>((void *)fA, g.N, OPP[q]), cell_off<S, 19>(n)));
     if constexpr (q < 7) {
-      if (source_solid<CX7[q], CY7[q], CZ7[q]>(r, nb)) {
-        const typename S::elem e = S::ld_raw(plane<S, 7>((void *)gA, g.N, OPP7[q]), cell_off<S, 7>(n));
-        if constexpr (S::theta_ref) gg[q] = S::dec_g(e, w7q(q), ld_theta(gA, g.th_off + n * 4u));
-        else gg[q] = S::dec_g(e, 0.0f, typename S::real(0));
+      sg[q] = source_solid<CX7[q], CY7[q], CZ7[q]>(r, nb);
+      if (sg[q]) eg[q] = to_slot(S::ld_raw(plane<S, 7>((void *)gA, g.N, OPP7[q]), cell_off<S, 7>(n)));
+    }
+    BounceFix<q + 1>::template load<S>(ef, eg, sf, sg, g, fA, gA, n, r, nb);
+  }
+  template <class S, class T>
+  __device__ __forceinline__ static void apply(typename S::real (&f)[19], typename S::real (&gg)[7], const T (&ef)[19],
+                                               const T (&eg)[7], const bool (&sf)[19], const bool (&sg)[7],
+                                               typename S::real th) {
+    using E = typename S::elem;
+    if (sf[q]) f[q] = S::dec_f(from_slot<E>(ef[q]), w19<q>());
+    if constexpr (q < 7) {
+      if (sg[q]) {
+        if constexpr (S::theta_ref) gg[q] = S::dec_g(from_slot<E>(eg[q]), w7q(q), th);
+        else gg[q] = S::dec_g(from_slot<E>(eg[q]), 0.0f, typename S::real(0));
       }
     }
-    BounceFix<q + 1>::template run<S>(f, gg, g, fA, gA, n, r, nb);
+    BounceFix<q + 1>::template apply<S>(f, gg, ef, eg, sf, sg, th);
   }
 };
 template <>
 struct BounceFix<19> {
-  template <class S>
-  __device__ __forceinline__ static void run(typename S::real (&)[19], typename S::real (&)[7], const Grid &,
-                                             const void *, const void *, uint32_t, int, uint32_t) {}
+  template <class S, class T>
+  __device__ __forceinline__ static void load(T (&)[19], T (&)[7], bool (&)[19], bool (&)[7], const Grid &,
+                                              const void *, const void *, uint32_t, int, uint32_t) {}
+  template <class S, class T>
+  __device__ __forceinline__ static void apply(typename S::real (&)[19], typename S::real (&)[7], const T (&)[19],
+                                               const T (&)[7], const bool (&)[19], const bool (&)[7],
+                                               typename S::real) {}
 };
+
+// an empty asm that takes a loaded element in and out: its decoding cannot move above it (into the conditional
+// block of its load), so the waits for all the conditional loads before it collapse into one
+template <class E>
+__device__ __forceinline__ void hold(E &e) {
+  if constexpr (sizeof(E) == 2) {
+    uint16_t b = __builtin_bit_cast(uint16_t, e);
+    asm volatile("" : "+v"(b));
+    e = __builtin_bit_cast(E, b);
+  } else if constexpr (sizeof(E) == 4) {
+    uint32_t b = __builtin_bit_cast(uint32_t, e);
+    asm volatile("" : "+v"(b));
+    e = __builtin_bit_cast(E, b);
+  } else {
+    uint64_t b = __builtin_bit_cast(uint64_t, e);
+    asm volatile("" : "+v"(b));
+    e = __builtin_bit_cast(E, b);
+  }
+}
+
+template <class S>
+__device__ __forceinline__ void bounce_fix(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
+                                           const void *fA, const void *gA, uint32_t n, int r, uint32_t nb) {
+  Slot<typename S::elem> ef[19], eg[7];
+  bool sf[19], sg[7];
+  BounceFix<1>::template load<S>(ef, eg, sf, sg, g, fA, gA, n, r, nb);
+  float th = 0.0f;
+  if constexpr (S::theta_ref) {   // the own cell's theta, the reference of its replaced D3Q7 deviations
+    if (sg[1] || sg[2] || sg[3] || sg[4] || sg[5] || sg[6]) th = ld_theta(gA, g.th_off + n * 4u);
+    hold(th);
+  }
+#pragma unroll
+  for (int q = 1; q < 19; q++) hold(ef[q]);
+#pragma unroll
+  for (int q = 1; q < 7; q++) hold(eg[q]);
+  BounceFix<1>::template apply<S>(f, gg, ef, eg, sf, sg, typename S::real(th));
+}
 
 // Face cells (below the lid) also share the fast path's pulls: populations whose source lies outside the lateral
 // faces are then replaced by the inflow / outlet value of the boundary path (lateral19, lateral7). The pulled values
@@ -479,7 +549,7 @@ __device__ __forceinline__ void stream_cell(const Grid &g, const Streams *sp, co
   using E = typename S::elem;
   if (fast) {
     FastPull<0>::template run<S, TAB>(f, gg, g, sp, fA, gA, n);
-    if (k < kfast) BounceFix<1>::template run<S>(f, gg, g, fA, gA, n, k - kg, ld_tab(g.colnb, (size_t)c));
+    if (k < kfast) bounce_fix<S>(f, gg, g, fA, gA, n, k - kg, ld_tab(g.colnb, (size_t)c));
     if (i == 0 || i == g.nx - 1 || j == 0 || j == g.ny - 1) {
       const Outflow<typename S::real> ob = outflow_prelude<S>(g, (const E *)fA, g.N, n, i, j);
       FaceFix<1>::template run<S>(f, gg, g, fA, gA, n, i, j, uI, vI, ob);
