@@ -28,6 +28,7 @@ namespace aht {
 template <typename R>
 struct Outflow {
   R rn, uxn, uyn, uzn, usqn, uxc, uyc, usqc;
+  R own[19];   // the own cell's pre-stream populations (set on an outlet face only, where the outlet reads them)
 };
 
 // element x of a column or profile table of the Grid. The tables live in device memory, but their pointers are read
@@ -174,7 +175,7 @@ __device__ __forceinline__ Outflow<typename S::real> outflow_prelude(const Grid 
   const bool onb = (i == 0 && P.inflow_w == 0) || (i == g.nx - 1 && P.inflow_e == 0) ||
                    (j == 0 && P.inflow_s == 0) || (j == g.ny - 1 && P.inflow_n == 0);
   if (onb) {
-    R a[19];
+    R(&a)[19] = o.own;
     OwnPops<0>::template run<S>(a, fA, N, n);
     R r = a[0], sx = R(0), sy = R(0), sz = R(0);
 #pragma unroll
@@ -204,8 +205,7 @@ __device__ __forceinline__ Outflow<typename S::real> outflow_prelude(const Grid 
 // the pressure outlet's non-equilibrium extrapolation towards rho = 1 (optionally backflow-clipped). Shared by the
 // boundary path and the fast path's face fix-up, so both give the same bits.
 template <class S, int q>
-__device__ __forceinline__ typename S::real lateral19(const Grid &g, const typename S::elem *__restrict__ fA, size_t N,
-                                                      uint32_t n, int i, int j, typename S::real uI,
+__device__ __forceinline__ typename S::real lateral19(const Grid &g, int i, int j, typename S::real uI,
                                                       typename S::real vI, const Outflow<typename S::real> &ob) {
   using R = typename S::real;
   constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
@@ -216,7 +216,7 @@ __device__ __forceinline__ typename S::real lateral19(const Grid &g, const typen
   const R w = R(w19<q>());
   if (inflow) return w * epoly(cdot<cx, cy, 0>(uI, vI, R(0)), uI * uI + vI * vI);   // equilibrium, rho = 1
   const R cu = cdot<cx, cy, cz>(ob.uxn, ob.uyn, ob.uzn);
-  const R own = ldf<S, q>(fA, N, q, n);
+  const R own = ob.own[q];   // not an inflow face: an outlet face, whose prelude loaded the own cell
   if (P.oclip != 0) {
     const R cuc = cdot<cx, cy, cz>(ob.uxc, ob.uyc, ob.uzn);
     return own + w * (epoly(cuc, ob.usqc) - ob.rn * epoly(cu, ob.usqn));
@@ -242,7 +242,7 @@ __device__ __forceinline__ typename S::real pull19(const Grid &g, const typename
   }
   const bool outx = (cx == 1 && i == 0) || (cx == -1 && i == NX - 1);
   const bool outy = (cy == 1 && j == 0) || (cy == -1 && j == NY - 1);
-  if (outx || outy) return lateral19<S, q>(g, fA, N, n, i, j, uI, vI, ob);
+  if (outx || outy) return lateral19<S, q>(g, i, j, uI, vI, ob);
   // interior: half-way bounce-back when the source cell is solid
   const int sk = k - cz;
   if (sk < kn[1 - cx][1 - cy]) return ldf<S, q>(fA, N, OPP[q], n);
@@ -484,29 +484,28 @@ template <int q>
 struct FaceFix {
   template <class S>
   __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
-                                             const void *fA, const void *gA, uint32_t n, int i, int j,
-                                             typename S::real uI, typename S::real vI,
-                                             const Outflow<typename S::real> &ob) {
+                                             const void *gA, uint32_t n, int i, int j, typename S::real uI,
+                                             typename S::real vI, const Outflow<typename S::real> &ob) {
     constexpr int cx = CX[q], cy = CY[q];
     if constexpr (cx != 0 || cy != 0) {
       const bool out = (cx == 1 && i == 0) || (cx == -1 && i == g.nx - 1) || (cy == 1 && j == 0) ||
                        (cy == -1 && j == g.ny - 1);
-      if (out) f[q] = lateral19<S, q>(g, (const typename S::elem *)fA, g.N, n, i, j, uI, vI, ob);
+      if (out) f[q] = lateral19<S, q>(g, i, j, uI, vI, ob);
     }
     if constexpr (q < 7 && (CX7[q] != 0 || CY7[q] != 0)) {
       const bool out = (CX7[q] == 1 && i == 0) || (CX7[q] == -1 && i == g.nx - 1) || (CY7[q] == 1 && j == 0) ||
                        (CY7[q] == -1 && j == g.ny - 1);
       if (out) gg[q] = lateral7<S, q>(g, gA, n, i, j);
     }
-    FaceFix<q + 1>::template run<S>(f, gg, g, fA, gA, n, i, j, uI, vI, ob);
+    FaceFix<q + 1>::template run<S>(f, gg, g, gA, n, i, j, uI, vI, ob);
   }
 };
 template <>
 struct FaceFix<19> {
   template <class S>
   __device__ __forceinline__ static void run(typename S::real (&)[19], typename S::real (&)[7], const Grid &,
-                                             const void *, const void *, uint32_t, int, int, typename S::real,
-                                             typename S::real, const Outflow<typename S::real> &) {}
+                                             const void *, uint32_t, int, int, typename S::real, typename S::real,
+                                             const Outflow<typename S::real> &) {}
 };
 
 template <int q>
@@ -552,7 +551,7 @@ __device__ __forceinline__ void stream_cell(const Grid &g, const Streams *sp, co
     if (k < kfast) bounce_fix<S>(f, gg, g, fA, gA, n, k - kg, ld_tab(g.colnb, (size_t)c));
     if (i == 0 || i == g.nx - 1 || j == 0 || j == g.ny - 1) {
       const Outflow<typename S::real> ob = outflow_prelude<S>(g, (const E *)fA, g.N, n, i, j);
-      FaceFix<1>::template run<S>(f, gg, g, fA, gA, n, i, j, uI, vI, ob);
+      FaceFix<1>::template run<S>(f, gg, g, gA, n, i, j, uI, vI, ob);
     }
   } else {
     int kn[3][3];
