@@ -580,10 +580,17 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *s
     const float4 pr = ld_tab(g.prof, (size_t)max(k - ktc, 0));
     const int kfast = col_kfast(ck);
     const bool fast = (k >= kfast || col_near(ck)) && k < g.nz - 1;
-    // drag and source act at k == ktop only
-    const float2 cq = (k == kg) ? ld_tab(g.colcq, (size_t)c) : make_float2(0.0f, 0.0f);
+    // drag and source act at k == ktop only. The 16-bit formats load them after the pulls are issued, so that the
+    // load does not delay the pulls (0.8 % faster per step); the f32 step measured 0.25 % faster with the load first
+    float2 cq = make_float2(0.0f, 0.0f);
+    if constexpr (S::bytes != 2) {
+      if (k == kg) cq = ld_tab(g.colcq, (size_t)c);
+    }
     R f[19], gg[7];
     stream_cell<S, TAB>(g, sp, fA, gA, n, c, i, j, k, kg, kfast, fast, R(pr.x), R(pr.y), f, gg);
+    if constexpr (S::bytes == 2) {
+      if (k == kg) cq = ld_tab(g.colcq, (size_t)c);
+    }
     const Moments<R> m = moments(f, gg);
     const Forces<R> F = forces<true, true>(g.P, g.rp, g.nz, k, kg, ktc, R(cq.x), R(pr.x), R(pr.y), m);
     const Relax<R> Rx = relax(g.P, g.rp, g.nx, g.ny, i, j, R(pr.z), m, F);
