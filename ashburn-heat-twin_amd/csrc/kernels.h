@@ -7,7 +7,8 @@
 
 namespace aht {
 
-// one solver's device view, passed by value as the kernel argument
+// one solver's device view: the step kernel reads it from a device copy through a pointer (as an argument it would
+// be pinned in scalar registers and spilled); the other kernels take it by value, batch members hold a copy each
 struct Grid {
   int nx, ny, nz;
   int kmin;            // lowest fluid level over all columns (first plane the step launches)
