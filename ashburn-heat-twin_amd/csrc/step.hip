@@ -428,6 +428,21 @@ struct BounceFix {
     }
     BounceFix<q + 1>::template apply<S>(f, gg, ef, eg, sf, sg, th);
   }
+  // one population at a time: load, wait, decode (fewer values live at once)
+  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
+                                             const void *fA, const void *gA, uint32_t n, int r, uint32_t nb) {
+    if (source_solid<CX[q], CY[q], CZ[q]>(r, nb))
+      f[q] = S::dec_f(S::ld_raw(plane<S, 19>((void *)fA, g.N, OPP[q]), cell_off<S, 19>(n)), w19<q>());
+    if constexpr (q < 7) {
+      if (source_solid<CX7[q], CY7[q], CZ7[q]>(r, nb)) {
+        const typename S::elem e = S::ld_raw(plane<S, 7>((void *)gA, g.N, OPP7[q]), cell_off<S, 7>(n));
+        if constexpr (S::theta_ref) gg[q] = S::dec_g(e, w7q(q), ld_theta(gA, g.th_off + n * 4u));
+        else gg[q] = S::dec_g(e, 0.0f, typename S::real(0));
+      }
+    }
+    BounceFix<q + 1>::template run<S>(f, gg, g, fA, gA, n, r, nb);
+  }
 };
 template <>
 struct BounceFix<19> {
@@ -437,7 +452,9 @@ struct BounceFix<19> {
   template <class S, class T>
   __device__ __forceinline__ static void apply(typename S::real (&)[19], typename S::real (&)[7], const T (&)[19],
                                                const T (&)[7], const bool (&)[19], const bool (&)[7],
-                                               typename S::real) {}
+                                               typename S::real) {}  template <class S>
+  __device__ __forceinline__ static void run(typename S::real (&)[19], typename S::real (&)[7], const Grid &,
+                                             const void *, const void *, uint32_t, int, uint32_t) {}
 };
 
 // an empty asm that takes a loaded element in and out: its decoding cannot move above it (into the conditional
@@ -459,22 +476,29 @@ __device__ __forceinline__ void hold(E &e) {
   }
 }
 
-template <class S>
+// TWO_PHASE (the single-domain step): all replacement loads first, one wait, then the decodes. The batched step,
+// whose member table already costs it scalar registers spilled into vector ones, replaces one population at a time:
+// the two-phase form took it from 73 to 109 VGPRs (7 to 4 waves per SIMD)
+template <class S, bool TWO_PHASE>
 __device__ __forceinline__ void bounce_fix(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
                                            const void *fA, const void *gA, uint32_t n, int r, uint32_t nb) {
-  Slot<typename S::elem> ef[19], eg[7];
-  bool sf[19], sg[7];
-  BounceFix<1>::template load<S>(ef, eg, sf, sg, g, fA, gA, n, r, nb);
-  float th = 0.0f;
-  if constexpr (S::theta_ref) {   // the own cell's theta, the reference of its replaced D3Q7 deviations
-    if (sg[1] || sg[2] || sg[3] || sg[4] || sg[5] || sg[6]) th = ld_theta(gA, g.th_off + n * 4u);
-    hold(th);
+  if constexpr (!TWO_PHASE) {
+    BounceFix<1>::template run<S>(f, gg, g, fA, gA, n, r, nb);
+  } else {
+    Slot<typename S::elem> ef[19], eg[7];
+    bool sf[19], sg[7];
+    BounceFix<1>::template load<S>(ef, eg, sf, sg, g, fA, gA, n, r, nb);
+    float th = 0.0f;
+    if constexpr (S::theta_ref) {   // the own cell's theta, the reference of its replaced D3Q7 deviations
+      if (sg[1] || sg[2] || sg[3] || sg[4] || sg[5] || sg[6]) th = ld_theta(gA, g.th_off + n * 4u);
+      hold(th);
+    }
+#pragma unroll
+    for (int q = 1; q < 19; q++) hold(ef[q]);
+#pragma unroll
+    for (int q = 1; q < 7; q++) hold(eg[q]);
+    BounceFix<1>::template apply<S>(f, gg, ef, eg, sf, sg, typename S::real(th));
   }
-#pragma unroll
-  for (int q = 1; q < 19; q++) hold(ef[q]);
-#pragma unroll
-  for (int q = 1; q < 7; q++) hold(eg[q]);
-  BounceFix<1>::template apply<S>(f, gg, ef, eg, sf, sg, typename S::real(th));
 }
 
 // Face cells (below the lid) also share the fast path's pulls: populations whose source lies outside the lateral
@@ -548,7 +572,7 @@ __device__ __forceinline__ void stream_cell(const Grid &g, const Streams *sp, co
   using E = typename S::elem;
   if (fast) {
     FastPull<0>::template run<S, TAB>(f, gg, g, sp, fA, gA, n);
-    if (k < kfast) bounce_fix<S>(f, gg, g, fA, gA, n, k - kg, ld_tab(g.colnb, (size_t)c));
+    if (k < kfast) bounce_fix<S, TAB>(f, gg, g, fA, gA, n, k - kg, ld_tab(g.colnb, (size_t)c));
     if (i == 0 || i == g.nx - 1 || j == 0 || j == g.ny - 1) {
       const Outflow<typename S::real> ob = outflow_prelude<S>(g, (const E *)fA, g.N, n, i, j);
       FaceFix<1>::template run<S>(f, gg, g, gA, n, i, j, uI, vI, ob);
@@ -682,14 +706,16 @@ __global__ __launch_bounds__(256) void step_kernel(const Grid *__restrict__ gp, 
 // One grid of G blocks walks one step of every batch member: member d contributes tiles (256-column block, level) for
 // its fluid levels, concatenated in member order. The grid strides through the list in windows of G tiles, taking
 // the tiles of a window in the XCD-aware order of the single-domain step (runs of 64 tiles per XCD), so the blocks in
-// flight keep one compact front through memory instead of G separate ones. G from a few blocks per CU (persistent)
+// flight keep one compact front through memory instead of G separate ones. With one block per tile (G = total, the
+// serving default) block b takes tile b, the order the single-domain step measured fastest (window 1: 0.836 against
+// 0.845 ms per step at 600x600x64 with 256-column tiles). G from a few blocks per CU (persistent)
 // up to one block per tile is the batch tuner's choice. Per-member fields make the result of a
 // member independent of the mix: it steps only while `it < nsteps` (it = index of this launch within the captured
 // chunk), reads lattice parity0 ^ (it & 1), and accumulates means from launch acc_from on.
 template <class S>
 __global__ __launch_bounds__(256)
 void batch_step_kernel(const Member *__restrict__ mem, int nmem, uint32_t total, int it) {
-  const uint32_t G = gridDim.x, tb = tile_of_block(blockIdx.x, G, 64);
+  const uint32_t G = gridDim.x, tb = G >= total ? blockIdx.x : tile_of_block(blockIdx.x, G, 64);
   // member of the block's first tile by binary search over tile_base (ascending), then advanced as t grows
   int d = 0;
   if (tb < total) {
