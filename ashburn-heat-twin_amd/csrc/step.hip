@@ -316,8 +316,9 @@ struct Pull7<7> {
 // fast path: all 26 sources are fluid interior cells; plain shifted loads through per-population descriptors
 template <int q>
 struct FastPull {
+  // population q of both lattices (D3Q7 velocity q = D3Q19 velocity q for q < 7)
   template <class S, bool TAB>
-  __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
+  __device__ __forceinline__ static void one(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
                                              const Streams *sp, const void *fA, const void *gA, uint32_t n) {
     if constexpr (S::LB != 0) {
       // blocked layout: unshifted population bases, per-lane offsets of the pulled cell (clamped into the lattice: a
@@ -357,11 +358,76 @@ struct FastPull {
         }
       }
     }
+  }
+  template <class S, bool TAB>
+  __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
+                                             const Streams *sp, const void *fA, const void *gA, uint32_t n) {
+    one<S, TAB>(f, gg, g, sp, fA, gA, n);
     FastPull<q + 1>::template run<S, TAB>(f, gg, g, sp, fA, gA, n);
   }
 };
 template <>
 struct FastPull<19> {
+  template <class S, bool TAB>
+  __device__ __forceinline__ static void run(typename S::real (&)[19], typename S::real (&)[7], const Grid &,
+                                             const Streams *, const void *, const void *, uint32_t) {}
+};
+
+// Lid cells (k = nz - 1) off the lateral faces whose 3 x 3 neighbourhood is fluid at the lid and the level below
+// (k >= kfast): the fast path's pulls, except for the populations whose source lies above the lid, which are never
+// issued. A D3Q19 population from above takes the free-slip mirror MIRZ(q) of the lateral neighbour at the same
+// level (js/live/kernel.js:168-181); the D3Q7 one keeps the own cell's value (zero gradient, :225-229). These are
+// the values the boundary path loads for such a cell.
+template <int q>
+struct LidPull {
+  template <class S, bool TAB>
+  __device__ __forceinline__ static void run(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
+                                             const Streams *sp, const void *fA, const void *gA, uint32_t n) {
+    using R = typename S::real;
+    if constexpr (CZ[q] != -1) {
+      FastPull<q>::template one<S, TAB>(f, gg, g, sp, fA, gA, n);
+    } else {
+      constexpr int m = MIRZ[q];              // c_m = (c_x, c_y, +1)
+      const int off = CX[q] + CY[q] * g.nx;   // the lateral neighbour at the same level
+      if constexpr (S::LB != 0) {
+        const uint32_t src = (uint32_t)min(max((int)n - off, 0), (int)g.N - 1);
+        const void *bf;
+        if constexpr (TAB) bf = (const void *)sp->rd[m];
+        else bf = plane<S, 19>((void *)fA, g.N, m);
+        f[q] = S::dec_f(S::ld_raw(bf, cell_off<S, 19>(src)), w19<q>());
+      } else {
+        const void *bf;   // the table's base of m folds in c_z(m) = +1, one plane below; undo that
+        if constexpr (TAB) bf = (const void *)(sp->rd[m] + (uint64_t)g.NXY * S::bytes);
+        else bf = shifted<S>(fA, g.N, m, off);
+        f[q] = S::dec_f(S::ld_raw(bf, n * (uint32_t)S::bytes), w19<q>());
+      }
+      if constexpr (q < 7) {                  // q = 6: the own cell's D3Q7 population
+        if constexpr (S::LB != 0) {
+          const void *bg;
+          if constexpr (TAB) bg = (const void *)sp->rd[19 + q];
+          else bg = plane<S, 7>((void *)gA, g.N, q);
+          gg[q] = S::dec_g(S::ld_raw(bg, cell_off<S, 7>(n)), 0.0f, R(0));
+        } else {
+          const void *bg;   // the table's base folds in c_z = -1, one plane above; undo that
+          if constexpr (TAB) bg = (const void *)(sp->rd[19 + q] - (uint64_t)g.NXY * S::bytes);
+          else bg = shifted<S>(gA, g.N, q, 0);
+          const typename S::elem e = S::ld_raw(bg, n * (uint32_t)S::bytes);
+          if constexpr (S::theta_ref) {
+            const void *thb;
+            if constexpr (TAB) thb = (const void *)(sp->rd[26 + q] - (uint64_t)g.NXY * 4u);
+            else thb = (const void *)((uintptr_t)gA + g.th_off);
+            gg[q] = S::dec_g(e, w7q(q), ld_theta(thb, n * 4u));
+          } else {
+            gg[q] = S::dec_g(e, 0.0f, R(0));
+          }
+        }
+      }
+    }
+    LidPull<q + 1>::template run<S, TAB>(f, gg, g, sp, fA, gA, n);
+  }
+};
+template <>
+struct LidPull<19> {
   template <class S, bool TAB>
   __device__ __forceinline__ static void run(typename S::real (&)[19], typename S::real (&)[7], const Grid &,
                                              const Streams *, const void *, const void *, uint32_t) {}
@@ -563,12 +629,12 @@ __device__ __forceinline__ void load_neighbours(const Grid &g, int i, int j, int
       kn[a][b] = (a == 1 && b == 1) ? kg : (int)ld_tab(g.ktop, (size_t)(ys[b] * g.nx + xs[a]));
 }
 
-// streaming of one fluid cell: the fast or the boundary path
+// streaming of one fluid cell: the fast path below the lid, the lid's fast path, or the boundary path
 template <class S, bool TAB>
 __device__ __forceinline__ void stream_cell(const Grid &g, const Streams *sp, const void *fA, const void *gA,
                                             uint32_t n, int c, int i, int j, int k, int kg, int kfast, bool fast,
-                                            typename S::real uI, typename S::real vI, typename S::real (&f)[19],
-                                            typename S::real (&gg)[7]) {
+                                            bool lid, typename S::real uI, typename S::real vI,
+                                            typename S::real (&f)[19], typename S::real (&gg)[7]) {
   using E = typename S::elem;
   if (fast) {
     FastPull<0>::template run<S, TAB>(f, gg, g, sp, fA, gA, n);
@@ -577,6 +643,8 @@ __device__ __forceinline__ void stream_cell(const Grid &g, const Streams *sp, co
       const Outflow<typename S::real> ob = outflow_prelude<S>(g, (const E *)fA, g.N, n, i, j);
       FaceFix<1>::template run<S>(f, gg, g, gA, n, i, j, uI, vI, ob);
     }
+  } else if (lid) {
+    LidPull<0>::template run<S, TAB>(f, gg, g, sp, fA, gA, n);
   } else {
     int kn[3][3];
     load_neighbours(g, i, j, kg, kn);
@@ -604,6 +672,7 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *s
     const float4 pr = ld_tab(g.prof, (size_t)max(k - ktc, 0));
     const int kfast = col_kfast(ck);
     const bool fast = (k >= kfast || col_near(ck)) && k < g.nz - 1;
+    const bool lid = k == g.nz - 1 && k >= kfast && i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
     // drag and source act at k == ktop only. The 16-bit formats load them after the pulls are issued, so that the
     // load does not delay the pulls (0.8 % faster per step); the f32 step measured 0.25 % faster with the load first
     float2 cq = make_float2(0.0f, 0.0f);
@@ -611,7 +680,7 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *s
       if (k == kg) cq = ld_tab(g.colcq, (size_t)c);
     }
     R f[19], gg[7];
-    stream_cell<S, TAB>(g, sp, fA, gA, n, c, i, j, k, kg, kfast, fast, R(pr.x), R(pr.y), f, gg);
+    stream_cell<S, TAB>(g, sp, fA, gA, n, c, i, j, k, kg, kfast, fast, lid, R(pr.x), R(pr.y), f, gg);
     if constexpr (S::bytes == 2) {
       if (k == kg) cq = ld_tab(g.colcq, (size_t)c);
     }
@@ -764,7 +833,7 @@ __global__ __launch_bounds__(256) void surface_kernel(Grid g, const void *fA, co
   const uint32_t n = (uint32_t)k * g.NXY + (uint32_t)c;
   const float4 pr = g.prof[max(k - ktc, 0)];
   R f[19], gg[7];
-  stream_cell<S, false>(g, nullptr, fA, gA, n, c, i, j, k, kg, 0, false, R(pr.x), R(pr.y), f, gg);
+  stream_cell<S, false>(g, nullptr, fA, gA, n, c, i, j, k, kg, 0, false, false, R(pr.x), R(pr.y), f, gg);
   const Moments<R> m = moments(f, gg);
   const Forces<R> F = forces<true, true>(g.P, g.rp, g.nz, k, kg, ktc, R(g.colcq[c].x), R(pr.x), R(pr.y), m);
   out3[o] = (float)m.th;
