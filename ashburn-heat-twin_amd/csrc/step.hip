@@ -7,16 +7,20 @@
 // a multiple of 64). blockIdx.y is the level k, starting at
 // the lowest fluid level of the grid, so all-solid planes are never launched. Solid cells above that plane exit.
 //
-// Two streaming paths feed one collision:
-//   * fast path: interior cells (not on a lateral face, below the lid). All 26 pulls are plain loads through buffer
+// Three streaming paths feed one collision:
+//   * fast path: every fluid cell below the lid (face and near-solid cells included, see FaceFix and BounceFix; the
+//     description here is of the pulls). All 26 pulls are plain loads through buffer
 //     descriptors: one per population with the pull offset folded into the scalar base, so the only per-lane address
 //     is the cell's byte offset. At or above the column's kfast level (the first level where every source cell in the
 //     3 x 3 x 3 neighbourhood is fluid, host-computed) that is all; below it (near-solid cells) the populations whose
 //     source is solid are then replaced by the bounce-back value, from a per-column word with the neighbours' ktop.
 //     A wavefront mixing cells next to terrain or roofs with free cells thus runs one pull sequence, not two.
+//   * lid fast path (LidPull): lid cells off the faces whose neighbourhood is fluid take the same pulls, with the
+//     populations from above the lid replaced by the free-slip mirror (D3Q19) or the own cell's value (D3Q7).
 //   * boundary path: every branch of js/live/kernel.js:159-243 per population (lid first, then lateral faces, then the
 //     interior pull with half-way bounce-back from solid sources), with the 3 x 3 neighbour ktop values loaded once;
-//     for face and lid cells, and near-solid cells of columns whose neighbour word cannot hold the heights.
+//     for lid cells on a face or next to a solid, and near-solid cells of columns whose neighbour word cannot hold
+//     the heights.
 #include <algorithm>
 #include <type_traits>
 
