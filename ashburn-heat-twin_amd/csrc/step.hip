@@ -531,11 +531,8 @@ struct BounceFix<19> {
 // block of its load), so the waits for all the conditional loads before it collapse into one
 template <class E>
 __device__ __forceinline__ void hold(E &e) {
-  if constexpr (sizeof(E) == 2) {
-    uint16_t b = __builtin_bit_cast(uint16_t, e);
-    asm volatile("" : "+v"(b));
-    e = __builtin_bit_cast(E, b);
-  } else if constexpr (sizeof(E) == 4) {
+  static_assert(sizeof(E) == 4 || sizeof(E) == 8, "16-bit elements are held in their 32-bit slot");
+  if constexpr (sizeof(E) == 4) {
     uint32_t b = __builtin_bit_cast(uint32_t, e);
     asm volatile("" : "+v"(b));
     e = __builtin_bit_cast(E, b);
