@@ -57,8 +57,8 @@ struct aht_solver {
   int levels = 1;             // cells per thread along z
   int storage = AHT_STORE_F32;
   int eb = 4;                 // bytes per stored population element
-  size_t fbytes = 0;          // bytes of one D3Q19 lattice (whole blocks in the blocked layout)
-  size_t gbytes = 0;          // bytes of one D3Q7 lattice (theta-referenced formats add the f32 theta plane)
+  size_t fbytes = 0;          // bytes of one D3Q19 lattice (a blocked f32 pair: the whole array, D3Q7 included)
+  size_t gbytes = 0;          // bytes of one D3Q7 lattice (theta-referenced: + the f32 theta plane; blocked pair: 0)
   void *f[2] = {nullptr, nullptr}, *gq[2] = {nullptr, nullptr};   // parts of `lattices`
   void *lattices = nullptr;
   void *stage = nullptr;      // one population plane in f64, for state conversion
@@ -215,7 +215,11 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
     char *base = nullptr;
     HIP_TRY(s, hipMalloc((void **)&base, 2 * (fb + gb)));
     s->lattices = base;
-    s->f[0] = base; s->gq[0] = base + fb; s->f[1] = base + fb + gb; s->gq[1] = base + 2 * fb + gb;
+    // a blocked f32 pair is one array (gb = 0) whose D3Q7 populations start 19 populations into the first block
+    const size_t go = aht::g_lattice_offset(d.storage, s->g.lb);
+    s->f[0] = base; s->f[1] = base + fb + gb;
+    s->gq[0] = gb ? base + fb : base + go;
+    s->gq[1] = gb ? base + 2 * fb + gb : base + fb + go;
     HIP_TRY(s, hipMemsetAsync(base, 0, 2 * (fb + gb), s->st));
   }
   HIP_TRY(s, hipMalloc(&s->colk, NXY * sizeof(uint32_t)));
@@ -335,6 +339,11 @@ int aht_reset_heat(aht_solver *s) {
   if (!s) return AHT_EINVAL;
   int rc = set_device(s);
   if (rc) return rc;
+  if (s->g.lb) {   // the D3Q7 populations of a blocked pair lie between D3Q19 ones
+    HIP_TRY(s, aht::launch_zero_g_blocked(s->g, s->gq[0], s->st));
+    HIP_TRY(s, aht::launch_zero_g_blocked(s->g, s->gq[1], s->st));
+    return AHT_OK;
+  }
   const size_t b = s->gbytes;                        // zero bits are 0.0 in every storage format (theta too)
   HIP_TRY(s, hipMemsetAsync(s->gq[0], 0, b, s->st));
   HIP_TRY(s, hipMemsetAsync(s->gq[1], 0, b, s->st));
@@ -888,12 +897,14 @@ int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best) {
   const size_t fb = s->fbytes, gb = s->gbytes;
   const int p0 = s->parity;
   const int64_t steps0 = s->steps, nacc0 = s->n_acc;
-  void *bf = nullptr, *bg = nullptr;
+  void *bf = nullptr, *bg = nullptr;   // gb = 0 for a blocked pair: fb covers both lattices
   HIP_TRY(s, hipMalloc(&bf, fb));
-  const hipError_t eg = hipMalloc(&bg, gb);
-  if (eg != hipSuccess) { (void)hipFree(bf); return fail(s, AHT_ENOMEM, "autotune backup: %s", hipGetErrorString(eg)); }
+  if (gb) {
+    const hipError_t eg = hipMalloc(&bg, gb);
+    if (eg != hipSuccess) { (void)hipFree(bf); return fail(s, AHT_ENOMEM, "autotune backup: %s", hipGetErrorString(eg)); }
+  }
   HIP_TRY(s, hipMemcpyAsync(bf, s->f[p0], fb, hipMemcpyDeviceToDevice, s->st));
-  HIP_TRY(s, hipMemcpyAsync(bg, s->gq[p0], gb, hipMemcpyDeviceToDevice, s->st));
+  if (gb) HIP_TRY(s, hipMemcpyAsync(bg, s->gq[p0], gb, hipMemcpyDeviceToDevice, s->st));
   // every shape over `steps` timed steps (after one untimed step that also uploads the changed grid), then the four
   // fastest again twice over 3 x `steps`, best of the two: neighbouring shapes differ by ~1 %, about the noise of a
   // single short timing
@@ -933,10 +944,10 @@ int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best) {
   apply_launch(s, keep);
   // restore the state and the counters: tuning leaves the simulation where it was
   (void)hipMemcpyAsync(s->f[p0], bf, fb, hipMemcpyDeviceToDevice, s->st);
-  (void)hipMemcpyAsync(s->gq[p0], bg, gb, hipMemcpyDeviceToDevice, s->st);
+  if (gb) (void)hipMemcpyAsync(s->gq[p0], bg, gb, hipMemcpyDeviceToDevice, s->st);
   (void)hipStreamSynchronize(s->st);
   (void)hipFree(bf);
-  (void)hipFree(bg);
+  if (bg) (void)hipFree(bg);
   s->parity = p0; s->steps = steps0; s->n_acc = nacc0;
   if (rc) return rc;
   if (best) *best = keep;

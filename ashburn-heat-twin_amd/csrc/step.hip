@@ -1072,14 +1072,32 @@ size_t theta_offset(int storage, size_t N) {
   return storage_theta_ref(storage) ? (7 * N * (size_t)storage_bytes(storage) + 255) / 256 * 256 : 0;
 }
 size_t lattice_cells(int lb, size_t N) { return lb ? (N + ((size_t)1 << lb) - 1) >> lb << lb : N; }
-size_t f_lattice_bytes(int storage, int lb, size_t N) { return 19 * lattice_cells(lb, N) * (size_t)storage_bytes(storage); }
+// a blocked pair is one array of QB populations (the D3Q19 "lattice" covers it, the D3Q7 one lies inside it)
+size_t f_lattice_bytes(int storage, int lb, size_t N) {
+  return (lb ? (size_t)StoreF32B::QB : 19) * lattice_cells(lb, N) * (size_t)storage_bytes(storage);
+}
 size_t g_lattice_bytes(int storage, int lb, size_t N) {
-  return storage_theta_ref(storage) ? theta_offset(storage, N) + 4 * N
-                                    : 7 * lattice_cells(lb, N) * (size_t)storage_bytes(storage);
+  if (lb) return 0;
+  return storage_theta_ref(storage) ? theta_offset(storage, N) + 4 * N : 7 * N * (size_t)storage_bytes(storage);
 }
 int population_layout(int storage, size_t N) {
-  // f32: blocked when every byte offset of the lattice fits 32 bits (cell_off), else [q][cell]
+  // f32: blocked when every byte offset of the pair's array fits 32 bits (cell_off), else [q][cell]
   return storage == 0 && f_lattice_bytes(0, StoreF32B::LB, N) < ((size_t)1 << 32) ? StoreF32B::LB : 0;
+}
+size_t g_lattice_offset(int storage, int lb) {
+  return lb ? ((size_t)19 << lb) * (size_t)storage_bytes(storage) : 0;
+}
+
+// the D3Q7 populations of a blocked f32 pair set to 0 (the D3Q19 populations between them untouched)
+__global__ __launch_bounds__(256) void zero_g_blocked_kernel(float *gq, uint32_t cells) {
+  const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= cells) return;
+  gq[lat_index<StoreF32B>(7, cells, blockIdx.y, n)] = 0.0f;
+}
+hipError_t launch_zero_g_blocked(const Grid &g, void *gq, hipStream_t st) {
+  const uint32_t cells = (uint32_t)lattice_cells(g.lb, g.N);
+  hipLaunchKernelGGL(zero_g_blocked_kernel, dim3((cells + 255) / 256, 7), dim3(256), 0, st, (float *)gq, cells);
+  return hipGetLastError();
 }
 
 #define AHT_DISPATCH(storage, lb, CALL)                                                \
