@@ -13,12 +13,13 @@
 // `lattice_aosoa_pull_nt` is the f32 pull pattern on a blocked layout [cell / 64][q][cell % 64] (26 populations of
 // 64 consecutive cells side by side), the bound of a layout in which a wavefront's streams are one contiguous region.
 // `lattice_pull_nt_x2` / `_x4` move 2 / 4 consecutive cells per thread (8- / 16-byte accesses, the pulled loads
-// dword-aligned), the bound of a step that widens its per-lane accesses. `lattice_blocked_pull_nt` is the f32 step's
-// own layout: a 19-population and a 7-population lattice, each [cell / 1024][q][cell % 1024].
+// dword-aligned), the bound of a step that widens its per-lane accesses. `lattice_blocked_pull_nt` has a
+// 19-population and a 7-population lattice, each [cell / 1024][q][cell % 1024]; `lattice_pair_pull_nt` is the f32
+// step's own layout, both in one array [cell / 1024][19 + 7][cell % 1024].
 // Usage: aht_copy_peak [bytes] [reps] [variant]
 //   (variant: all | grid | flat | unroll4 | unroll4_xcd | unroll4_nt | lattice | lattice_pull | lattice_aa_even
 //    | lattice_aa_odd | lattice_pull_nt | lattice16_pull_nt | lattice_aosoa_pull_nt | lattice_pull_nt_x2
-//    | lattice_pull_nt_x4 | lattice_blocked_pull_nt)
+//    | lattice_pull_nt_x4 | lattice_blocked_pull_nt | lattice_pair_pull_nt)
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -159,6 +160,26 @@ __global__ __launch_bounds__(256) void copy_lattice_blocked(const float *__restr
   }
 }
 
+// the f32 step's layout: the D3Q19 and D3Q7 populations in one array [cell >> LB][26][cell & (2^LB - 1)] (N a
+// multiple of 2^LB), pulled, non-temporal stores
+template <int LB>
+__global__ __launch_bounds__(256) void copy_lattice_pair(const float *__restrict__ a, float *__restrict__ b, uint32_t N,
+                                                         int nx, int nxy) {
+  const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  constexpr uint32_t M = (1u << LB) - 1;
+  float v[26];
+#pragma unroll
+  for (int q = 0; q < 26; q++) {
+    int64_t src = (int64_t)n - (LCX[q] + LCY[q] * nx + LCZ[q] * nxy);
+    src = src < 0 ? 0 : (src >= (int64_t)N ? (int64_t)N - 1 : src);
+    v[q] = a[((((size_t)src >> LB) * 26 + q) << LB) | ((size_t)src & M)];
+  }
+#pragma unroll
+  for (int q = 0; q < 26; q++)
+    __builtin_nontemporal_store(v[q], &b[((((size_t)n >> LB) * 26 + q) << LB) | ((size_t)n & M)]);
+}
+
 // blocked layout: element (q, n) at ((n / 64) * 26 + q) * 64 + n % 64
 __global__ __launch_bounds__(256) void copy_lattice_aosoa(const float *__restrict__ a, float *__restrict__ b, uint32_t N,
                                                           int nx, int nxy) {
@@ -218,15 +239,15 @@ int main(int argc, char **argv) {
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
-  const char *names[15] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
+  const char *names[16] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
                            "lattice_aa_even", "lattice_aa_odd", "lattice_pull_nt", "lattice16_pull_nt",
                            "lattice_aosoa_pull_nt", "lattice_pull_nt_x2", "lattice_pull_nt_x4",
-                           "lattice_blocked_pull_nt"};
+                           "lattice_blocked_pull_nt", "lattice_pair_pull_nt"};
   const int lnx = 600, lnxy = 600 * 600;
   const uint32_t lN = 600u * 600u * 64u;   // lattice variants: 26 * lN floats each way (2.4 GB)
   double best = 0;
   const char *best_name = "";
-  for (int v = 0; v < 15; v++) {
+  for (int v = 0; v < 16; v++) {
     if (strcmp(which, "all") && strcmp(which, names[v])) continue;
     if (v >= 5 && n * 16 < (size_t)26 * lN * 4) continue;   // the lattice needs 2.4 GB per buffer
     auto launch = [&]() {
@@ -246,6 +267,7 @@ int main(int argc, char **argv) {
         case 12: hipLaunchKernelGGL(copy_lattice_wide<2>, dim3((lN / 2 + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 13: hipLaunchKernelGGL(copy_lattice_wide<4>, dim3((lN / 4 + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 14: hipLaunchKernelGGL(copy_lattice_blocked<10>, dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
+        case 15: hipLaunchKernelGGL(copy_lattice_pair<10>, dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
       }
     };
     launch();
