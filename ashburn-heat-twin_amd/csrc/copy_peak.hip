@@ -15,7 +15,7 @@
 // `lattice_pull_nt_x2` / `_x4` move 2 / 4 consecutive cells per thread (8- / 16-byte accesses, the pulled loads
 // dword-aligned), the bound of a step that widens its per-lane accesses. `lattice_blocked_pull_nt` has a
 // 19-population and a 7-population lattice, each [cell / 1024][q][cell % 1024]; `lattice_pair_pull_nt` is the f32
-// step's own layout, both in one array [cell / 1024][19 + 7][cell % 1024].
+// step's own layout, both in one array [cell / 2048][19 + 7][cell % 2048].
 // Usage: aht_copy_peak [bytes] [reps] [variant]
 //   (variant: all | grid | flat | unroll4 | unroll4_xcd | unroll4_nt | lattice | lattice_pull | lattice_aa_even
 //    | lattice_aa_odd | lattice_pull_nt | lattice16_pull_nt | lattice_aosoa_pull_nt | lattice_pull_nt_x2
@@ -267,7 +267,7 @@ int main(int argc, char **argv) {
         case 12: hipLaunchKernelGGL(copy_lattice_wide<2>, dim3((lN / 2 + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 13: hipLaunchKernelGGL(copy_lattice_wide<4>, dim3((lN / 4 + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 14: hipLaunchKernelGGL(copy_lattice_blocked<10>, dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
-        case 15: hipLaunchKernelGGL(copy_lattice_pair<10>, dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
+        case 15: hipLaunchKernelGGL(copy_lattice_pair<11>, dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
       }
     };
     launch();

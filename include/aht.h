@@ -7,7 +7,7 @@
  *
  * Host arrays (as the reference, js/live/kernel.js:17): populations [q][cell], cell = (k*ny + j)*nx + i, j = 0 at the
  * south edge; per-column arrays [j*nx + i] in that solver orientation; surface outputs north-up [row*nx + col], row 0 =
- * north. (On the device, f32 lattice pairs below 4 GiB are blocked by 1024 cells; aht_read_state / aht_write_state
+ * north. (On the device, f32 lattice pairs below 4 GiB are blocked by 2048 cells; aht_read_state / aht_write_state
  * convert.)
  *
  * Threading: one host thread per solver at a time. Work is enqueued on the solver's own HIP stream; aht_step returns
@@ -55,7 +55,7 @@ typedef struct {
 } aht_dims;
 
 /* population layout on the device (the state exchange is [q][cell] either way): AUTO blocks f32 lattice pairs below
- * 4 GiB by 1024 cells (one array [cell >> 10][19 D3Q19 + 7 D3Q7][cell & 1023] per pair, faster), QCELL keeps the
+ * 4 GiB by 2048 cells (one array [cell >> 11][19 D3Q19 + 7 D3Q7][cell & 2047] per pair, faster), QCELL keeps the
  * reference's [q][cell] for every format */
 #define AHT_LAYOUT_AUTO 0
 #define AHT_LAYOUT_QCELL 1
