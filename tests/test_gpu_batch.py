@@ -92,6 +92,24 @@ def test_qcell_layout_matches_blocked_layout(tmp_path, pkg):
     assert a[3] == q[3] and a[7] == q[7]
 
 
+def test_reset_heat_zeroes_theta_populations_only(tmp_path, pkg):
+    """aht_reset_heat sets every D3Q7 population to zero and leaves the D3Q19 populations as they were, in the blocked
+    f32 layout (one array per lattice pair, the D3Q7 populations of each block after its D3Q19 ones; this 5,760-cell
+    domain also has a partly filled last block) as in [q][cell]."""
+    dom = domains(tmp_path)[0]
+    for layout in (0, 1):
+        with pkg.Solver.from_dump(dom, layout=layout) as s:
+            s.step(25)
+            f0, g0 = s.read_state()
+            assert np.abs(g0).max() > 0
+            s.reset_heat()
+            f1, g1 = s.read_state()
+            assert np.array_equal(f1, f0), layout
+            assert not np.any(g1), layout
+            s.step(3)                       # and the step runs on from there
+            assert s.stats()['non_finite'] == 0
+
+
 def test_members_join_and_leave_mid_run(tmp_path, pkg):
     """continuous batching: a member added after others have stepped, another removed early"""
     doms = domains(tmp_path)
