@@ -777,8 +777,8 @@ __global__ __launch_bounds__(256) void step_kernel(const Grid *__restrict__ gp, 
 // its fluid levels, concatenated in member order. The grid strides through the list in windows of G tiles, taking
 // the tiles of a window in the XCD-aware order of the single-domain step (runs of 64 tiles per XCD), so the blocks in
 // flight keep one compact front through memory instead of G separate ones. With one block per tile (G = total, the
-// serving default) block b takes tile b, the order the single-domain step measured fastest (window 1: 0.836 against
-// 0.845 ms per step at 600x600x64 with 256-column tiles). G from a few blocks per CU (persistent)
+// serving default) block b takes tile b, the order the single-domain step measured fastest (window 1 about 1 % faster
+// than window 64 at 600x600x64 with 256-column tiles). G from a few blocks per CU (persistent)
 // up to one block per tile is the batch tuner's choice. Per-member fields make the result of a
 // member independent of the mix: it steps only while `it < nsteps` (it = index of this launch within the captured
 // chunk), reads lattice parity0 ^ (it & 1), and accumulates means from launch acc_from on.
