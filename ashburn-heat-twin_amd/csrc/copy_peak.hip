@@ -15,11 +15,12 @@
 // `lattice_pull_nt_x2` / `_x4` move 2 / 4 consecutive cells per thread (8- / 16-byte accesses, the pulled loads
 // dword-aligned), the bound of a step that widens its per-lane accesses. `lattice_blocked_pull_nt` has a
 // 19-population and a 7-population lattice, each [cell / 1024][q][cell % 1024]; `lattice_pair_pull_nt` is the f32
-// step's own layout, both in one array [cell / 2048][19 + 7][cell % 2048].
+// step's own layout, both in one array [cell / 2048][19 + 7][cell % 2048]; `lattice16_pair_pull_nt` the same for
+// 2-byte elements in blocks of 256 cells, with the f32 theta array of the 16-bit formats pulled beside it.
 // Usage: aht_copy_peak [bytes] [reps] [variant]
 //   (variant: all | grid | flat | unroll4 | unroll4_xcd | unroll4_nt | lattice | lattice_pull | lattice_aa_even
 //    | lattice_aa_odd | lattice_pull_nt | lattice16_pull_nt | lattice_aosoa_pull_nt | lattice_pull_nt_x2
-//    | lattice_pull_nt_x4 | lattice_blocked_pull_nt | lattice_pair_pull_nt)
+//    | lattice_pull_nt_x4 | lattice_blocked_pull_nt | lattice_pair_pull_nt | lattice16_pair_pull_nt)
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -180,6 +181,31 @@ __global__ __launch_bounds__(256) void copy_lattice_pair(const float *__restrict
     __builtin_nontemporal_store(v[q], &b[((((size_t)n >> LB) * 26 + q) << LB) | ((size_t)n & M)]);
 }
 
+// 16-bit populations interleaved as the f32 pair, [cell >> LB][26][cell & (2^LB - 1)], and an f32 theta array [cell]
+// pulled at the 7 D3Q7 offsets, non-temporal stores
+template <int LB>
+__global__ __launch_bounds__(256) void copy_lattice16_pair(const uint16_t *__restrict__ a, uint16_t *__restrict__ b,
+                                                           uint32_t N, int nx, int nxy) {
+  const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  constexpr uint32_t M = (1u << LB) - 1;
+  const float *ta = (const float *)(a + (size_t)26 * N);
+  float *tb = (float *)(b + (size_t)26 * N);
+  uint16_t v[26];
+  float th = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 26; q++) {
+    int64_t src = (int64_t)n - (LCX[q] + LCY[q] * nx + LCZ[q] * nxy);
+    src = src < 0 ? 0 : (src >= (int64_t)N ? (int64_t)N - 1 : src);
+    v[q] = a[((((size_t)src >> LB) * 26 + q) << LB) | ((size_t)src & M)];
+    if (q >= 19) th += ta[src];
+  }
+#pragma unroll
+  for (int q = 0; q < 26; q++)
+    __builtin_nontemporal_store(v[q], &b[((((size_t)n >> LB) * 26 + q) << LB) | ((size_t)n & M)]);
+  __builtin_nontemporal_store(th, &tb[n]);
+}
+
 // blocked layout: element (q, n) at ((n / 64) * 26 + q) * 64 + n % 64
 __global__ __launch_bounds__(256) void copy_lattice_aosoa(const float *__restrict__ a, float *__restrict__ b, uint32_t N,
                                                           int nx, int nxy) {
@@ -239,15 +265,15 @@ int main(int argc, char **argv) {
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
-  const char *names[16] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
+  const char *names[17] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
                            "lattice_aa_even", "lattice_aa_odd", "lattice_pull_nt", "lattice16_pull_nt",
                            "lattice_aosoa_pull_nt", "lattice_pull_nt_x2", "lattice_pull_nt_x4",
-                           "lattice_blocked_pull_nt", "lattice_pair_pull_nt"};
+                           "lattice_blocked_pull_nt", "lattice_pair_pull_nt", "lattice16_pair_pull_nt"};
   const int lnx = 600, lnxy = 600 * 600;
   const uint32_t lN = 600u * 600u * 64u;   // lattice variants: 26 * lN floats each way (2.4 GB)
   double best = 0;
   const char *best_name = "";
-  for (int v = 0; v < 16; v++) {
+  for (int v = 0; v < 17; v++) {
     if (strcmp(which, "all") && strcmp(which, names[v])) continue;
     if (v >= 5 && n * 16 < (size_t)26 * lN * 4) continue;   // the lattice needs 2.4 GB per buffer
     auto launch = [&]() {
@@ -268,6 +294,7 @@ int main(int argc, char **argv) {
         case 13: hipLaunchKernelGGL(copy_lattice_wide<4>, dim3((lN / 4 + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 14: hipLaunchKernelGGL(copy_lattice_blocked<10>, dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 15: hipLaunchKernelGGL(copy_lattice_pair<11>, dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
+        case 16: hipLaunchKernelGGL(copy_lattice16_pair<8>, dim3((lN + 255) / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy); break;
       }
     };
     launch();
@@ -278,7 +305,7 @@ int main(int argc, char **argv) {
     CK(hipEventSynchronize(e1));
     float ms = 0;
     CK(hipEventElapsedTime(&ms, e0, e1));
-    const size_t moved = v == 10 ? (size_t)28 * lN * 2 : (v >= 5 ? (size_t)26 * lN * 4 : n * 16);
+    const size_t moved = (v == 10 || v == 16) ? (size_t)28 * lN * 2 : (v >= 5 ? (size_t)26 * lN * 4 : n * 16);
     const double per = ms / reps, gbs = 2.0 * moved / (per * 1e-3) / 1e9;
     printf("{\"kernel\": \"copy_%s\", \"bytes_copied\": %zu, \"reps\": %d, \"ms_per_launch\": %.5f, "
            "\"GBps_read_plus_write\": %.1f, \"cus\": %d}\n", names[v], moved, reps, per, gbs, cus);
