@@ -206,7 +206,7 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   g.nx = d.nx; g.ny = d.ny; g.nz = d.nz; g.kmin = d.nz; g.NXY = NXY; g.N = (uint32_t)N;
   g.inv_nx = 1.0f / (float)d.nx;
   g.win = 1;
-  g.th_off = (uint32_t)aht::theta_offset(d.storage, (size_t)N);
+  g.th_off = (uint32_t)aht::theta_offset(d.storage, s->g.lb, (size_t)N);
   {
     // both lattice pairs in one allocation (2 MiB-aligned parts): one large mapping keeps the 52+ streams of a step
     // inside few translation entries (the f16 step's L1 TLB misses went from 4 M per launch to none at 600x600x64)
@@ -339,9 +339,12 @@ int aht_reset_heat(aht_solver *s) {
   if (!s) return AHT_EINVAL;
   int rc = set_device(s);
   if (rc) return rc;
-  if (s->g.lb) {   // the D3Q7 populations of a blocked pair lie between D3Q19 ones
-    HIP_TRY(s, aht::launch_zero_g_blocked(s->g, s->gq[0], s->st));
-    HIP_TRY(s, aht::launch_zero_g_blocked(s->g, s->gq[1], s->st));
+  if (s->g.lb) {   // the D3Q7 populations of a blocked pair lie between D3Q19 ones; the theta plane is one range
+    for (int p = 0; p < 2; p++) {
+      HIP_TRY(s, aht::launch_zero_g_blocked(s->storage, s->g, s->gq[p], s->st));
+      if (aht::storage_theta_ref(s->storage))
+        HIP_TRY(s, hipMemsetAsync((char *)s->gq[p] + s->g.th_off, 0, (size_t)s->g.N * 4, s->st));
+    }
     return AHT_OK;
   }
   const size_t b = s->gbytes;                        // zero bits are 0.0 in every storage format (theta too)
@@ -712,7 +715,7 @@ int aht_batch_add(aht_batch *b, aht_solver *s) {
   if (!s->has_cols || !s->has_params) return bfail(b, AHT_ESTATE, "solver needs columns and parameters first");
   if (s->block != 256) return bfail(b, AHT_EINVAL, "batch members use 256-thread blocks");
   if (!b->mem.empty() && b->mem[0]->g.lb != s->g.lb)
-    return bfail(b, AHT_EINVAL, "batch members share one population layout (f32 lattices of 4 GiB and more use the "
+    return bfail(b, AHT_EINVAL, "batch members share one population layout (lattice pairs of 4 GiB and more use the "
                                 "[q][cell] layout, smaller ones the blocked one)");
   b->mem.push_back(s);
   s->batch = b;

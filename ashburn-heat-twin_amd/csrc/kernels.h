@@ -67,13 +67,13 @@ uint32_t content_slots(const Grid &g, int block);       // wavefront slots of th
 
 int storage_bytes(int storage);                         // bytes per population element of an AHT_STORE_* format
 bool storage_theta_ref(int storage);                    // D3Q7 lattice = f32 theta + deviations (f16, bf16)
-size_t theta_offset(int storage, size_t N);             // byte offset of the theta plane in a D3Q7 lattice (or 0)
+size_t theta_offset(int storage, int lb, size_t N);     // byte offset of the theta plane from the D3Q7 base (or 0)
 int population_layout(int storage, size_t N);           // Grid::lb for a lattice of N cells in this format
 size_t lattice_cells(int lb, size_t N);                 // cells per population, rounded up to whole blocks
 size_t f_lattice_bytes(int storage, int lb, size_t N);  // bytes of one D3Q19 lattice (a blocked pair: the whole pair)
 size_t g_lattice_bytes(int storage, int lb, size_t N);  // bytes of one D3Q7 lattice (a blocked pair: 0, inside the D3Q19)
 size_t g_lattice_offset(int storage, int lb);           // a blocked pair's D3Q7 base from its D3Q19 base (else 0)
-hipError_t launch_zero_g_blocked(const Grid &g, void *gq, hipStream_t st);   // D3Q7 populations of a blocked pair = 0
+hipError_t launch_zero_g_blocked(int storage, const Grid &g, void *gq, hipStream_t st);   // blocked pair: D3Q7 = 0
 
 // kernels are templated on the storage format; the launchers dispatch on the AHT_STORE_* id
 // levels = cells per thread along z (1, 2 or 4): the tile is (block of columns, chunk of `levels` levels);

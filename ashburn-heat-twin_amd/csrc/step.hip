@@ -327,7 +327,6 @@ struct FastPull {
     if constexpr (S::LB != 0) {
       // blocked layout: unshifted population bases, per-lane offsets of the pulled cell (clamped into the lattice: a
       // face cell's out-of-domain pulls are replaced afterwards, and the block arithmetic needs a cell in range)
-      static_assert(!S::theta_ref, "the blocked layout is an f32 / f64 layout");
       const int off = CX[q] + CY[q] * g.nx + CZ[q] * (int)g.NXY;
       const uint32_t src = (uint32_t)min(max((int)n - off, 0), (int)g.N - 1);
       const void *bf;
@@ -338,7 +337,15 @@ struct FastPull {
         const void *bg;
         if constexpr (TAB) bg = (const void *)sp->rd[19 + q];
         else bg = plane<S, 7>((void *)gA, g.N, q);
-        gg[q] = S::dec_g(S::ld_raw(bg, cell_off<S, 7>(src)), 0.0f, typename S::real(0));
+        if constexpr (S::theta_ref) {                          // theta plane [cell], pulled as in [q][cell]
+          const int o7 = CX7[q] + CY7[q] * g.nx + CZ7[q] * (int)g.NXY;
+          const void *thb;
+          if constexpr (TAB) thb = (const void *)sp->rd[26 + q];
+          else thb = (const void *)((uintptr_t)gA + g.th_off - (intptr_t)o7 * 4);
+          gg[q] = S::dec_g(S::ld_raw(bg, cell_off<S, 7>(src)), w7q(q), ld_theta(thb, n * 4u));
+        } else {
+          gg[q] = S::dec_g(S::ld_raw(bg, cell_off<S, 7>(src)), 0.0f, typename S::real(0));
+        }
       }
     } else {   // [q][cell]: the pull offset is folded into each population's base
       const uint32_t boff = n * (uint32_t)S::bytes;
@@ -410,7 +417,14 @@ struct LidPull {
           const void *bg;
           if constexpr (TAB) bg = (const void *)sp->rd[19 + q];
           else bg = plane<S, 7>((void *)gA, g.N, q);
-          gg[q] = S::dec_g(S::ld_raw(bg, cell_off<S, 7>(n)), 0.0f, R(0));
+          if constexpr (S::theta_ref) {
+            const void *thb;   // the table's theta base folds in c_z = -1, one plane above; undo that
+            if constexpr (TAB) thb = (const void *)(sp->rd[26 + q] - (uint64_t)g.NXY * 4u);
+            else thb = (const void *)((uintptr_t)gA + g.th_off);
+            gg[q] = S::dec_g(S::ld_raw(bg, cell_off<S, 7>(n)), w7q(q), ld_theta(thb, n * 4u));
+          } else {
+            gg[q] = S::dec_g(S::ld_raw(bg, cell_off<S, 7>(n)), 0.0f, R(0));
+          }
         } else {
           const void *bg;   // the table's base folds in c_z = -1, one plane above; undo that
           if constexpr (TAB) bg = (const void *)(sp->rd[19 + q] - (uint64_t)g.NXY * S::bytes);
@@ -1068,36 +1082,39 @@ uint32_t content_slots(const Grid &g, int block) {
 
 int storage_bytes(int storage) { return storage == 0 ? 4 : (storage == 3 ? 8 : 2); }
 bool storage_theta_ref(int storage) { return storage == 1 || storage == 2; }
-size_t theta_offset(int storage, size_t N) {
-  return storage_theta_ref(storage) ? (7 * N * (size_t)storage_bytes(storage) + 255) / 256 * 256 : 0;
+size_t theta_offset(int storage, int lb, size_t N) {
+  if (!storage_theta_ref(storage)) return 0;
+  if (lb)   // after the blocked pair's array, from the D3Q7 base (19 populations into it)
+    return ((size_t)26 * lattice_cells(lb, N) * (size_t)storage_bytes(storage) + 255) / 256 * 256 -
+           ((size_t)19 << lb) * (size_t)storage_bytes(storage);
+  return (7 * N * (size_t)storage_bytes(storage) + 255) / 256 * 256;
 }
 size_t lattice_cells(int lb, size_t N) { return lb ? (N + ((size_t)1 << lb) - 1) >> lb << lb : N; }
 // a blocked pair is one array of QB populations (the D3Q19 "lattice" covers it, the D3Q7 one lies inside it)
 size_t f_lattice_bytes(int storage, int lb, size_t N) {
-  return (lb ? (size_t)StoreF32B::QB : 19) * lattice_cells(lb, N) * (size_t)storage_bytes(storage);
+  if (!lb) return 19 * N * (size_t)storage_bytes(storage);
+  const size_t pair = (26 * lattice_cells(lb, N) * (size_t)storage_bytes(storage) + 255) / 256 * 256;
+  return pair + (storage_theta_ref(storage) ? 4 * N : 0);   // the theta plane of the 16-bit formats after the array
 }
 size_t g_lattice_bytes(int storage, int lb, size_t N) {
   if (lb) return 0;
-  return storage_theta_ref(storage) ? theta_offset(storage, N) + 4 * N : 7 * N * (size_t)storage_bytes(storage);
+  return storage_theta_ref(storage) ? theta_offset(storage, 0, N) + 4 * N : 7 * N * (size_t)storage_bytes(storage);
 }
 int population_layout(int storage, size_t N) {
-  // f32: blocked when every byte offset of the pair's array fits 32 bits (cell_off), else [q][cell]
-  return storage == 0 && f_lattice_bytes(0, StoreF32B::LB, N) < ((size_t)1 << 32) ? StoreF32B::LB : 0;
+  // blocked when every byte offset of the pair's array (and theta plane) fits 32 bits (cell_off), else [q][cell]
+  const int lb = storage == 0 ? StoreF32B::LB : (storage == 1 || storage == 2 ? StoreF16B::LB : 0);
+  return lb && f_lattice_bytes(storage, lb, N) < ((size_t)1 << 32) ? lb : 0;
 }
 size_t g_lattice_offset(int storage, int lb) {
   return lb ? ((size_t)19 << lb) * (size_t)storage_bytes(storage) : 0;
 }
 
-// the D3Q7 populations of a blocked f32 pair set to 0 (the D3Q19 populations between them untouched)
-__global__ __launch_bounds__(256) void zero_g_blocked_kernel(float *gq, uint32_t cells) {
+// the D3Q7 populations of a blocked pair set to 0 (the D3Q19 populations between them untouched)
+template <class S>
+__global__ __launch_bounds__(256) void zero_g_blocked_kernel(typename S::elem *gq, uint32_t cells) {
   const uint32_t n = blockIdx.x * 256 + threadIdx.x;
   if (n >= cells) return;
-  gq[lat_index<StoreF32B>(7, cells, blockIdx.y, n)] = 0.0f;
-}
-hipError_t launch_zero_g_blocked(const Grid &g, void *gq, hipStream_t st) {
-  const uint32_t cells = (uint32_t)lattice_cells(g.lb, g.N);
-  hipLaunchKernelGGL(zero_g_blocked_kernel, dim3((cells + 255) / 256, 7), dim3(256), 0, st, (float *)gq, cells);
-  return hipGetLastError();
+  gq[lat_index<S>(7, cells, blockIdx.y, n)] = typename S::elem{};
 }
 
 #define AHT_DISPATCH(storage, lb, CALL)                                                \
@@ -1105,8 +1122,12 @@ hipError_t launch_zero_g_blocked(const Grid &g, void *gq, hipStream_t st) {
     case 0:                                                                            \
       if (lb) { using S = StoreF32B; CALL; } else { using S = StoreF32; CALL; }        \
       break;                                                                           \
-    case 1: { using S = StoreF16; CALL; } break;                                        \
-    case 2: { using S = StoreBF16; CALL; } break;                                       \
+    case 1:                                                                            \
+      if (lb) { using S = StoreF16B; CALL; } else { using S = StoreF16; CALL; }        \
+      break;                                                                           \
+    case 2:                                                                            \
+      if (lb) { using S = StoreBF16B; CALL; } else { using S = StoreBF16; CALL; }      \
+      break;                                                                           \
     case 3: { using S = StoreF64; CALL; } break;                                        \
     default: return hipErrorInvalidValue;                                              \
   }
@@ -1150,8 +1171,13 @@ Streams make_streams(int storage, const Grid &g, const void *fA, void *fB, const
       if (q < 7) {
         s.rd[19 + q] = (uint64_t)((uintptr_t)gA + ((size_t)q << g.lb) * b);
         s.wr[19 + q] = (uint64_t)((uintptr_t)gB + ((size_t)q << g.lb) * b);
+        if (storage_theta_ref(storage)) {   // the theta plane stays [cell]: pulled bases as in [q][cell]
+          const int64_t o7 = cx[q] + (int64_t)cy[q] * g.nx + (int64_t)cz[q] * g.NXY;
+          s.rd[26 + q] = (uint64_t)((uintptr_t)gA + g.th_off - o7 * 4);
+        }
       }
     }
+    if (storage_theta_ref(storage)) s.wr[26] = (uint64_t)((uintptr_t)gB + g.th_off);
     return s;
   }
   for (int q = 0; q < 19; q++) {
@@ -1166,6 +1192,14 @@ Streams make_streams(int storage, const Grid &g, const void *fA, void *fB, const
   }
   if (storage_theta_ref(storage)) s.wr[26] = (uint64_t)((uintptr_t)gB + g.th_off);
   return s;
+}
+
+hipError_t launch_zero_g_blocked(int storage, const Grid &g, void *gq, hipStream_t st) {
+  if (!g.lb) return hipErrorInvalidValue;
+  const uint32_t cells = (uint32_t)lattice_cells(g.lb, g.N);
+  AHT_DISPATCH(storage, g.lb, hipLaunchKernelGGL(zero_g_blocked_kernel<S>, dim3((cells + 255) / 256, 7), dim3(256), 0,
+                                                 st, (typename S::elem *)gq, cells))
+  return hipGetLastError();
 }
 
 hipError_t launch_batch_step(int storage, int lb, const Member *mem, int nmem, uint32_t total_tiles, int it, int grid,

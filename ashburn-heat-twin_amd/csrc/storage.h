@@ -136,6 +136,18 @@ struct StoreF32B : StoreF32 {
   static constexpr int QB = 26;   // populations per block: 19 D3Q19, then 7 D3Q7
 };
 
+// the 16-bit formats in the same interleaved layout, in blocks of 256 cells: [cell >> 8][26][cell & 255] of 2-byte
+// deviations, with the f32 theta plane [cell] after the array (the f16 step runs 4.9 % faster than in [q][cell] at
+// 600 x 600 x 64)
+struct StoreF16B : StoreF16 {
+  static constexpr int LB = 8;
+  static constexpr int QB = 26;
+};
+struct StoreBF16B : StoreBF16 {
+  static constexpr int LB = 8;
+  static constexpr int QB = 26;
+};
+
 // element index of population q (of Q) at cell n in a lattice of N cells: [q][cell] when S::LB == 0, else
 // [cell >> LB][S::QB][cell & (2^LB - 1)] from the lattice's base (the cell count rounded up to a whole block,
 // lattice_cells)

@@ -67,14 +67,16 @@ def test_batch_matches_alone_bitwise(tmp_path, pkg):
             s.close()
 
 
-def test_qcell_layout_matches_blocked_layout(tmp_path, pkg):
-    """f32 lattices below 4 GiB are blocked on the device unless the reference's [q][cell] layout is asked for (the
-    layout of larger f32 lattices): both give the same bits, state and FP64 means alike, through the whole step,
-    read-back and state-exchange path"""
+@pytest.mark.parametrize('storage', [0, 1, 2])
+def test_qcell_layout_matches_blocked_layout(tmp_path, pkg, storage):
+    """Lattice pairs below 4 GiB are blocked on the device (f32 in 2048-cell blocks, f16 / bf16 in 256-cell blocks with
+    the theta plane after the array) unless the reference's [q][cell] layout is asked for (the layout of larger
+    pairs): both give the same bits, state and FP64 means alike, through the whole step, read-back and state-exchange
+    path"""
     dom = domains(tmp_path)[0]
     out = []
     for layout in (0, 1):
-        with pkg.Solver.from_dump(dom, layout=layout) as s:
+        with pkg.Solver.from_dump(dom, storage=storage, layout=layout) as s:
             s.step(25)
             s.mean_reset()
             s.step(30, acc=True)
@@ -92,13 +94,14 @@ def test_qcell_layout_matches_blocked_layout(tmp_path, pkg):
     assert a[3] == q[3] and a[7] == q[7]
 
 
-def test_reset_heat_zeroes_theta_populations_only(tmp_path, pkg):
-    """aht_reset_heat sets every D3Q7 population to zero and leaves the D3Q19 populations as they were, in the blocked
-    f32 layout (one array per lattice pair, the D3Q7 populations of each block after its D3Q19 ones; this 5,760-cell
-    domain also has a partly filled last block) as in [q][cell]."""
+@pytest.mark.parametrize('storage', [0, 1, 2])
+def test_reset_heat_zeroes_theta_populations_only(tmp_path, pkg, storage):
+    """aht_reset_heat sets every D3Q7 population (and the theta plane of the 16-bit formats) to zero and leaves the
+    D3Q19 populations as they were, in the blocked layout (one array per lattice pair, the D3Q7 populations of each
+    block after its D3Q19 ones; this 5,760-cell domain also has a partly filled last block) as in [q][cell]."""
     dom = domains(tmp_path)[0]
     for layout in (0, 1):
-        with pkg.Solver.from_dump(dom, layout=layout) as s:
+        with pkg.Solver.from_dump(dom, storage=storage, layout=layout) as s:
             s.step(25)
             f0, g0 = s.read_state()
             assert np.abs(g0).max() > 0
