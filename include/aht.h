@@ -7,8 +7,8 @@
  *
  * Host arrays (as the reference, js/live/kernel.js:17): populations [q][cell], cell = (k*ny + j)*nx + i, j = 0 at the
  * south edge; per-column arrays [j*nx + i] in that solver orientation; surface outputs north-up [row*nx + col], row 0 =
- * north. (On the device, f32 lattice pairs below 4 GiB are blocked by 2048 cells; aht_read_state / aht_write_state
- * convert.)
+ * north. (On the device, lattice pairs below 4 GiB are blocked, f32 by 2048 cells, f16/bf16 by 256; aht_read_state /
+ * aht_write_state convert.)
  *
  * Threading: one host thread per solver at a time. Work is enqueued on the solver's own HIP stream; aht_step returns
  * once enqueued, every read returns after the data is on the host.
@@ -54,9 +54,9 @@ typedef struct {
   int32_t reserved;
 } aht_dims;
 
-/* population layout on the device (the state exchange is [q][cell] either way): AUTO blocks f32 lattice pairs below
- * 4 GiB by 2048 cells (one array [cell >> 11][19 D3Q19 + 7 D3Q7][cell & 2047] per pair, faster), QCELL keeps the
- * reference's [q][cell] for every format */
+/* population layout on the device (the state exchange is [q][cell] either way): AUTO blocks f32 / f16 / bf16 lattice
+ * pairs below 4 GiB (one array [cell >> B][19 D3Q19 + 7 D3Q7][cell % 2^B] per pair, B = 11 for f32, 8 for the 16-bit
+ * formats with their f32 theta plane after it; faster), QCELL keeps the reference's [q][cell] for every format */
 #define AHT_LAYOUT_AUTO 0
 #define AHT_LAYOUT_QCELL 1
 
