@@ -22,8 +22,8 @@ def domains(tmp_path):
     return out
 
 
-def alone(pkg, dom, steps, acc_from):
-    with pkg.Solver.from_dump(dom) as s:
+def alone(pkg, dom, steps, acc_from, storage=0):
+    with pkg.Solver.from_dump(dom, storage=storage) as s:
         s.step(acc_from)
         s.mean_reset()
         s.step(steps - acc_from, acc=True)
@@ -32,14 +32,17 @@ def alone(pkg, dom, steps, acc_from):
     return f, g, s3, b
 
 
-def test_batch_matches_alone_bitwise(tmp_path, pkg):
+@pytest.mark.parametrize('storage', [0, 1, 2])
+def test_batch_matches_alone_bitwise(tmp_path, pkg, storage):
+    """batched members (stream bases formed on the device, blocked layout of every format) give the bits of the
+    plain step (bases from the host-built table)"""
     doms = domains(tmp_path)
     steps = [37, 52, 45]
     acc_from = [20, 30, 0]
-    ref = [alone(pkg, d, n, a) for d, n, a in zip(doms, steps, acc_from)]
+    ref = [alone(pkg, d, n, a, storage) for d, n, a in zip(doms, steps, acc_from)]
     for use_graph, grid in ((True, 0), (False, 64), (True, 3)):
-        sol = [pkg.Solver.from_dump(d) for d in doms]
-        with pkg.Batch() as b:
+        sol = [pkg.Solver.from_dump(d, storage=storage) for d in doms]
+        with pkg.Batch(storage=storage) as b:
             for s in sol:
                 s.mean_reset()
                 b.add(s)
