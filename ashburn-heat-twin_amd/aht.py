@@ -388,3 +388,13 @@ def budget_physical(b, dx, dt, conv):
         heat_content_MJ=b['content'] * RHO_CP * dx ** 3 / 1e6,
         mass_imbalance_frac=b['mass_imbalance'],
     )
+
+
+def stats_physical(d, dx, conv):
+    """aht_stats diagnostics -> the units of the reference's _dbg.stats() (js/live/solver.js:559-576):
+    heat content in MJ (sum theta * rho cp dx^3), mean kinetic energy per fluid cell in m2/s2 (sum 0.5|j|^2/rho over
+    the fluid count, times conv^2), max theta in K, max |u| in m/s"""
+    nf = max(d['n_fluid'], 1.0)
+    return dict(heat_content_MJ=d['sum_theta'] * RHO_CP * dx ** 3 / 1e6, ke_mean=d['sum_ke'] / nf * conv * conv,
+                theta_max=d['theta_max'], umax=d['umax'] * conv, rho_dev_mean=d['sum_drho'] / nf,
+                n_fluid=d['n_fluid'], non_finite=d['non_finite'])

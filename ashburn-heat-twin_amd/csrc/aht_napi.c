@@ -5,6 +5,12 @@
  * step(n) resolves on queue completion, js/live/solver.js:401-406). Errors throw with aht_last_error's message.
  *
  * The solver handle is a JS external; the host serialises calls per solver (the C-ABI is one caller per solver).
+ *
+ * Lifetime: a step, benchmark or batch step runs on a libuv worker while the main thread goes on. destroy() of a
+ * solver (or batch) with such work in flight does not free it under the worker: the handle is marked dead at once
+ * (every later call on it throws) and the library object is destroyed by the completion callback of the last job
+ * that uses it, on the main thread. A solver that is a member of a batch counts the batch's in-flight steps as its
+ * own. Batch membership cannot change while a batch step is in flight.
  */
 #define NAPI_VERSION 4
 #include <node_api.h>
@@ -23,15 +29,68 @@
     }                                                                            \
   } while (0)
 
-typedef struct {
+typedef struct bhandle bhandle_t;
+
+typedef struct handle {
   aht_solver *s;
+  int inflight;        /* step / benchmark jobs queued or running on a worker                     */
+  int dead;            /* destroy() was called; the solver goes when no job uses it any more       */
+  int released;        /* the JS external was finalized (the struct is freed once nothing uses it) */
+  bhandle_t *batch;    /* the batch this solver belongs to (NULL: none)                            */
 } handle_t;
+
+struct bhandle {
+  aht_batch *b;
+  int inflight, dead, released;
+  handle_t **mem;      /* member handles, in add order */
+  int nmem, cap;
+};
+
+static int solver_busy(const handle_t *h) { return h->inflight > 0 || (h->batch && h->batch->inflight > 0); }
+
+static void batch_forget(bhandle_t *b, handle_t *h) {
+  for (int k = 0; k < b->nmem; k++)
+    if (b->mem[k] == h) {
+      memmove(b->mem + k, b->mem + k + 1, (size_t)(b->nmem - k - 1) * sizeof *b->mem);
+      b->nmem--;
+      break;
+    }
+  h->batch = NULL;
+}
+
+/* the library solver goes now (nothing uses it); frees the struct too once JS has let go of it */
+static void solver_release(handle_t *h) {
+  if (h->s) {
+    if (h->batch) batch_forget(h->batch, h);   /* aht_destroy leaves the library batch itself */
+    aht_destroy(h->s);
+    h->s = NULL;
+  }
+  if (h->released) free(h);
+}
+
+static void batch_release(bhandle_t *b) {
+  if (b->b) {
+    for (int k = 0; k < b->nmem; k++) b->mem[k]->batch = NULL;
+    b->nmem = 0;
+    aht_batch_destroy(b->b);
+    b->b = NULL;
+  }
+  if (b->released) {
+    free(b->mem);
+    free(b);
+  }
+}
+
+/* after a job finished: release what was destroyed while it ran */
+static void solver_settle(handle_t *h) {
+  if ((h->dead || h->released) && !solver_busy(h)) solver_release(h);
+}
 
 static void finalize_handle(napi_env env, void *data, void *hint) {
   (void)env; (void)hint;
   handle_t *h = (handle_t *)data;
-  if (h->s) aht_destroy(h->s);
-  free(h);
+  h->released = 1;
+  if (!solver_busy(h)) solver_release(h);   /* else the last job's completion releases it */
 }
 
 static napi_value throw_rc(napi_env env, aht_solver *s, int rc) {
@@ -59,7 +118,7 @@ static int get_args(napi_env env, napi_callback_info info, size_t want, napi_val
 
 static handle_t *get_handle(napi_env env, napi_value v) {
   void *p = NULL;
-  if (napi_get_value_external(env, v, &p) != napi_ok || !p || !((handle_t *)p)->s) {
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p || !((handle_t *)p)->s || ((handle_t *)p)->dead) {
     napi_throw_error(env, NULL, "aht addon: invalid or destroyed solver handle");
     return NULL;
   }
@@ -140,6 +199,11 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
     return r;
   }
   handle_t *h = (handle_t *)calloc(1, sizeof *h);
+  if (!h) {
+    aht_destroy(s);
+    napi_throw_error(env, NULL, "aht: out of host memory");
+    return NULL;
+  }
   h->s = s;
   napi_value ext;
   CHECK_NAPI(napi_create_external(env, h, finalize_handle, NULL, &ext));
@@ -152,7 +216,10 @@ static napi_value js_destroy(napi_env env, napi_callback_info info) {
   void *p = NULL;
   napi_get_value_external(env, argv[0], &p);
   handle_t *h = (handle_t *)p;
-  if (h && h->s) { aht_destroy(h->s); h->s = NULL; }
+  if (h && h->s && !h->dead) {
+    h->dead = 1;                                /* later calls throw from here on */
+    if (!solver_busy(h)) solver_release(h);     /* else the last job using it releases it */
+  }
   return undef(env);
 }
 
@@ -255,9 +322,12 @@ static void step_complete(napi_env env, napi_status status, void *data) {
     else napi_get_undefined(env, &v);
     napi_resolve_deferred(env, j->deferred, v);
   }
+  handle_t *h = j->h;
+  h->inflight--;
   napi_delete_reference(env, j->keep);
   napi_delete_async_work(env, j->work);
   free(j);
+  solver_settle(h);
 }
 
 static napi_value start_job(napi_env env, napi_value hv, handle_t *h, int32_t n, uint32_t flags, int bench) {
@@ -269,6 +339,7 @@ static napi_value start_job(napi_env env, napi_value hv, handle_t *h, int32_t n,
   CHECK_NAPI(napi_create_string_utf8(env, bench ? "aht_benchmark" : "aht_step", NAPI_AUTO_LENGTH, &name));
   CHECK_NAPI(napi_create_async_work(env, NULL, name, step_execute, step_complete, j, &j->work));
   CHECK_NAPI(napi_queue_async_work(env, j->work));
+  h->inflight++;
   return promise;
 }
 
@@ -280,6 +351,10 @@ static napi_value js_step(napi_env env, napi_callback_info info) {
   if (!h) return NULL;
   int32_t n, fl;
   if (!get_i32(env, argv[1], &n) || !get_i32(env, argv[2], &fl)) return NULL;
+  if (h->batch && h->batch->inflight) {
+    napi_throw_error(env, NULL, "aht: the solver's batch has a step in flight");
+    return NULL;
+  }
   return start_job(env, argv[0], h, n, (uint32_t)fl, 0);
 }
 
@@ -291,6 +366,10 @@ static napi_value js_benchmark(napi_env env, napi_callback_info info) {
   if (!h) return NULL;
   int32_t n;
   if (!get_i32(env, argv[1], &n)) return NULL;
+  if (h->batch && h->batch->inflight) {
+    napi_throw_error(env, NULL, "aht: the solver's batch has a step in flight");
+    return NULL;
+  }
   return start_job(env, argv[0], h, n, 0, 1);
 }
 
@@ -478,20 +557,16 @@ static napi_value js_steps_done(napi_env env, napi_callback_info info) {
 }
 
 /* ---- batches -------------------------------------------------------------------------------------------------- */
-typedef struct {
-  aht_batch *b;
-} bhandle_t;
-
 static void finalize_batch(napi_env env, void *data, void *hint) {
   (void)env; (void)hint;
-  bhandle_t *h = (bhandle_t *)data;
-  if (h->b) aht_batch_destroy(h->b);
-  free(h);
+  bhandle_t *b = (bhandle_t *)data;
+  b->released = 1;
+  if (!b->inflight) batch_release(b);
 }
 
 static bhandle_t *get_bhandle(napi_env env, napi_value v) {
   void *p = NULL;
-  if (napi_get_value_external(env, v, &p) != napi_ok || !p || !((bhandle_t *)p)->b) {
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p || !((bhandle_t *)p)->b || ((bhandle_t *)p)->dead) {
     napi_throw_error(env, NULL, "aht addon: invalid or destroyed batch handle");
     return NULL;
   }
@@ -519,6 +594,11 @@ static napi_value js_batch_create(napi_env env, napi_callback_info info) {
     return r;
   }
   bhandle_t *h = (bhandle_t *)calloc(1, sizeof *h);
+  if (!h) {
+    aht_batch_destroy(b);
+    napi_throw_error(env, NULL, "aht batch: out of host memory");
+    return NULL;
+  }
   h->b = b;
   napi_value ext;
   CHECK_NAPI(napi_create_external(env, h, finalize_batch, NULL, &ext));
@@ -530,8 +610,11 @@ static napi_value js_batch_destroy(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 1, argv)) return NULL;
   void *p = NULL;
   napi_get_value_external(env, argv[0], &p);
-  bhandle_t *h = (bhandle_t *)p;
-  if (h && h->b) { aht_batch_destroy(h->b); h->b = NULL; }
+  bhandle_t *b = (bhandle_t *)p;
+  if (b && b->b && !b->dead) {
+    b->dead = 1;
+    if (!b->inflight) batch_release(b);
+  }
   return undef(env);
 }
 
@@ -543,8 +626,30 @@ static napi_value js_batch_member(napi_env env, napi_callback_info info, int add
   if (!b) return NULL;
   handle_t *h = get_handle(env, argv[1]);
   if (!h) return NULL;
+  if (b->inflight || h->inflight) {
+    napi_throw_error(env, NULL, "aht batch: membership cannot change while a step of the batch or the solver is in flight");
+    return NULL;
+  }
   int rc = add ? aht_batch_add(b->b, h->s) : aht_batch_remove(b->b, h->s);
-  return rc ? throw_brc(env, b->b, rc) : undef(env);
+  if (rc) return throw_brc(env, b->b, rc);
+  if (add) {
+    if (b->nmem == b->cap) {
+      const int cap = b->cap ? 2 * b->cap : 16;
+      handle_t **m = (handle_t **)realloc(b->mem, (size_t)cap * sizeof *m);
+      if (!m) {
+        (void)aht_batch_remove(b->b, h->s);
+        napi_throw_error(env, NULL, "aht batch: out of host memory");
+        return NULL;
+      }
+      b->mem = m;
+      b->cap = cap;
+    }
+    b->mem[b->nmem++] = h;
+    h->batch = b;
+  } else {
+    batch_forget(b, h);
+  }
+  return undef(env);
 }
 static napi_value js_batch_add(napi_env env, napi_callback_info info) { return js_batch_member(env, info, 1); }
 static napi_value js_batch_remove(napi_env env, napi_callback_info info) { return js_batch_member(env, info, 0); }
@@ -602,11 +707,21 @@ static void bstep_complete(napi_env env, napi_status status, void *data) {
     napi_create_double(env, j->ms, &v);
     napi_resolve_deferred(env, j->deferred, v);
   }
+  bhandle_t *b = j->b;
+  b->inflight--;
   napi_delete_reference(env, j->keep);
   napi_delete_async_work(env, j->work);
   free(j->ns);
   free(j->af);
   free(j);
+  if (!b->inflight) {
+    /* members destroyed while the step ran go now (solver_release edits the member list: walk it backwards) */
+    for (int k = b->nmem - 1; k >= 0; k--) {
+      handle_t *h = b->mem[k];
+      if ((h->dead || h->released) && !h->inflight) solver_release(h);
+    }
+    if (b->dead || b->released) batch_release(b);
+  }
 }
 
 /* batchStep(b, n, nsteps Int32Array|null, accFrom Int32Array|null) -> Promise<ms> (the arrays are copied) */
@@ -621,6 +736,11 @@ static napi_value js_batch_step(napi_env env, napi_callback_info info) {
   size_t bn, ba;
   if (!get_buf(env, argv[2], &pn, &bn) || !get_buf(env, argv[3], &pa, &ba)) return NULL;
   const size_t m = (size_t)aht_batch_size(b->b);
+  for (int k = 0; k < b->nmem; k++)
+    if (b->mem[k]->inflight) {
+      napi_throw_error(env, NULL, "aht batch: a member solver has its own step in flight");
+      return NULL;
+    }
   if ((pn && bn < m * 4) || (pa && ba < m * 4)) {
     napi_throw_range_error(env, NULL, "aht batch: nsteps/accFrom need one entry per member");
     return NULL;
@@ -635,6 +755,7 @@ static napi_value js_batch_step(napi_env env, napi_callback_info info) {
   CHECK_NAPI(napi_create_string_utf8(env, "aht_batch_step", NAPI_AUTO_LENGTH, &name));
   CHECK_NAPI(napi_create_async_work(env, NULL, name, bstep_execute, bstep_complete, j, &j->work));
   CHECK_NAPI(napi_queue_async_work(env, j->work));
+  b->inflight++;
   return promise;
 }
 

@@ -27,11 +27,11 @@ function loadAddon() {
 const STEP_ACC = 1, STEP_OUT = 2;
 const STORAGE = { f32: 0, f16: 1, bf16: 2 };
 
-/** createLiveSolver's counterpart (js/live/solver.js:44-101). opts: dx, u10Max, heightM, base (static10 directory),
+/** createLiveSolver (js/live/solver.js:44-101). opts: dx, u10Max, heightM, base (static10 directory),
  *  synthetic (generator spec), nz, alpha, nsp, nsw, outflowClip, existingFlux, met, sideInflowDeg, metReinit, dirTol,
  *  allowAnyDx, onProgress, device (HIP ordinal), storage ('f32' | 'f16' | 'bf16'), block, autotune (tuning steps
  *  per launch-shape candidate, run once after the flow is initialised; 0 or absent = default shape) */
-async function createSolver(opts) {
+async function createLiveSolver(opts) {
   const o = Object.assign({ dx: 30, u10Max: 8, heightM: 720, onProgress: () => {}, device: 0 }, opts || {});
   const prog = (t, f) => { try { o.onProgress(t, f); } catch (e) { /* UI errors ignored */ } };
   const A = loadAddon();
@@ -81,8 +81,11 @@ class Solver {
     if (this._destroyed) throw new Error('live solver: destroyed');
   }
 
+  // device work in call order; a call made after destroy() rejects, work queued before it still runs (destroy waits
+  // for it, as the reference's queued GPU work outlives buffer.destroy())
   _enqueue(fn) {
-    const p = this._q.then(() => { this._check(); return fn(); });
+    if (this._destroyed) return Promise.reject(new Error('live solver: destroyed'));
+    const p = this._q.then(() => fn());
     this._q = p.catch(() => {});
     return p;
   }
@@ -212,11 +215,13 @@ class Solver {
     });
   }
 
+  /** later calls reject at once; the device buffers go when the work queued before this call has finished
+   *  (js/live/solver.js:485-493; the caller js/live.js:44 destroys a solver whose step may still be in flight) */
   destroy() {
     if (this._destroyed) return;
     this._destroyed = true;
-    if (this._h) this._A.destroy(this._h);
-    this._h = null;
+    const h = this._h, A = this._A;
+    if (h) this._q = this._q.then(() => { A.destroy(h); this._h = null; });
   }
 
   /** validation hooks (js/live/solver.js:496-577) */
@@ -273,4 +278,8 @@ class Solver {
   }
 }
 
-module.exports = { createSolver, Solver, STEP_ACC, STEP_OUT, loadAddon };
+// the reference's entry point (js/live/solver.js:44; its caller js/live.js:42 imports it by this name), also under a
+// shorter alias
+const createSolver = createLiveSolver;
+
+module.exports = { createLiveSolver, createSolver, Solver, STEP_ACC, STEP_OUT, loadAddon };

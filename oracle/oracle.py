@@ -5,8 +5,7 @@ the timed CPU baseline. The product (libaht.so and its hosts) never imports it a
 
 Wraps oracle/lib/liblbm_oracle.so (a C restatement of the reference kernel generator js/live/kernel.js; see
 lbm_oracle.h) and the domain-dump format written by ashburn-heat-twin_amd/js/dump.js. Also restates the
-reference's comparison metrics of tools/live/validate_live.py (rel_l2 :146-149, the init closed form :132-143,
-surface pick :127-129, heat_diag :226-240, surf_stats :216-217).
+reference's comparison metric rel_l2 (tools/live/validate_live.py:146-149) and the init closed form (:132-143).
 """
 import ctypes
 import json
@@ -197,25 +196,3 @@ def init_closed_form(dom):
         f[q] = np.float32(W19[q]) * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * usq)
     return f.reshape(19, -1)
 
-
-def surf_stats(th):
-    return dict(max=float(np.nanmax(th)), mean=float(np.nanmean(th)), p99=float(np.nanpercentile(th, 99)))
-
-
-def heat_diag(mean4, fluid, dx, q, dt):
-    """budgets of time-mean fields (validate_live.py:226-240); mean4 = (theta, u, v, w) [nz, ny, nx], u/v in m/s"""
-    th, u, v, _ = mean4
-
-    def face(arr, sl):
-        return float((arr[sl] * fluid[sl]).sum() * dx * dx)
-
-    hw = face(th * u, (slice(None), slice(None), 0)); he = face(th * u, (slice(None), slice(None), -1))
-    hs = face(th * v, (slice(None), 0, slice(None))); hn = face(th * v, (slice(None), -1, slice(None)))
-    out_W = RHO_CP * (he - hw + hn - hs)
-    in_W = float(np.asarray(q, np.float64).sum() * RHO_CP * dx ** 3 / dt)
-    content_MJ = float(np.where(fluid, th, 0).sum() * RHO_CP * dx ** 3 / 1e6)
-    qw = face(u, (slice(None), slice(None), 0)); qe = face(u, (slice(None), slice(None), -1))
-    qs = face(v, (slice(None), 0, slice(None))); qn = face(v, (slice(None), -1, slice(None)))
-    thru = abs(qw) + abs(qe) + abs(qs) + abs(qn)
-    return dict(heat_injected_MW=in_W / 1e6, heat_advected_out_MW=out_W / 1e6, heat_content_MJ=content_MJ,
-                mass_imbalance_frac=abs(qw - qe + qs - qn) / max(thru, 1e-9))

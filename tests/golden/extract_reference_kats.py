@@ -42,7 +42,12 @@ def main():
             'heat_budget_webgpu': lr['heat_budget_webgpu'], 'heat_budget_numba': lr['heat_budget_numba'],
             'speed_mean': {'webgpu': lr['surface_mean_wind']['speed_mean_webgpu'],
                            'numba': lr['surface_mean_wind']['speed_mean_numba']},
+            # the WebGPU run's last _dbg.stats() sample (step nspin + navg): heat content MJ, max theta, keMean
+            'final': d[tag]['result']['final'],
         }
+    # the stability matrix (tests/live/validate.html:162-198): per dx and met/campus combination, read() + stats()
+    # every 1,000 steps of a 10,000-step run
+    out['stab'] = [{k: r[k] for k in ('dx', 'name', 'ok', 'checks', 'dt')} for r in d['stab']['result']['results']]
     api = d['api20']['result']
     out['api20'] = {k: api[k] for k in ('badDx', 'badU10Max', 'readNoVolNoWind', 'concurrentReads', 'campuses',
                                         'heatExistingMW800', 'afterResetHeat', 'afterResetFlow', 'flowReinits',

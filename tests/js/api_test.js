@@ -4,7 +4,7 @@
 // resets, flow re-initialisation on a large turn, use-after-destroy. Prints one JSON object.
 // Usage: node tests/js/api_test.js [--cpu]   (--cpu: only the checks that need no GPU)
 const path = require('path');
-const { createSolver } = require(path.join(__dirname, '..', '..', 'ashburn-heat-twin_amd', 'js', 'solver.js'));
+const { createLiveSolver: createSolver, loadAddon } = require(path.join(__dirname, '..', '..', 'ashburn-heat-twin_amd', 'js', 'solver.js'));
 const { CAMPUSES, MET_AUG25 } = require(path.join(__dirname, '..', '..', 'ashburn-heat-twin_amd', 'js', 'dump.js'));
 
 const expectReject = async p => { try { await p; return 'resolved (unexpected)'; } catch (e) { return 'rejected: ' + e.message; } };
@@ -68,6 +68,23 @@ async function main() {
   s.destroy();
   out.afterDestroy = await expectReject(Promise.resolve().then(() => s.step(1)));
   out.readAfterDestroy = await expectReject(Promise.resolve().then(() => s.read()));
+  // destroy with a step in flight, as the reference's caller does (js/live.js:44, :76): the step issued before
+  // destroy() settles, later calls reject
+  const s2 = await createSolver({ dx: 40, met: MET_AUG25 });
+  const inflight = s2.step(200);
+  s2.destroy();
+  out.stepBeforeDestroy = await inflight.then(() => 'resolved', e => 'rejected: ' + e.message);
+  out.stepAfterDestroy = await expectReject(s2.step(1));
+  // the same on the addon directly: destroy() while the step job runs on a worker is deferred to its completion
+  const A = loadAddon(), sc = s2._sc, c = sc.col;
+  const h = A.create({ nx: sc.nx, ny: sc.ny, nz: sc.nz, storage: 0, device: 0 });
+  A.setColumns(h, c.kt, c.ktop, c.cd, c.q);
+  A.setParams(h, sc.params, sc.profBlock());
+  A.initFlow(h);
+  const job = A.step(h, 300, 0);
+  A.destroy(h);
+  out.addonStepBeforeDestroy = await job.then(() => 'resolved', e => 'rejected: ' + e.message);
+  try { A.step(h, 1, 0); out.addonStepAfterDestroy = 'no error (unexpected)'; } catch (e) { out.addonStepAfterDestroy = 'threw: ' + e.message; }
   console.log(JSON.stringify(out));
 }
 

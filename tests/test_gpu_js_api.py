@@ -36,6 +36,12 @@ def test_api_sequence_matches_reference_record():
     assert out['flowReinits'] == K['flowReinits']
     assert out['afterDestroy'] == K['afterDestroy']
     assert out['readAfterDestroy'] == 'rejected: live solver: destroyed'
+    # destroy() with a step in flight (the reference caller's pattern, js/live.js:44): that step settles, later
+    # calls reject; at the addon the library solver is freed only when the worker's step has completed
+    assert out['stepBeforeDestroy'] == 'resolved'
+    assert out['stepAfterDestroy'] == 'rejected: live solver: destroyed'
+    assert out['addonStepBeforeDestroy'] == 'resolved'
+    assert out['addonStepAfterDestroy'] == 'threw: aht addon: invalid or destroyed solver handle'
     assert out['mlups'] > 1000
     # the launch-shape autotuner (no counterpart in the reference) leaves the simulation where it was
     assert out['autotune']['block'] in (64, 128, 256) and out['autotune']['levels'] in (1, 2, 4)

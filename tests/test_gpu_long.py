@@ -5,7 +5,10 @@ u* 0.32, w* 1.67, blh 903 m, 400 W/m2), 2200 s spin-up then a 600 s mean accumul
 Compared with the recorded WebGPU and numba results (tests/golden/reference_kats.json):
   * 10-minute mean near-ground theta: relative error of the mean <= 1e-4 and the same peak (<= 1e-4 relative);
   * heat-budget closure (advected out / injected) within 0.5 percentage points; heat content within 1e-4;
-  * mass imbalance within 5e-5 absolute.
+  * mass imbalance within 5e-5 absolute;
+  * the diagnostics of the final state (the run's last _dbg.stats() sample at step nspin + navg,
+    js/live/solver.js:559-576): heat content in MJ and max theta within 1e-4 relative, mean kinetic energy per fluid
+    cell (keMean = sum 0.5|j|^2/rho / n_fluid * conv^2, m2/s2) within 1e-3 relative, the fluid count exactly.
 L30 runs the prototype outlet, L30c the backflow-clipped outlet; both use the default outflow-sponge width 5.
 """
 import numpy as np
@@ -28,6 +31,14 @@ def test_ten_minute_mean_and_budgets(tmp_path, pkg, tag):
         s3, b = s.read_mean()
         st = s.stats()
     assert st['non_finite'] == 0
+    fin = K['final']
+    assert fin['step'] == K['nspin'] + K['navg'] and fin['nonFinite'] == 0
+    ph = pkg.stats_physical(st, dom['dx'], dom['conv'])
+    print(tag, 'final stats', ph, 'record', fin)
+    assert ph['n_fluid'] == dom['info']['nFluid']
+    assert abs(ph['heat_content_MJ'] - fin['heatMJ']) <= 1e-4 * fin['heatMJ']
+    assert abs(ph['theta_max'] - fin['thetaMax']) <= 1e-4 * fin['thetaMax']
+    assert abs(ph['ke_mean'] - fin['ke']) <= 1e-3 * fin['ke']
     th = s3[0]
     want = K['surface_mean_theta']['webgpu']
     print(tag, 'surface mean theta max %.7f mean %.8f p99 %.6f' % (th.max(), th.mean(), np.percentile(th, 99)),

@@ -6,9 +6,9 @@ These run on the CPU only. They pin the checker the GPU parity tests rely on:
   * Ashburn domain facts (zbase, kt range, ktop max, building columns, nz, fluid cells; js/live/domain.js:166-190);
   * the initial state, bit-identical to the numpy closed form (tools/live/validate_live.py:132-143);
   * the p30a run: 300 steps from a cold start on the Ashburn 30 m grid, read() surface max / mean / wind mean
-    (the f32 oracle against the WebGPU record, the f64-arithmetic oracle against the numba record).
+    (the f32 oracle against the WebGPU record, the f64-arithmetic oracle against the numba record);
+  * the p30b run: campus A, backflow-clipped outlets, 2,000 + 300 steps, against the WebGPU record.
 """
-import os
 
 import numpy as np
 import pytest
@@ -81,10 +81,9 @@ def test_p30a_f64_oracle_matches_numba_record(ashburn30, O):
 
 
 @pytest.mark.slow
-@pytest.mark.skipif(not os.environ.get('AHT_SLOW'), reason='2,300 oracle steps on the 30 m grid: ~4 min on 8 cores '
-                                                          '(set AHT_SLOW=1); the GPU runs the same case every time')
 def test_p30b_f32_oracle_matches_webgpu_record(tmp_path, O):
-    """p30b: campus A, backflow-clipped outlets, outflow sponge 5, 2,000 + 300 steps (js/live/README.md:284)"""
+    """p30b: campus A, backflow-clipped outlets, outflow sponge 5, 2,000 + 300 steps (js/live/README.md:284); pins
+    the oracle's clipped-outlet branch and rooftop sources directly (about 3 minutes on 8 cores)"""
     K = load_summary()['short']['p30b']
     dom = dump_domain(tmp_path, '--dx', 30, '--nsw', 5, '--oclip', 1, '--campus', 'A')
     o = O.Oracle(dom, 'f32')
