@@ -107,6 +107,8 @@ SIGNATURES = {
     'aht_batch_sync': (ctypes.c_int, [_vp]),
     'aht_batch_set_grid': (ctypes.c_int, [_vp, _i32, _i32]),
     'aht_batch_grid': (ctypes.c_int, [_vp]),
+    'aht_batch_set_block': (ctypes.c_int, [_vp, _i32]),
+    'aht_batch_block': (ctypes.c_int, [_vp]),
     'aht_autotune': (ctypes.c_int, [_vp, _i32, ctypes.POINTER(Launch)]),
     'aht_set_launch': (ctypes.c_int, [_vp, ctypes.POINTER(Launch)]),
     'aht_get_launch': (ctypes.c_int, [_vp, ctypes.POINTER(Launch)]),
@@ -360,6 +362,14 @@ class Batch:
     @property
     def grid(self):
         return int(lib().aht_batch_grid(self.h))
+
+    def set_block(self, block):
+        """threads per block of the batched step (64, 128, 256; 0 = tuned online)"""
+        self._check(lib().aht_batch_set_block(self.h, int(block)))
+
+    @property
+    def block(self):
+        return int(lib().aht_batch_block(self.h))
 
     def close(self):
         if self.h:

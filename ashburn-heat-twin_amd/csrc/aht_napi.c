@@ -675,6 +675,27 @@ static napi_value js_batch_grid(napi_env env, napi_callback_info info) {
   return r;
 }
 
+/* batchSetBlock(b, threads: 0 | 64 | 128 | 256), batchBlock(b) -> threads (0 while tuning) */
+static napi_value js_batch_set_block(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  bhandle_t *b = get_bhandle(env, argv[0]);
+  if (!b) return NULL;
+  int32_t bl;
+  if (!get_i32(env, argv[1], &bl)) return NULL;
+  int rc = aht_batch_set_block(b->b, bl);
+  return rc ? throw_brc(env, b->b, rc) : undef(env);
+}
+static napi_value js_batch_block(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  bhandle_t *b = get_bhandle(env, argv[0]);
+  if (!b) return NULL;
+  napi_value r;
+  napi_create_int32(env, aht_batch_block(b->b), &r);
+  return r;
+}
+
 typedef struct {
   napi_async_work work;
   napi_deferred deferred;
@@ -770,7 +791,7 @@ static napi_value init(napi_env env, napi_value exports) {
     {"autotune", js_autotune}, {"setLaunch", js_set_launch}, {"getLaunch", js_get_launch},
     {"batchCreate", js_batch_create}, {"batchDestroy", js_batch_destroy}, {"batchAdd", js_batch_add},
     {"batchRemove", js_batch_remove}, {"batchSetGrid", js_batch_set_grid}, {"batchGrid", js_batch_grid},
-    {"batchStep", js_batch_step},
+    {"batchStep", js_batch_step}, {"batchSetBlock", js_batch_set_block}, {"batchBlock", js_batch_block},
   };
   for (size_t k = 0; k < sizeof fns / sizeof fns[0]; k++) {
     napi_value f;

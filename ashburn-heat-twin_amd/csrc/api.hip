@@ -109,14 +109,8 @@ int fail(aht_solver *s, int code, const char *fmt, ...) {
 
 size_t face_total(int nx, int ny, int nz) { return aht::face_offset(4, nx, ny, nz); }
 
-// wavefront slots of the step grid, over every block size a launch shape may pick (64, 128, 192, 256)
-uint32_t max_content_slots(const aht_dims &d) {
-  const uint32_t nxy = (uint32_t)d.nx * (uint32_t)d.ny;
-  uint32_t m = 0;
-  for (uint32_t block = 64; block <= 256; block += 64)
-    m = std::max(m, ((nxy + block - 1) / block) * (uint32_t)d.nz * (block / 64));
-  return m;
-}
+// heat-content slots (one per level and group of 64 columns, whatever the launch shape)
+uint32_t max_content_slots(const aht_dims &d) { return aht::content_slots((uint32_t)d.nx * (uint32_t)d.ny, d.nz); }
 
 int check_ready(aht_solver *s) {
   if (!s) return AHT_EINVAL;
@@ -626,22 +620,27 @@ struct aht_batch {
   int device = 0, storage = 0;
   hipStream_t st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  std::vector<aht_solver *> mem;
-  aht::Member *dmem = nullptr;        // device table
+  std::vector<aht_solver *> mem;       // add order
+  aht::Member *dmem = nullptr;         // device table, ordered by the chunk's step counts (descending)
   size_t dcap = 0;
   std::vector<aht::Member> hmem;
-  uint32_t total = 0;
-  int version = 0;                    // bumps on membership change
-  int grid = 0;                       // persistent blocks; 0 = tune online
+  std::vector<int> order;              // table row -> member (add order)
+  uint32_t total = 0;                  // tiles of every member, at the block size of the last chunk
+  int version = 0;                     // bumps on membership change
+  int grid = 0;                        // fixed grid (blocks), 0 = tuned
+  int block = 0;                       // fixed block size (threads), 0 = tuned
   bool use_graph = true;
+  // the captured chunk: valid while the block size and every launch's (members, tiles, grid) are unchanged (a change
+  // of a member's columns or levels changes its tile count, and with it the launches)
   hipGraphExec_t gexec = nullptr;
-  int g_version = -1, g_n = -1, g_grid = -1;
-  // online tuner: candidate grids timed on real chunks (device time per tile-step, so chunks with different members
-  // compare), tuned again only when the tile count moves by more than 2x from where the last tuning started
-  std::vector<int> cand;
+  int g_block = -1, g_grid = -1;
+  std::vector<uint64_t> g_key;
+  // online tuner: (block, grid) candidates timed on real chunks as device time per cell-step (chunks with different
+  // members compare), tuned again when the chunk's cell count moves by more than 2x from where the last tuning started
+  std::vector<std::pair<int, int>> cand;
   std::vector<double> cand_ms;
   int tune_next = 0;
-  uint32_t tune_total = 0;
+  double tune_cells = 0.0;
   int64_t steps = 0;
   std::string err;
 };
@@ -670,6 +669,37 @@ int cu_count(int device) {
   if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) n = 256;
   return n;
 }
+
+constexpr int ONE_BLOCK_PER_TILE = 1 << 30;
+
+// candidates of the batch tuner: fixed values or, when tuned, block sizes 256 / 128 / 64 and grids of one block per
+// tile or 32 persistent blocks per CU
+void reset_tuner(aht_batch *b) {
+  std::vector<int> blocks = b->block ? std::vector<int>{b->block} : std::vector<int>{256, 128, 64};
+  std::vector<int> grids = b->grid ? std::vector<int>{b->grid} : std::vector<int>{ONE_BLOCK_PER_TILE, 32 * cu_count(b->device)};
+  b->cand.clear();
+  for (int g : grids)
+    for (int bl : blocks) b->cand.emplace_back(bl, g);
+  b->cand_ms.assign(b->cand.size(), 0.0);
+  b->tune_next = 0;
+  b->tune_cells = 0.0;
+}
+
+size_t best_candidate(const aht_batch *b) {
+  size_t best = 0;
+  for (size_t k = 1; k < b->cand.size(); k++)
+    if (b->cand_ms[k] < b->cand_ms[best]) best = k;
+  return best;
+}
+
+// tiles of one step of solver s with blocks of `block` columns and its own levels per thread
+void member_tiles(const aht_solver *s, int block, aht::Member &M) {
+  const aht::Grid &g = s->g;
+  M.levels = s->levels;
+  M.bx = (g.NXY + (uint32_t)block - 1) / (uint32_t)block;
+  M.inv_bx = 1.0f / (float)M.bx;
+  M.tiles = g.kmin < g.nz ? M.bx * (uint32_t)((g.nz - g.kmin + M.levels - 1) / M.levels) : 0;
+}
 }  // namespace
 
 int aht_batch_create(int32_t device, int32_t storage, aht_batch **out) {
@@ -684,10 +714,7 @@ int aht_batch_create(int32_t device, int32_t storage, aht_batch **out) {
   HIP_TRYB(b, hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
   HIP_TRYB(b, hipEventCreate(&b->ev0));
   HIP_TRYB(b, hipEventCreate(&b->ev1));
-  const int cus = cu_count(device);
-  // persistent (8 or 32 blocks per CU, each looping over tiles) or one block per tile (clamped to the tile count)
-  b->cand = {8 * cus, 32 * cus, 1 << 30};
-  b->cand_ms.assign(b->cand.size(), 0.0);
+  reset_tuner(b);
   return AHT_OK;
 }
 
@@ -695,7 +722,10 @@ void aht_batch_destroy(aht_batch *b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   if (b->st) (void)hipStreamSynchronize(b->st);
-  for (aht_solver *s : b->mem) s->batch = nullptr;
+  for (aht_solver *s : b->mem) {
+    s->batch = nullptr;
+    s->batch_pending = false;
+  }
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->dmem) (void)hipFree(b->dmem);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -713,7 +743,6 @@ int aht_batch_add(aht_batch *b, aht_solver *s) {
     return bfail(b, AHT_EINVAL, "solver device/storage (%d/%d) differ from the batch's (%d/%d)", s->device, s->storage,
                  b->device, b->storage);
   if (!s->has_cols || !s->has_params) return bfail(b, AHT_ESTATE, "solver needs columns and parameters first");
-  if (s->block != 256) return bfail(b, AHT_EINVAL, "batch members use 256-thread blocks");
   if (!b->mem.empty() && b->mem[0]->g.lb != s->g.lb)
     return bfail(b, AHT_EINVAL, "batch members share one population layout (lattice pairs of 4 GiB and more use the "
                                 "[q][cell] layout, smaller ones the blocked one)");
@@ -745,6 +774,16 @@ int aht_batch_set_grid(aht_batch *b, int32_t blocks, int32_t use_graph) {
   if (!b || blocks < 0) return AHT_EINVAL;
   b->grid = blocks;
   b->use_graph = use_graph != 0;
+  reset_tuner(b);
+  return AHT_OK;
+}
+
+int aht_batch_set_block(aht_batch *b, int32_t block) {
+  if (!b) return AHT_EINVAL;
+  if (block != 0 && block != 64 && block != 128 && block != 256)
+    return bfail(b, AHT_EINVAL, "batch block must be 0 (tuned), 64, 128 or 256");
+  b->block = block;
+  reset_tuner(b);
   return AHT_OK;
 }
 
@@ -754,67 +793,91 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
   if (n == 0 || b->mem.empty()) return AHT_OK;
   HIP_TRYB(b, hipSetDevice(b->device));
   const size_t nm = b->mem.size();
-  // member table for this chunk
-  b->hmem.resize(nm);
-  uint32_t base = 0;
   for (size_t d = 0; d < nm; d++) {
-    aht_solver *s = b->mem[d];
     const int ns = nsteps ? nsteps[d] : n;
     if (ns < 0 || ns > n) return bfail(b, AHT_EINVAL, "member %zu: nsteps %d outside 0..%d", d, ns, n);
-    aht::Member &M = b->hmem[d];
+  }
+  // (block, grid): fixed, or the online tuner's next candidate / its pick
+  double cells = 0.0;                                 // cell-steps of this chunk (all cells of the fluid levels)
+  for (size_t d = 0; d < nm; d++) {
+    const aht::Grid &g = b->mem[d]->g;
+    cells += (double)g.NXY * (double)std::max(0, g.nz - g.kmin) * (double)(nsteps ? nsteps[d] : n);
+  }
+  int cand_idx = -1;
+  size_t pick;
+  if (b->cand.size() > 1) {
+    if (b->tune_cells == 0.0 || cells > 2 * b->tune_cells || 2 * cells < b->tune_cells) {
+      std::fill(b->cand_ms.begin(), b->cand_ms.end(), 0.0);
+      b->tune_next = 0;
+      b->tune_cells = std::max(cells, 1.0);
+    }
+    if (b->tune_next < (int)b->cand.size()) cand_idx = b->tune_next++;
+    pick = cand_idx >= 0 ? (size_t)cand_idx : best_candidate(b);
+  } else {
+    pick = 0;
+  }
+  const int block = b->cand[pick].first, grid = b->cand[pick].second;
+  // member table, longest-stepping members first: the members that step in launch `it` are then a prefix of it
+  b->order.resize(nm);
+  for (size_t d = 0; d < nm; d++) b->order[d] = (int)d;
+  std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) {
+    return (nsteps ? nsteps[x] : n) > (nsteps ? nsteps[y] : n);
+  });
+  b->hmem.resize(nm);
+  uint32_t base = 0;
+  for (size_t r = 0; r < nm; r++) {
+    const int d = b->order[r];
+    aht_solver *s = b->mem[d];
+    aht::Member &M = b->hmem[r];
     M.g = s->g;
     M.f[0] = s->f[0]; M.f[1] = s->f[1]; M.gq[0] = s->gq[0]; M.gq[1] = s->gq[1];
     M.acc = aht::Acc{s->acc_surf, s->acc_face, s->acc_content};
-    const uint32_t bx = (s->g.NXY + 255) / 256;
-    M.tiles = s->g.kmin < s->g.nz ? bx * (uint32_t)(s->g.nz - s->g.kmin) : 0;
+    member_tiles(s, block, M);
     M.tile_base = base;
     base += M.tiles;
     M.parity0 = s->parity;
-    M.nsteps = ns;
+    M.nsteps = nsteps ? nsteps[d] : n;
     M.acc_from = acc_from ? acc_from[d] : n;
-    M.pad = 0;
     // the batch stream waits for work the member enqueued on its own stream
     HIP_TRYB(b, hipEventRecord(s->own, s->st));
     HIP_TRYB(b, hipStreamWaitEvent(b->st, s->own, 0));
   }
   b->total = base;
+  // launch it covers the first m members (those with nsteps > it) and their tiles
+  std::vector<uint64_t> key((size_t)n);
+  std::vector<std::pair<int, uint32_t>> launches((size_t)n);
+  {
+    size_t m = nm;
+    for (int it = 0; it < n; it++) {
+      while (m > 0 && b->hmem[m - 1].nsteps <= it) m--;
+      const uint32_t tiles = m < nm ? b->hmem[m].tile_base : base;
+      launches[(size_t)it] = {(int)m, tiles};
+      key[(size_t)it] = ((uint64_t)m << 32) | (uint64_t)tiles;
+    }
+  }
   if (b->dcap < nm) {
+    HIP_TRYB(b, hipStreamSynchronize(b->st));        // a captured chunk may still read the old table
     if (b->dmem) HIP_TRYB(b, hipFree(b->dmem));
+    b->dmem = nullptr;
     b->dcap = nm + 16;
     HIP_TRYB(b, hipMalloc(&b->dmem, b->dcap * sizeof(aht::Member)));
-    b->version++;
+    if (b->gexec) HIP_TRYB(b, hipGraphExecDestroy(b->gexec));
+    b->gexec = nullptr;
   }
   HIP_TRYB(b, hipMemcpyAsync(b->dmem, b->hmem.data(), nm * sizeof(aht::Member), hipMemcpyHostToDevice, b->st));
-  // grid: fixed, or the online tuner's pick (each candidate computes the same result)
-  double work = 0.0;                                  // tile-steps of this chunk
-  for (size_t d = 0; d < nm; d++) work += (double)b->hmem[d].tiles * (double)b->hmem[d].nsteps;
-  int grid = b->grid;
-  int cand_idx = -1;
-  if (grid == 0) {
-    if (b->tune_total == 0 || b->total > 2 * b->tune_total || 2 * b->total < b->tune_total) {
-      b->tune_total = std::max<uint32_t>(b->total, 1);
-      b->tune_next = 0;
-      std::fill(b->cand_ms.begin(), b->cand_ms.end(), 0.0);
-    }
-    if (b->tune_next < (int)b->cand.size()) {
-      cand_idx = b->tune_next++;
-      grid = b->cand[cand_idx];
-    } else {
-      size_t best = 0;
-      for (size_t k = 1; k < b->cand.size(); k++)
-        if (b->cand_ms[k] < b->cand_ms[best]) best = k;
-      grid = b->cand[best];
-    }
-  }
+  const int lb = b->mem[0]->g.lb;
+  auto enqueue = [&](int it) {
+    const auto &L = launches[(size_t)it];
+    return aht::launch_batch_step(b->storage, lb, block, b->dmem, L.first, L.second, it, grid, b->st);
+  };
   if (b->use_graph) {
-    if (!b->gexec || b->g_version != b->version || b->g_n != n || b->g_grid != grid) {
+    if (!b->gexec || b->g_block != block || b->g_grid != grid || b->g_key != key) {
       if (b->gexec) HIP_TRYB(b, hipGraphExecDestroy(b->gexec));
       b->gexec = nullptr;
       hipGraph_t graph;
       HIP_TRYB(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
       for (int it = 0; it < n; it++) {
-        const hipError_t e = aht::launch_batch_step(b->storage, b->mem[0]->g.lb, b->dmem, (int)nm, b->total, it, grid,
-                                                    b->st);
+        const hipError_t e = enqueue(it);
         if (e != hipSuccess) {
           hipGraph_t dummy;
           (void)hipStreamEndCapture(b->st, &dummy);
@@ -824,23 +887,24 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
       HIP_TRYB(b, hipStreamEndCapture(b->st, &graph));
       HIP_TRYB(b, hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0));
       HIP_TRYB(b, hipGraphDestroy(graph));
-      b->g_version = b->version; b->g_n = n; b->g_grid = grid;
+      b->g_block = block;
+      b->g_grid = grid;
+      b->g_key = key;
     }
     HIP_TRYB(b, hipEventRecord(b->ev0, b->st));       // after the (host-side) capture: times the device work only
     HIP_TRYB(b, hipGraphLaunch(b->gexec, b->st));
   } else {
     HIP_TRYB(b, hipEventRecord(b->ev0, b->st));
-    for (int it = 0; it < n; it++)
-      HIP_TRYB(b, aht::launch_batch_step(b->storage, b->mem[0]->g.lb, b->dmem, (int)nm, b->total, it, grid, b->st));
+    for (int it = 0; it < n; it++) HIP_TRYB(b, enqueue(it));
   }
   HIP_TRYB(b, hipEventRecord(b->ev1, b->st));
   // member bookkeeping; their next own-stream work waits for this chunk
   for (size_t d = 0; d < nm; d++) {
     aht_solver *s = b->mem[d];
-    const aht::Member &M = b->hmem[d];
-    s->parity ^= (M.nsteps & 1);
-    s->steps += M.nsteps;
-    s->n_acc += std::max(0, M.nsteps - std::max(0, M.acc_from));
+    const int ns = nsteps ? nsteps[d] : n, af = acc_from ? acc_from[d] : n;
+    s->parity ^= (ns & 1);
+    s->steps += ns;
+    s->n_acc += std::max(0, ns - std::max(0, af));
     HIP_TRYB(b, hipEventRecord(s->after_batch, b->st));
     s->batch_pending = true;
   }
@@ -849,7 +913,7 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
     HIP_TRYB(b, hipEventSynchronize(b->ev1));
     float t = 0.0f;
     HIP_TRYB(b, hipEventElapsedTime(&t, b->ev0, b->ev1));
-    if (cand_idx >= 0) b->cand_ms[cand_idx] = t / std::max(work, 1.0);
+    if (cand_idx >= 0) b->cand_ms[cand_idx] = t / std::max(cells, 1.0);
     if (ms) *ms = t;
   }
   return AHT_OK;
@@ -864,15 +928,15 @@ int aht_batch_sync(aht_batch *b) {
 
 int aht_batch_grid(const aht_batch *b) {
   if (!b) return -1;
-  int g = b->grid;
-  if (!g) {
-    if (b->tune_next < (int)b->cand.size()) return 0;
-    size_t best = 0;
-    for (size_t k = 1; k < b->cand.size(); k++)
-      if (b->cand_ms[k] < b->cand_ms[best]) best = k;
-    g = b->cand[best];
-  }
-  return (int)std::min<uint32_t>((uint32_t)g, std::max<uint32_t>(b->total, 1));   // blocks actually launched
+  if (b->cand.size() > 1 && b->tune_next < (int)b->cand.size()) return 0;
+  const int g = b->cand[b->cand.size() > 1 ? best_candidate(b) : 0].second;
+  return (int)std::min<uint32_t>((uint32_t)g, std::max<uint32_t>(b->total, 1));   // blocks launched for all members
+}
+
+int aht_batch_block(const aht_batch *b) {
+  if (!b) return -1;
+  if (b->cand.size() > 1 && b->tune_next < (int)b->cand.size()) return 0;
+  return b->cand[b->cand.size() > 1 ? best_candidate(b) : 0].first;
 }
 
 // ---- launch-shape autotuner for one solver ------------------------------------------------------------------------

@@ -32,7 +32,7 @@ struct Grid {
 struct Acc {
   double *surf;     // [3][NXY]  theta, ux, uy at k = ktop
   double *face;     // W [2][nz*ny], E [2][nz*ny], S [2][nz*nx], N [2][nz*nx] (theta, normal velocity)
-  double *content;  // one slot per wavefront of the step grid: sum of theta over its fluid cells
+  double *content;  // [nz][ceil(NXY / 64)]: sum of theta over the fluid cells of one level of 64 columns
 };
 
 // Stream bases of the single-domain fast path, host-built once per lattice parity (the step that reads lattice pair A
@@ -47,23 +47,25 @@ struct Streams {
 constexpr int STEP_BLOCK = 256;
 constexpr uint32_t NUM_XCD = 8;   // MI355X: workgroups are dispatched round-robin over 8 XCDs, each with its own L2
 
-// one member of a batched launch (device-resident table, rebuilt by the host when the batch changes)
+// one member of a batched launch (device-resident table, uploaded by the host before every chunk)
 struct Member {
   Grid g;
   void *f[2];
   void *gq[2];
   Acc acc;
-  uint32_t tiles;      // (256-column block, level) tiles of one step
+  uint32_t tiles;      // (block of BS columns, chunk of `levels` levels) tiles of one step
   uint32_t tile_base;  // first tile in the concatenated list
+  uint32_t bx;         // column blocks per level chunk: ceil(NXY / BS)
+  float inv_bx;        // 1 / bx
+  int levels;          // cells per thread along z (the member's own launch shape)
   int parity0;         // lattice holding the state at launch index 0 of the chunk
-  int nsteps;          // launches of the chunk in which this member steps
+  int nsteps;          // launches of the chunk in which this member steps (the table is ordered by it, descending)
   int acc_from;        // first launch index that accumulates the FP64 means (>= nsteps: none)
-  int pad;
 };
 
 size_t face_offset(int face, int nx, int ny, int nz);   // offset of face 0..3 in Acc::face
 size_t face_len(int face, int nx, int ny, int nz);      // doubles per face (2 * nz * ny or 2 * nz * nx)
-uint32_t content_slots(const Grid &g, int block);       // wavefront slots of the step grid
+uint32_t content_slots(uint32_t nxy, int nz);           // heat-content slots: one per level and 64-column group
 
 int storage_bytes(int storage);                         // bytes per population element of an AHT_STORE_* format
 bool storage_theta_ref(int storage);                    // D3Q7 lattice = f32 theta + deviations (f16, bf16)
@@ -84,9 +86,10 @@ hipError_t launch_zero_g_blocked(int storage, const Grid &g, void *gq, hipStream
 hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const Streams *sd, const void *fA, void *fB,
                        const void *gA, void *gB, const Acc *acc, float *out4, int block, int levels, hipStream_t st);
 Streams make_streams(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB);
-// one step of every batch member (launch index `it` of a chunk) on a persistent grid of `grid` blocks of 256
-hipError_t launch_batch_step(int storage, int lb, const Member *mem, int nmem, uint32_t total_tiles, int it, int grid,
-                             hipStream_t st);
+// one step of the first nmem batch members (launch index `it` of a chunk; their tiles are the first total_tiles) on
+// a grid of min(grid, total_tiles) blocks of `block` (64, 128 or 256) threads
+hipError_t launch_batch_step(int storage, int lb, int block, const Member *mem, int nmem, uint32_t total_tiles, int it,
+                             int grid, hipStream_t st);
 hipError_t launch_init(int storage, const Grid &g, void *f0, void *f1, hipStream_t st);
 hipError_t launch_reinit(int storage, const Grid &g, const uint32_t *old_ktop, void *f, void *gq, hipStream_t st);
 hipError_t launch_surface(int storage, const Grid &g, const void *fA, const void *gA, float conv, float *out3,

@@ -753,18 +753,18 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *s
   return th_cell;
 }
 
-// heat content: one slot per wavefront, each updated by that wavefront only (deterministic, no atomics)
-__device__ __forceinline__ void content_add(const Acc &acc, uint32_t slot, double th) {
-  const double sum = wave_sum(th);
-  if ((threadIdx.x & 63) == 0) acc.content[slot] += sum;
+// Heat content: one FP64 slot per (level, group of 64 consecutive columns), i.e. per wavefront of any launch shape
+// (blocks are whole multiples of 64 columns), each slot updated by exactly one wavefront per step: deterministic, no
+// atomics, and the same slots whatever the block size or levels per thread, so the summed content does not depend on
+// the launch shape (a batch member gives the bits of the same domain stepped alone). k and c0 (the wavefront's first
+// column) are wave-uniform; a wavefront entirely beyond the last column or level writes nothing.
+__device__ __forceinline__ uint32_t content_slot(const Grid &g, int k, int c0) {
+  return (uint32_t)k * ((g.NXY + 63u) >> 6) + ((uint32_t)c0 >> 6);
 }
-
-template <class S, bool ACC, bool OUT>
-__device__ __forceinline__ void step_cell(const Grid &g, const void *fA, void *fB, const void *gA, void *gB,
-                                          const Acc &acc, float *__restrict__ out4, int c, int k, uint32_t slot) {
-  const uint32_t ck = ld_tab(g.colk, (size_t)(c < (int)g.NXY ? c : 0));
-  const double th = step_cell_body<S, ACC, OUT, false>(g, nullptr, fA, fB, gA, gB, acc, out4, c, k, ck);
-  if (ACC) content_add(acc, slot, th);
+__device__ __forceinline__ void content_add(const Grid &g, const Acc &acc, int k, int c0, double th) {
+  if (k >= g.nz || c0 >= (int)g.NXY) return;
+  const double sum = wave_sum(th);
+  if ((threadIdx.x & 63) == 0) acc.content[content_slot(g, k, c0)] += sum;
 }
 
 // one block per (column block, KB-level chunk) tile, numbered column block fastest, then level chunk, taken in the
@@ -778,27 +778,39 @@ __global__ __launch_bounds__(256) void step_kernel(const Grid *__restrict__ gp, 
   uint32_t cb;
   const uint32_t kk = div_by(t, bx, inv_bx, cb);
   const int c = (int)(cb * blockDim.x + threadIdx.x);
+  const int c0 = c - (int)(threadIdx.x & 63);
   const int k0 = g.kmin + (int)kk * KB;
   const uint32_t ck = ld_tab(g.colk, (size_t)(c < (int)g.NXY ? c : 0));
-  double th = 0.0;
-  for (int u = 0; u < KB; u++)
-    th += step_cell_body<S, ACC, OUT, true>(g, sp, fA, fB, gA, gB, acc, out4, c, k0 + u, ck);
-  if (ACC) content_add(acc, t * (blockDim.x >> 6) + (threadIdx.x >> 6), th);
+  for (int u = 0; u < KB; u++) {
+    const double th = step_cell_body<S, ACC, OUT, true>(g, sp, fA, fB, gA, gB, acc, out4, c, k0 + u, ck);
+    if (ACC) content_add(g, acc, k0 + u, c0, th);
+  }
 }
 
 // ---- many domains in one launch ----------------------------------------------------------------------------------
-// One grid of G blocks walks one step of every batch member: member d contributes tiles (256-column block, level) for
-// its fluid levels, concatenated in member order. The grid strides through the list in windows of G tiles, taking
-// the tiles of a window in the XCD-aware order of the single-domain step (runs of 64 tiles per XCD), so the blocks in
-// flight keep one compact front through memory instead of G separate ones. With one block per tile (G = total, the
-// serving default) block b takes tile b, the order the single-domain step measured fastest (window 1 about 1 % faster
-// than window 64 at 600x600x64 with 256-column tiles). G from a few blocks per CU (persistent)
-// up to one block per tile is the batch tuner's choice. Per-member fields make the result of a
-// member independent of the mix: it steps only while `it < nsteps` (it = index of this launch within the captured
-// chunk), reads lattice parity0 ^ (it & 1), and accumulates means from launch acc_from on.
-template <class S>
-__global__ __launch_bounds__(256)
-void batch_step_kernel(const Member *__restrict__ mem, int nmem, uint32_t total, int it) {
+// One grid of G blocks of BS threads walks one step of the batch members that step in this launch. Member d
+// contributes tiles (BS-column block, chunk of M.levels levels) for its fluid levels, concatenated in table order. The
+// host orders the table by step count, longest first, so the members stepping in launch `it` of a chunk are a prefix
+// of it and the launch covers only their tiles (nmem, total): a member that sits out a launch costs no workgroups.
+// The grid strides through the list in windows of G tiles, taking the tiles of a window in the XCD-aware order of the
+// single-domain step (runs of 64 tiles per XCD); with one block per tile (G = total, the default) block b takes tile
+// b. Per-member fields make a member's result independent of the mix: it reads lattice parity0 ^ (it & 1) and
+// accumulates means from launch acc_from on. BS (64, 128, 256) and the grid are the batch tuner's choice.
+template <class S, bool ACC>
+__device__ __forceinline__ void batch_tile(const Member &M, int p, int c, int k0) {
+  const Grid &g = M.g;
+  const uint32_t ck = ld_tab(g.colk, (size_t)(c < (int)g.NXY ? c : 0));
+  const int c0 = c - (int)(threadIdx.x & 63);
+  for (int u = 0; u < M.levels; u++) {
+    const double th = step_cell_body<S, ACC, false, false>(g, nullptr, M.f[p], M.f[1 - p], M.gq[p], M.gq[1 - p],
+                                                           M.acc, nullptr, c, k0 + u, ck);
+    if (ACC) content_add(g, M.acc, k0 + u, c0, th);
+  }
+}
+
+template <class S, int BS>
+__global__ __launch_bounds__(BS) void batch_step_kernel(const Member *__restrict__ mem, int nmem, uint32_t total,
+                                                        int it) {
   const uint32_t G = gridDim.x, tb = G >= total ? blockIdx.x : tile_of_block(blockIdx.x, G, 64);
   // member of the block's first tile by binary search over tile_base (ascending), then advanced as t grows
   int d = 0;
@@ -815,17 +827,14 @@ void batch_step_kernel(const Member *__restrict__ mem, int nmem, uint32_t total,
     if (t >= total) continue;                                   // partial last window
     while (d + 1 < nmem && mem[d + 1].tile_base <= t) d++;      // t grows with base: the member index only advances
     const Member &M = mem[d];
-    if (it >= M.nsteps) continue;
-    const uint32_t lt = t - M.tile_base;
-    const uint32_t bx = (M.g.NXY + 255) / 256;
-    const int k = M.g.kmin + (int)(lt / bx);
-    const int c = (int)(lt % bx) * 256 + (int)threadIdx.x;
+    if (it >= M.nsteps) continue;                               // not reached: the launch covers stepping members
+    uint32_t cb;
+    const uint32_t kk = div_by(t - M.tile_base, M.bx, M.inv_bx, cb);
+    const int c = (int)(cb * BS + threadIdx.x);
+    const int k0 = M.g.kmin + (int)kk * M.levels;
     const int p = M.parity0 ^ (it & 1);
-    const uint32_t slot = lt * 4 + (threadIdx.x >> 6);
-    if (it >= M.acc_from)
-      step_cell<S, true, false>(M.g, M.f[p], M.f[1 - p], M.gq[p], M.gq[1 - p], M.acc, nullptr, c, k, slot);
-    else
-      step_cell<S, false, false>(M.g, M.f[p], M.f[1 - p], M.gq[p], M.gq[1 - p], M.acc, nullptr, c, k, slot);
+    if (it >= M.acc_from) batch_tile<S, true>(M, p, c, k0);
+    else batch_tile<S, false>(M, p, c, k0);
   }
 }
 
@@ -1075,10 +1084,7 @@ size_t face_offset(int face, int nx, int ny, int nz) {
   return o;
 }
 
-uint32_t content_slots(const Grid &g, int block) {
-  const uint32_t bx = (g.NXY + block - 1) / block;
-  return bx * (uint32_t)(g.nz - g.kmin) * (uint32_t)(block / 64);
-}
+uint32_t content_slots(uint32_t nxy, int nz) { return ((nxy + 63u) >> 6) * (uint32_t)nz; }
 
 int storage_bytes(int storage) { return storage == 0 ? 4 : (storage == 3 ? 8 : 2); }
 bool storage_theta_ref(int storage) { return storage == 1 || storage == 2; }
@@ -1202,11 +1208,16 @@ hipError_t launch_zero_g_blocked(int storage, const Grid &g, void *gq, hipStream
   return hipGetLastError();
 }
 
-hipError_t launch_batch_step(int storage, int lb, const Member *mem, int nmem, uint32_t total_tiles, int it, int grid,
-                             hipStream_t st) {
+hipError_t launch_batch_step(int storage, int lb, int block, const Member *mem, int nmem, uint32_t total_tiles, int it,
+                             int grid, hipStream_t st) {
   if (total_tiles == 0 || nmem == 0) return hipSuccess;
   const int G = (int)std::min<uint32_t>((uint32_t)grid, total_tiles);
-  AHT_DISPATCH(storage, lb, hipLaunchKernelGGL(batch_step_kernel<S>, dim3(G), dim3(256), 0, st, mem, nmem, total_tiles, it))
+  AHT_DISPATCH(storage, lb, {
+    if (block == 64) hipLaunchKernelGGL((batch_step_kernel<S, 64>), dim3(G), dim3(64), 0, st, mem, nmem, total_tiles, it);
+    else if (block == 128) hipLaunchKernelGGL((batch_step_kernel<S, 128>), dim3(G), dim3(128), 0, st, mem, nmem, total_tiles, it);
+    else if (block == 256) hipLaunchKernelGGL((batch_step_kernel<S, 256>), dim3(G), dim3(256), 0, st, mem, nmem, total_tiles, it);
+    else return hipErrorInvalidValue;
+  })
   return hipGetLastError();
 }
 

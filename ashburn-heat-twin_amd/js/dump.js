@@ -51,6 +51,24 @@ function workloadOptions(name, over) {
   return Object.assign(o, over || {});
 }
 
+/** member d of the batched uneven-domain workload (S.BATCH_WORKLOAD): a Scenario with its campus placed; the seeds
+ *  derive from (workload seed, d), so every process builds the identical domain */
+function batchMember(d, seed) {
+  const W = S.BATCH_WORKLOAD, base = 1000 * (seed === undefined ? W.seed : seed) + d;
+  const rng = S.mulberry32(base);
+  const grid = W.grids[Math.floor(rng() * W.grids.length)];
+  const u10 = W.u10[0] + (W.u10[1] - W.u10[0]) * rng(), dirFrom = 360 * rng();
+  const [nx, ny, nz] = grid;
+  const sc = new Scenario({
+    synthetic: { nx, ny, hills: 12, buildings: W.buildings, sources: W.sources, seed: base },
+    dx: 6000 / nx, nz, u10Max: u10, alpha: 0.2, existingFlux: W.flux,
+    met: { u10, dirFrom, wstar: 1.67, blh: 903 },
+  });
+  const halls = [S.placeCampus(sc.agg, W.campus[0], W.campus[1], base + 7919)];
+  sc.setCampuses([{ id: 'campus', mw: W.campusMW, halls }]);
+  return { sc, grid, u10, dirFrom, seed: base };
+}
+
 function parseArgs(argv) {
   const a = {};
   for (let i = 0; i < argv.length; i++) {
@@ -67,6 +85,7 @@ function parseArgs(argv) {
 
 /** options from CLI flags:
  *  --workload c1|c2|c3                     synthetic benchmark workload
+ *  --batch-member d[,seed]                  member d of the batched uneven-domain workload (handled in main)
  *  --tiny nx,ny,nz,seed[,buildings,sources] small synthetic grid (dx from --dx, default 30)
  *  (default)                                the Ashburn static10 grid at --dx
  *  --u10 --dir --wstar --blh --ustar --flux --nsw --nsp --oclip --campus ABC --u10max --alpha --side */
@@ -103,13 +122,26 @@ function optionsFromArgs(a) {
   return o;
 }
 
-module.exports = { MET_AUG25, CAMPUSES, campusList, dumpScenario, workloadOptions, optionsFromArgs, parseArgs };
+module.exports = { MET_AUG25, CAMPUSES, campusList, dumpScenario, workloadOptions, optionsFromArgs, parseArgs,
+  batchMember };
 
 if (require.main === module) {
   const a = parseArgs(process.argv.slice(2));
-  if (!a.out) { console.error('usage: node dump.js --out DIR [--workload c2 | --tiny nx,ny,nz,seed | --dx 30] [flags]'); process.exit(2); }
-  const sc = new Scenario(optionsFromArgs(a));
-  if (a.campus) sc.setCampuses(campusList(a.campus));
-  const meta = dumpScenario(sc, a.out, { source: a.workload || (a.tiny ? 'tiny' : 'ashburn') });
+  if (!a.out) {
+    console.error('usage: node dump.js --out DIR [--workload c2 | --tiny nx,ny,nz,seed | --batch-member d | --dx 30] [flags]');
+    process.exit(2);
+  }
+  let sc, source;
+  if (a['batch-member'] !== undefined) {
+    const t = String(a['batch-member']).split(',').map(Number);
+    const m = batchMember(t[0], t[1]);
+    sc = m.sc;
+    source = `batch member ${t[0]} (seed ${m.seed}): ${m.grid.join('x')}, ${m.u10.toFixed(3)} m/s from ${m.dirFrom.toFixed(1)} deg`;
+  } else {
+    sc = new Scenario(optionsFromArgs(a));
+    if (a.campus) sc.setCampuses(campusList(a.campus));
+    source = a.workload || (a.tiny ? 'tiny' : 'ashburn');
+  }
+  const meta = dumpScenario(sc, a.out, { source });
   console.log(JSON.stringify({ out: a.out, nx: meta.nx, ny: meta.ny, nz: meta.nz, nFluid: meta.info.nFluid, heatMW: meta.heatMW }));
 }

@@ -8,17 +8,18 @@
 // batch and free their memory, which admits the next ones. Each member steps exactly its own step count and averages
 // exactly its own window, so its result is bit-identical to running it alone (tests/test_gpu_serve.py).
 //
-// Iteration-level priority (policy 'short-first', the default): every launch of a chunk steps only some members, so
-// a member that has little work left (cells x remaining steps <= --short-work) steps in every launch while members
-// with much more work step in the first 1/--long-share of the chunk's launches only, as long as any short member is
-// active. Small scenarios that arrive behind a 600 x 600 x 64 one then finish in about their own compute time instead
+// Iteration-level priority (policy 'short-first', the default): a launch covers only the members that step in it
+// (a member sitting out costs no workgroups), so a member that has little work left (cells x remaining steps <=
+// --short-work) steps in every launch while members with much more work step in the first 1/--long-share of the
+// chunk's launches only, as long as any short member is active. Small scenarios that arrive behind a 600 x 600 x 64 one then finish in about their own compute time instead
 // of one big step per launch, and the big one resumes its full rate as soon as they leave. Policy 'fifo' steps every
 // member in every launch.
 //
 // The baseline (`mode: 'sequential'`) runs the same requests one at a time with the plain per-solver path.
 // Usage: node serve.js --rate 5 --requests 24 [--mode batch|sequential] [--spinup 2200] [--mean 600] [--seed 1]
 //        [--chunk 50] [--budget-gb 200] [--slots 64] [--policy short-first|fifo] [--short-work 3e10]
-//        [--long-share 8] [--grid blocks (0 = online tuning; default one block per tile)] [--out file.json]
+//        [--long-share 8] [--grid blocks (0 = online tuning; default one block per tile)]
+//        [--block 64|128|256 (threads per block of the batched step; default tuned online)] [--out file.json]
 
 const fs = require('fs');
 const { Scenario } = require('./scenario');
@@ -143,6 +144,7 @@ async function serveBatched(reqs, o) {
   const A = loadAddon();
   const B = A.batchCreate(o.device || 0, o.storage || 0);
   A.batchSetGrid(B, o.grid === undefined ? ONE_BLOCK_PER_TILE : o.grid, o.useGraph === false ? 0 : 1);
+  A.batchSetBlock(B, o.block || 0);
   const budget = (o.budgetGB || 200) * 1e9, slots = o.slots || 64, chunk = o.chunk || 50;
   const shortWork = o.shortWork || 3e10, longShare = o.longShare || 8;
   const future = reqs.slice().sort((a, b) => a.arrival - b.arrival);
@@ -188,12 +190,12 @@ async function serveBatched(reqs, o) {
     }
   }
   const t1 = now();
-  const grid = A.batchGrid(B);
+  const grid = A.batchGrid(B), block = A.batchBlock(B);
   A.batchDestroy(B);
   results.sort((a, b) => a.id - b.id);
   return {
     summary: Object.assign(summary('batch', o, results, 0, t1, peak, busy),
-      { persistent_grid: grid, policy: o.policy || 'short-first' }),
+      { batch_grid: grid, batch_block: block, policy: o.policy || 'short-first' }),
     results,
   };
 }
@@ -251,7 +253,7 @@ async function main() {
   const o = {
     rate: +(a.rate || 5), requests: +(a.requests || 24), seed: +(a.seed || 1), chunk: +(a.chunk || 50),
     spinupS: +(a.spinup || 2200), meanS: +(a.mean || 600), budgetGB: +(a['budget-gb'] || 200), slots: +(a.slots || 64),
-    storage: +(a.storage || 0), grid: a.grid === undefined ? ONE_BLOCK_PER_TILE : +a.grid,
+    storage: +(a.storage || 0), grid: a.grid === undefined ? ONE_BLOCK_PER_TILE : +a.grid, block: +(a.block || 0),
     useGraph: a['no-graph'] ? false : true,
     policy: a.policy || 'short-first', shortWork: +(a['short-work'] || 3e10), longShare: +(a['long-share'] || 8),
   };

@@ -37,6 +37,11 @@ const WORKLOADS = {
 
 /** grids of the batched and served workloads (6 km box, dx = 6000 / nx) */
 const BATCH_GRIDS = [[100, 100, 32], [150, 150, 32], [200, 200, 48], [300, 300, 48]];
+/** the batched uneven-domain workload: 64 domains with grids drawn uniformly from BATCH_GRIDS, wind 2-9 m/s from any
+ *  direction, the 6 km box's seeded buildings and source roofs, one inserted 120 MW campus of 300 x 200 m each,
+ *  3,000 steps */
+const BATCH_WORKLOAD = { domains: 64, grids: BATCH_GRIDS, u10: [2, 9], campusMW: 120, campus: [300, 200],
+  buildings: 4000, sources: 210, flux: 400, steps: 3000, seed: 4 };
 const SERVE_GRIDS = [[[100, 100, 32], 0.50], [[200, 200, 48], 0.35], [[600, 600, 64], 0.15]];
 
 /** exact overlap of the rectangle [xa,xb) x [za,zb) (metres from the north-west corner, z southward) with the grid;
@@ -144,4 +149,5 @@ function placeCampus(agg, w, h, seed) {
   return [agg.origin.x0 + best[0], agg.origin.z0 + best[1], agg.origin.x0 + best[0] + w, agg.origin.z0 + best[1] + h];
 }
 
-module.exports = { mulberry32, generate, placeCampus, overlapCells, WORKLOADS, BATCH_GRIDS, SERVE_GRIDS, Z0_CHOICES };
+module.exports = { mulberry32, generate, placeCampus, overlapCells, WORKLOADS, BATCH_GRIDS, BATCH_WORKLOAD, SERVE_GRIDS,
+  Z0_CHOICES };

@@ -148,22 +148,32 @@ typedef struct aht_batch aht_batch;
 int aht_batch_create(int32_t device, int32_t storage, aht_batch **out);
 void aht_batch_destroy(aht_batch *b);
 const char *aht_batch_last_error(const aht_batch *b);
-/* members: same device and storage, 256-thread blocks, and one population layout (f32 solvers below 4 GiB of
- * lattice are blocked on the device, larger ones are not; a batch cannot mix them) */
+/* members: same device and storage and one population layout (lattice pairs below 4 GiB are blocked on the device
+ * in every format but f64 — f32 by 2048 cells, f16 / bf16 by 256 with their theta plane — larger pairs and f64 keep
+ * [q][cell]; a batch cannot mix the two). A member keeps its own levels per thread (aht_set_launch / aht_autotune);
+ * the batch's block size applies to all members of a launch. */
 int aht_batch_add(aht_batch *b, aht_solver *s);
 int aht_batch_remove(aht_batch *b, aht_solver *s);
 int aht_batch_size(const aht_batch *b);
 /* one chunk of n launches: member d (in add order) steps nsteps[d] <= n times (NULL: n) and accumulates the FP64
- * means from its launch acc_from[d] on (NULL or >= nsteps[d]: none). The n launches are captured once as a hipGraph
- * and replayed while membership, n and the grid stay the same. *ms (may be NULL) = event time of the chunk. */
+ * means from its launch acc_from[d] on (NULL or >= nsteps[d]: none). Launch `it` covers only the members with
+ * nsteps > it: a member that sits out a launch costs no workgroups. The n launches are captured once as a hipGraph
+ * and replayed while the block size, the grid and every launch's member and tile counts stay the same (member tables
+ * are uploaded before each chunk, so a change of members with the same counts replays too). *ms (may be NULL) = event
+ * time of the chunk. */
 int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t *acc_from, double *ms);
 int aht_batch_sync(aht_batch *b);
 /* grid size of the batched step: blocks > 0 fixes it (clamped to the tile count: one block per tile), 0 = online
- * tuning over 8 and 32 blocks per CU (persistent, looping over tiles) and one block per tile, one candidate per chunk,
- * timed per tile-step, again whenever the batch's tile count has moved by more than 2x since the last tuning (every
- * candidate computes the same result); use_graph 0 launches without graph capture */
+ * tuning between one block per tile and 32 blocks per CU (persistent, looping over tiles); use_graph 0 launches
+ * without graph capture */
 int aht_batch_set_grid(aht_batch *b, int32_t blocks, int32_t use_graph);
+/* threads per block of the batched step: 64, 128 or 256 (tiles of that many columns), 0 = online tuning over the
+ * three. The tuner times one (block, grid) candidate per chunk as device time per cell-step and tunes again whenever
+ * a chunk's cell-steps move by more than 2x from where the last tuning started; every candidate computes the same
+ * bits (no counterpart in the reference: its kernel has one workgroup size, js/live/kernel.js:395) */
+int aht_batch_set_block(aht_batch *b, int32_t block);
 int aht_batch_grid(const aht_batch *b);              /* chosen grid (0 while still tuning)                 */
+int aht_batch_block(const aht_batch *b);             /* chosen block size (0 while still tuning)           */
 
 /* launch shape of one solver's step kernel (no counterpart in the reference: its WGSL kernel has one fixed shape) */
 typedef struct {

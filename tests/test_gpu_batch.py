@@ -1,8 +1,9 @@
-"""Batched stepping of uneven domains in one persistent launch per step (hipGraph-captured chunks).
+"""Batched stepping of uneven domains in one launch per step (hipGraph-captured chunks).
 
 The bar is exact: a member's state and FP64 means after a batched run must be bit-identical to the same domain run
-alone through aht_step with the same steps and averaging window, whatever the mix, the chunking, the persistent grid
-size (online-tuned or fixed) and graph capture on or off. Tiny domains of different sizes, winds and outlets.
+alone through aht_step with the same steps and averaging window, whatever the mix, the chunking, the block size and
+grid (online-tuned or fixed), the members' levels per thread, graph capture on or off, and changes of a member's
+columns or launch shape between chunks. Tiny domains of different sizes, winds and outlets.
 """
 import numpy as np
 import pytest
@@ -40,13 +41,17 @@ def test_batch_matches_alone_bitwise(tmp_path, pkg, storage):
     steps = [37, 52, 45]
     acc_from = [20, 30, 0]
     ref = [alone(pkg, d, n, a, storage) for d, n, a in zip(doms, steps, acc_from)]
-    for use_graph, grid in ((True, 0), (False, 64), (True, 3)):
+    # (graph, grid, block, levels per member): tuned, fixed persistent grids, one block per tile, 64/128/256 threads
+    for use_graph, grid, block, levels in ((True, 0, 0, (1, 1, 1)), (False, 64, 128, (2, 1, 4)),
+                                           (True, 3, 64, (1, 4, 2)), (True, 1 << 30, 256, (4, 2, 1))):
         sol = [pkg.Solver.from_dump(d, storage=storage) for d in doms]
         with pkg.Batch(storage=storage) as b:
-            for s in sol:
+            for s, lv in zip(sol, levels):
+                s.set_launch(256, lv, 1)
                 s.mean_reset()
                 b.add(s)
             b.set_grid(grid, use_graph)
+            b.set_block(block)
             done = [0, 0, 0]
             chunk = 10
             while any(d < n for d, n in zip(done, steps)):
@@ -58,12 +63,15 @@ def test_batch_matches_alone_bitwise(tmp_path, pkg, storage):
             for k, s in enumerate(sol):
                 f, g = s.read_state()
                 s3, bud = s.read_mean()
-                assert np.array_equal(f.view(np.uint32), ref[k][0].view(np.uint32)), (use_graph, grid, k)
-                assert np.array_equal(g.view(np.uint32), ref[k][1].view(np.uint32)), (use_graph, grid, k)
-                assert np.array_equal(s3, ref[k][2]), (use_graph, grid, k)
+                case = (use_graph, grid, block, levels, k)
+                assert np.array_equal(f.view(np.uint32), ref[k][0].view(np.uint32)), case
+                assert np.array_equal(g.view(np.uint32), ref[k][1].view(np.uint32)), case
+                assert np.array_equal(s3, ref[k][2]), case
                 assert bud['n_acc'] == ref[k][3]['n_acc'] == steps[k] - acc_from[k]
-                assert bud['heat_out'] == ref[k][3]['heat_out'] and bud['content'] == ref[k][3]['content']
+                assert bud['heat_out'] == ref[k][3]['heat_out'] and bud['content'] == ref[k][3]['content'], case
                 assert s.steps == steps[k]
+            if block:
+                assert b.block == block
             for s in sol:
                 b.remove(s)
         for s in sol:
@@ -137,6 +145,46 @@ def test_members_join_and_leave_mid_run(tmp_path, pkg):
         assert s1.steps == 20 and s0.steps == 40 and s2.steps == 20
     for s in (s0, s1, s2):
         s.close()
+
+
+def test_member_columns_and_levels_change_between_chunks(tmp_path, pkg):
+    """A captured chunk is replayed only while every launch keeps its member and tile counts: raising a member's lowest
+    fluid level (fewer tiles) or its levels per thread between chunks re-captures, and the member still gives the
+    bits of the same changes applied alone"""
+    doms = domains(tmp_path)
+    d0 = doms[0]
+    ktop2 = np.maximum(d0['ktop'], d0['ktop'].min() + 2).astype(np.uint32)    # lowest fluid level up by two
+    kt2 = np.minimum(d0['kt'], ktop2).astype(np.uint32)
+
+    def run(batched):
+        s0, s1 = pkg.Solver.from_dump(d0), pkg.Solver.from_dump(doms[1])
+        b = pkg.Batch() if batched else None
+        if b:
+            b.set_block(128)
+            b.add(s0)
+            b.add(s1)
+        chunk = lambda: b.step(8) if b else (s0.step(8), s1.step(8))
+        chunk()
+        chunk()
+        if b:
+            b.sync()
+        s0.set_columns(kt2, ktop2, d0['cd'], d0['q'])
+        chunk()
+        s0.set_launch(256, 2, 1)
+        chunk()
+        if b:
+            b.sync()
+        out = [s.read_state() for s in (s0, s1)]
+        if b:
+            b.close()
+        s0.close()
+        s1.close()
+        return out
+
+    ref, got = run(False), run(True)
+    for (fa, ga), (fb, gb) in zip(ref, got):
+        assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32))
+        assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32))
 
 
 def test_autotune_restores_state(tmp_path, pkg):

@@ -1,0 +1,174 @@
+"""The benchmark configurations on the GPU at their own grid sizes (the seeded synthetic generator, js/synth.js).
+
+* 600 x 600 x 64 (the plain drop-in): the full protocol, 20,000 steps from rest with the 10-minute mean of the last
+  4,126 steps (dt 0.1455 s). No oracle run of that length exists (about 14 hours on 8 cores); the run is held to
+  sanity bounds and its figures are printed: heat-budget closure (advected out / injected, the reference's
+  heat_diag, tools/live/validate_live.py:226-240) within [0.85, 1.05], mass imbalance <= 5e-3 (the reference
+  reports 0.2 % for its 10 m prototype, README.md:45; the oracle's 200 x 200 x 32 protocol gives 0.20 %), a
+  positive mean warming below its peak, no non-finite cell. The oracle pins the same domain over 1,300 steps
+  (test_gpu_long_synthetic.py) and the single-step bar below.
+* 600 x 600 x 64, north_star's single-step bar at full size: after 1,000 GPU steps the state is read back, the oracle
+  takes one step from exactly that state on the host and the GPU one more step; the theta fields (sum of the D3Q7
+  populations) and the force-corrected velocities agree to relative L2 <= 1e-5.
+* 1200 x 1200 x 96 (reduced-precision storage): f16 deviations against the FP64 twin on the same domain, 1,000 steps
+  plus a 500-step mean (shortened from 5,000): mean near-ground warming within 1e-3 relative, the f32 build within
+  1e-5; bf16's figures are printed.
+* 64 batched uneven domains: the first 10 members (all four grid sizes) stepped 300 times in one launch per step and
+  one at a time with the plain step, states bit-identical (FNV digests of every population); a 100 x 100 x 32 member
+  stepped in a batch against the oracle (50 steps from rest within 1e-5, then one step from the batch's own state:
+  theta within 1e-5).
+* Serving: requests of the serving grid mix (100 x 100 x 32, 200 x 200 x 48, 600 x 600 x 64) served by continuous
+  batching, every result bit-identical to the same request run alone.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, dump_domain, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+JS = os.path.join(ROOT, 'ashburn-heat-twin_amd', 'js')
+
+
+def node(script, *args, timeout=600):
+    p = subprocess.run(['node', os.path.join(JS, script)] + [str(a) for a in args], capture_output=True, text=True,
+                       timeout=timeout)
+    assert p.returncode == 0, p.stderr[-3000:]
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
+@pytest.fixture(scope='module')
+def c2(tmp_path_factory):
+    return dump_domain(tmp_path_factory.mktemp('c2'), '--workload', 'c2')
+
+
+def test_c2_full_protocol_budgets(c2, pkg):
+    dom = c2
+    dt = dom['dt']
+    mean_steps = int(np.ceil(600.0 / dt))
+    assert mean_steps == 4126
+    with pkg.Solver.from_dump(dom) as s:
+        ms = 0.0
+        done = 0
+        while done < 20000 - mean_steps:
+            n = min(2000, 20000 - mean_steps - done)
+            ms += s.benchmark(n)
+            done += n
+            print(f'c2 protocol: step {done}', flush=True)
+        s.mean_reset()
+        s.step(mean_steps, acc=True)
+        s3, b = s.read_mean()
+        st = s.stats()
+    phys = pkg.budget_physical(b, dom['dx'], dt, dom['conv'])
+    closure = phys['heat_advected_out_MW'] / phys['heat_injected_MW']
+    th = s3[0]
+    sp = pkg.stats_physical(st, dom['dx'], dom['conv'])
+    print(json.dumps({'steps': 20000, 'mean_steps': mean_steps, 'ms_per_step': ms / (20000 - mean_steps),
+                      'closure': closure, 'mass_imbalance': phys['mass_imbalance_frac'],
+                      'heat_injected_MW': phys['heat_injected_MW'], 'heat_out_MW': phys['heat_advected_out_MW'],
+                      'heat_content_MJ': phys['heat_content_MJ'], 'warming_mean': float(th.mean()),
+                      'warming_peak': float(th.max()), 'theta_max': sp['theta_max'], 'umax': sp['umax']}))
+    assert st['non_finite'] == 0 and b['n_acc'] == mean_steps
+    assert 0.85 <= closure <= 1.05
+    assert phys['mass_imbalance_frac'] <= 5e-3
+    assert 0 < th.mean() < th.max() and sp['umax'] < 25
+
+
+def test_c2_single_step_theta_against_oracle(c2, pkg, O):
+    dom = c2
+    o = O.Oracle(dom, 'f32')
+    fl = o.fluid_mask()
+    with pkg.Solver.from_dump(dom) as s:
+        s.step(1000)
+        f, g = s.read_state()
+        o.f[:] = f
+        o.g[:] = g
+        del f, g
+        ro = o.step(1, out=True)
+        s.step(1, out=True)
+        go = s.read_out()
+        f1, g1 = s.read_state()
+    th_g = g1[:, fl].astype(np.float64).sum(0)
+    th_o = o.g[:, fl].astype(np.float64).sum(0)
+    errs = {'theta_state': rel_l2(th_g, th_o), 'theta_out': rel_l2(go[0, fl], ro[0, fl]),
+            'u_out': rel_l2(go[1:3, fl], ro[1:3, fl]), 'f': rel_l2(f1[:, fl], o.f[:, fl])}
+    print('c2 single step after 1,000 steps:', json.dumps(errs))
+    assert errs['theta_state'] <= 1e-5 and errs['theta_out'] <= 1e-5 and errs['u_out'] <= 1e-5
+    assert errs['f'] <= 1e-6
+
+
+def test_c3_reduced_precision_against_fp64_twin(tmp_path, pkg):
+    dom = dump_domain(tmp_path, '--workload', 'c3')
+    steps, mean_steps = 1500, 500
+    res = {}
+    for storage in ('f64', 'f16', 'bf16', 'f32'):
+        with pkg.Solver.from_dump(dom, storage=storage) as s:
+            s.step(steps - mean_steps)
+            s.mean_reset()
+            s.step(mean_steps, acc=True)
+            s3, _ = s.read_mean()
+            st = s.stats()
+        assert st['non_finite'] == 0, storage
+        res[storage] = s3[0].copy()
+        print(f'c3 {storage}: done', flush=True)
+    ref = res.pop('f64')
+    out = {}
+    for storage, sm in res.items():
+        out[storage] = {'warming_mean_rel': abs(sm.mean() - ref.mean()) / abs(ref.mean()),
+                        'warming_peak_rel': abs(sm.max() - ref.max()) / abs(ref.max()),
+                        'warming_rel_l2': rel_l2(sm, ref)}
+    print('c3 1200x1200x96, 1,000 + 500 steps, against the FP64 twin:', json.dumps(out))
+    assert out['f16']['warming_mean_rel'] <= 1e-3
+    assert out['f32']['warming_mean_rel'] <= 1e-5
+
+
+def test_c4_batched_members_equal_alone():
+    out = node('batch_run.js', '--domains', 10, '--steps', 300, '--chunk', 50, '--check', 'all', '--tune-steps', 2)
+    print(json.dumps(out))
+    assert set(out['workload']['mix']) == {'100x100x32', '150x150x32', '200x200x48', '300x300x48'}
+    assert len(out['identical']) == 10 and all(out['identical'])
+
+
+def test_c4_member_in_batch_against_oracle(tmp_path, pkg, O):
+    doms = [dump_domain(tmp_path / f'm{d}', '--batch-member', d) for d in (0, 9, 2)]
+    assert (doms[0]['nx'], doms[0]['ny'], doms[0]['nz']) == (100, 100, 32)
+    o = O.Oracle(doms[0], 'f32')
+    o.init_flow()
+    fl = o.fluid_mask()
+    sol = [pkg.Solver.from_dump(d) for d in doms]
+    with pkg.Batch() as b:
+        for s in sol:
+            b.add(s)
+        b.step(25)
+        b.step(25)
+        b.sync()
+        o.step(50)
+        f, g = sol[0].read_state()
+        e50 = (rel_l2(f[:, fl], o.f[:, fl]), rel_l2(g[:, fl], o.g[:, fl]))
+        o.f[:] = f
+        o.g[:] = g
+        o.step(1)
+        b.step(1)
+        b.sync()
+        f1, g1 = sol[0].read_state()
+        for s in sol:
+            b.remove(s)
+    for s in sol:
+        s.close()
+    e1 = rel_l2(g1[:, fl].astype(np.float64).sum(0), o.g[:, fl].astype(np.float64).sum(0))
+    print(f'c4 member 0 in a batch: 50 steps f {e50[0]:.2e} g {e50[1]:.2e}; one step theta {e1:.2e}')
+    assert e50[0] <= 1e-5 and e50[1] <= 1e-5 and e1 <= 1e-5
+
+
+def test_c5_grid_mix_served_equal_alone():
+    out = node('../../tests/js/serve_test.js', '--c5')
+    print(json.dumps(out))
+    assert out['summary']['requests'] == 8
+    assert {tuple(g) for g in out['grids']} == {(100, 100, 32), (200, 200, 48), (600, 600, 64)}
+    assert all(out['identical']), out['identical']
+    for a, b, m in out['nAcc']:
+        assert a == b == m

@@ -12,7 +12,8 @@ _dbg.stats(), js/live/solver.js:559-576) pinned against closed forms on the CPU.
   max(theta, 0) and max |u| are the plain sums and maxima of the drawn fields over the finite fluid cells, the NaN
   fluid cell counts once as non-finite and the solid one not at all.
 The f64 build is held to 1e-10 relative (the oracle sums ~1e6 cells in sequence), the f32 build (the WebGPU kernel's
-arithmetic) to f32 accuracy. A wrong lattice velocity, a dropped 1/rho, solid cells in the sums or a wrong NaN rule fail one of these.
+arithmetic) to f32 accuracy. A wrong lattice velocity, a dropped 1/rho, solid cells in the sums or a wrong NaN rule
+fail one of these.
 """
 import numpy as np
 import pytest
