@@ -149,12 +149,15 @@ int64_t aht_footprint_bytes(const aht_dims *d) {
   const int64_t N = (int64_t)d->nx * d->ny * d->nz, NXY = (int64_t)d->nx * d->ny;
   const int eb = d->storage == AHT_STORE_F32 ? 4 : (d->storage == AHT_STORE_F64 ? 8 : 2);
   const int lb = d->layout == AHT_LAYOUT_QCELL ? 0 : aht::population_layout(d->storage, (size_t)N);
-  const int64_t pop = 2 * (int64_t)(aht::f_lattice_bytes(d->storage, lb, (size_t)N) +
-                                    aht::g_lattice_bytes(d->storage, lb, (size_t)N));
+  // both lattice pairs in one allocation of 2 MiB-aligned parts, as aht_create allocates them
+  const int64_t A = (int64_t)2 << 20;
+  const int64_t fb = ((int64_t)aht::f_lattice_bytes(d->storage, lb, (size_t)N) + A - 1) / A * A;
+  const int64_t gb = ((int64_t)aht::g_lattice_bytes(d->storage, lb, (size_t)N) + A - 1) / A * A;
+  const int64_t pop = 2 * (fb + gb);
   (void)eb;
   const int64_t cols = NXY * (4 + 8 + 4 + 4) + (int64_t)d->nz * 16;
   const int64_t acc = 3 * NXY * 8 + (int64_t)face_total(d->nx, d->ny, d->nz) * 8 + (int64_t)max_content_slots(*d) * 8;
-  const int64_t misc = ((N + 255) / 256) * 64 + 3 * NXY * 4 + (N + 3) / 4 * 4 / 1 + 4096;
+  const int64_t misc = 7 * (int64_t)aht::STATS_MAX_BLOCKS * 8 + 3 * NXY * 4 + (N + 3) / 4 * 4 + 4096;
   return pop + cols + acc + misc;
 }
 
@@ -227,7 +230,7 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   s->content_cap = max_content_slots(d);
   HIP_TRY(s, hipMalloc(&s->acc_content, s->content_cap * sizeof(double)));
   HIP_TRY(s, hipMalloc(&s->red_out, 8 * sizeof(double)));
-  HIP_TRY(s, hipMalloc(&s->stat_part, 8 * (size_t)((N + 255) / 256) * sizeof(double)));
+  HIP_TRY(s, hipMalloc(&s->stat_part, 7 * (size_t)aht::STATS_MAX_BLOCKS * sizeof(double)));
   HIP_TRY(s, hipMalloc(&s->surf3, 3 * (size_t)NXY * sizeof(float)));
   HIP_TRY(s, hipMalloc(&s->vol, ((N + 3) / 4) * sizeof(uint32_t)));
   HIP_TRY(s, hipMalloc(&s->tmax, sizeof(uint32_t)));
@@ -641,6 +644,7 @@ struct aht_batch {
   std::vector<double> cand_ms;
   int tune_next = 0;
   double tune_cells = 0.0;
+  int last_block = 0, last_grid = 0;   // used by the most recent chunk
   int64_t steps = 0;
   std::string err;
 };
@@ -817,6 +821,8 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
     pick = 0;
   }
   const int block = b->cand[pick].first, grid = b->cand[pick].second;
+  b->last_block = block;
+  b->last_grid = grid;
   // member table, longest-stepping members first: the members that step in launch `it` are then a prefix of it
   b->order.resize(nm);
   for (size_t d = 0; d < nm; d++) b->order[d] = (int)d;
@@ -928,24 +934,18 @@ int aht_batch_sync(aht_batch *b) {
 
 int aht_batch_grid(const aht_batch *b) {
   if (!b) return -1;
-  if (b->cand.size() > 1 && b->tune_next < (int)b->cand.size()) return 0;
-  const int g = b->cand[b->cand.size() > 1 ? best_candidate(b) : 0].second;
-  return (int)std::min<uint32_t>((uint32_t)g, std::max<uint32_t>(b->total, 1));   // blocks launched for all members
+  return b->last_grid ? (int)std::min<uint32_t>((uint32_t)b->last_grid, std::max<uint32_t>(b->total, 1)) : 0;
 }
 
-int aht_batch_block(const aht_batch *b) {
-  if (!b) return -1;
-  if (b->cand.size() > 1 && b->tune_next < (int)b->cand.size()) return 0;
-  return b->cand[b->cand.size() > 1 ? best_candidate(b) : 0].first;
-}
+int aht_batch_block(const aht_batch *b) { return b ? b->last_block : -1; }
 
 // ---- launch-shape autotuner for one solver ------------------------------------------------------------------------
 namespace {
 int check_launch(aht_solver *s, const aht_launch &l) {
   if (l.block < 64 || l.block > 256 || l.block % 64) return fail(s, AHT_EINVAL, "block must be 64, 128, 192 or 256");
   if (l.levels != 1 && l.levels != 2 && l.levels != 4) return fail(s, AHT_EINVAL, "levels must be 1, 2 or 4");
-  if (l.window < 0 || l.window > 4096 || (l.window & (l.window - 1)))
-    return fail(s, AHT_EINVAL, "window must be 0 (slabs) or a power of two up to 4096");
+  if (l.window != -1 && (l.window < 0 || l.window > 4096 || (l.window & (l.window - 1))))
+    return fail(s, AHT_EINVAL, "window must be -1 (plane slabs), 0 (slabs) or a power of two up to 4096");
   return AHT_OK;
 }
 void apply_launch(aht_solver *s, const aht_launch &l) {
@@ -975,7 +975,7 @@ int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best) {
   // every shape over `steps` timed steps (after one untimed step that also uploads the changed grid), then the four
   // fastest again twice over 3 x `steps`, best of the two: neighbouring shapes differ by ~1 %, about the noise of a
   // single short timing
-  const int cb[3] = {64, 128, 256}, cl[3] = {1, 2, 4}, cw[5] = {1, 2, 4, 16, 0};
+  const int cb[3] = {64, 128, 256}, cl[3] = {1, 2, 4}, cw[6] = {1, 2, 4, 16, 0, -1};
   struct Cand {
     aht_launch l;
     double ms;

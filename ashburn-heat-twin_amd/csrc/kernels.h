@@ -45,6 +45,9 @@ struct Streams {
 };
 
 constexpr int STEP_BLOCK = 256;
+// Grid::win value of the plane-slab tile order (aht_launch window -1): XCD x steps the x-th eighth of the column
+// blocks of every level chunk, level chunk after level chunk
+constexpr uint32_t WIN_PLANE_SLABS = 0xffffffffu;
 constexpr uint32_t NUM_XCD = 8;   // MI355X: workgroups are dispatched round-robin over 8 XCDs, each with its own L2
 
 // one member of a batched launch (device-resident table, uploaded by the host before every chunk)
@@ -96,10 +99,12 @@ hipError_t launch_surface(int storage, const Grid &g, const void *fA, const void
                           hipStream_t st);
 hipError_t launch_volume(int storage, const Grid &g, const void *gA, int nzv, float volscale, uint32_t *vout,
                          uint32_t *tmax_bits, hipStream_t st);
-// two-pass deterministic reduction: per-block partials into part (>= stats_partials(g) entries), then one block
+// diagnostics, two-level deterministic reduction: a grid of stats_blocks(g) <= STATS_MAX_BLOCKS blocks writes
+// part[7 * stats_blocks(g)], then one block per statistic writes out8[0..6]
+constexpr uint32_t STATS_MAX_BLOCKS = 2048;
 hipError_t launch_stats(int storage, const Grid &g, const void *fA, const void *gA, double *part, double *out8,
                         hipStream_t st);
-uint32_t stats_partials(const Grid &g);
+uint32_t stats_blocks(const Grid &g);
 // fixed-order sum of n doubles into *out (one block)
 hipError_t launch_sum(const double *x, uint32_t n, double *out, hipStream_t st);
 // population q of a lattice of n cells (layout lb) between storage and an f32 (or f64) host-format plane; w = rest

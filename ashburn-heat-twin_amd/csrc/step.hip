@@ -768,15 +768,26 @@ __device__ __forceinline__ void content_add(const Grid &g, const Acc &acc, int k
 }
 
 // one block per (column block, KB-level chunk) tile, numbered column block fastest, then level chunk, taken in the
-// order g.win selects; a thread walks its column through the KB levels (column word and index math once per column)
+// order g.win selects; a thread walks its column through the KB levels (column word and index math once per column).
+// Plane slabs (sw > 0, g.win == WIN_PLANE_SLABS): the column blocks of a level chunk are cut into eight slabs of sw
+// blocks, XCD x stepping slab x of every level chunk in turn, so the eight XCDs move through the levels together and
+// the tiles that pull from one another across levels (the same columns one level up or down) meet in one XCD's L2;
+// blocks past the last column block (at most 7 per level chunk) exit.
 template <class S, bool ACC, bool OUT, int KB>
 __global__ __launch_bounds__(256) void step_kernel(const Grid *__restrict__ gp, const Streams *__restrict__ sp,
                                                    const void *fA, void *fB, const void *gA, void *gB, Acc acc,
-                                                   float *__restrict__ out4, uint32_t bx, float inv_bx) {
+                                                   float *__restrict__ out4, uint32_t bx, float inv_bx, uint32_t sw,
+                                                   float inv_sw) {
   const Grid &g = *gp;
-  const uint32_t t = tile_of_block(blockIdx.x, gridDim.x, g.win);
-  uint32_t cb;
-  const uint32_t kk = div_by(t, bx, inv_bx, cb);
+  uint32_t cb, kk;
+  if (sw) {
+    uint32_t r;
+    kk = div_by(blockIdx.x >> 3, sw, inv_sw, r);
+    cb = (blockIdx.x & (NUM_XCD - 1)) * sw + r;
+    if (cb >= bx) return;
+  } else {
+    kk = div_by(tile_of_block(blockIdx.x, gridDim.x, g.win), bx, inv_bx, cb);
+  }
   const int c = (int)(cb * blockDim.x + threadIdx.x);
   const int c0 = c - (int)(threadIdx.x & 63);
   const int k0 = g.kmin + (int)kk * KB;
@@ -958,42 +969,44 @@ __global__ __launch_bounds__(256) void volume_kernel(Grid g, const void *gA, int
   if (threadIdx.x == 0) atomicMax(tmax_bits, __float_as_uint(red[0]));
 }
 
-// diagnostics over the stored state (genStatsWGSL :580-633), FP64 per-block partials:
-// part[8*b + (0..6)] = sum theta, sum ke, sum (rho-1), fluid count, max theta, max |u|, non-finite count
+// diagnostics over the stored state (genStatsWGSL :580-633) in FP64, two levels, deterministic: a grid of at most
+// STATS_MAX_BLOCKS blocks strides over the cells (a thread takes cells t, t + G * 256, ... in order), each block
+// reduces its threads and writes its partials part[s * G + block] for s = 0..6 (sum theta, sum ke, sum (rho - 1),
+// fluid count, max theta, max |u|, non-finite count); then one block per statistic reduces the G partials in a
+// fixed order (stats_finish). The partials take 7 * G doubles whatever the grid size.
 template <class S>
 __global__ __launch_bounds__(256) void stats_kernel(Grid g, const void *fA, const void *gA, double *__restrict__ part) {
   using R = typename S::real;
   using E = typename S::elem;
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
   double v[7] = {0, 0, 0, 0, 0, 0, 0};
-  if (n < g.N) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < g.N; n += stride) {
     const uint32_t c = n % g.NXY, k = n / g.NXY;
-    if ((int)k >= (int)g.ktop[c]) {
-      const size_t N = g.N;
-      R a[19];
-      OwnPops<0>::template run<S>(a, (const E *)fA, N, n);
-      R rho = a[0], jx = R(0), jy = R(0), jz = R(0);
+    if ((int)k < (int)g.ktop[c]) continue;
+    const size_t N = g.N;
+    R a[19];
+    OwnPops<0>::template run<S>(a, (const E *)fA, N, n);
+    R rho = a[0], jx = R(0), jy = R(0), jz = R(0);
 #pragma unroll
-      for (int q = 1; q < 19; q++) {
-        rho += a[q];
-        jx += R(CX[q]) * a[q];
-        jy += R(CY[q]) * a[q];
-        jz += R(CZ[q]) * a[q];
-      }
-      R th = R(0);
+    for (int q = 1; q < 19; q++) {
+      rho += a[q];
+      jx += R(CX[q]) * a[q];
+      jy += R(CY[q]) * a[q];
+      jz += R(CZ[q]) * a[q];
+    }
+    R th = R(0);
 #pragma unroll
-      for (int q = 0; q < 7; q++) th += ldg<S>(g, gA, q, n);
-      if (!isfinite((double)rho) || !isfinite((double)th)) {
-        v[6] = 1.0;
-      } else {
-        const double jj = (double)jx * jx + (double)jy * jy + (double)jz * jz;
-        v[0] = (double)th;
-        v[1] = 0.5 * jj / (double)rho;
-        v[2] = (double)rho - 1.0;
-        v[3] = 1.0;
-        v[4] = fmax((double)th, 0.0);
-        v[5] = sqrt(jj) / (double)rho;
-      }
+    for (int q = 0; q < 7; q++) th += ldg<S>(g, gA, q, n);
+    if (!isfinite((double)rho) || !isfinite((double)th)) {
+      v[6] += 1.0;
+    } else {
+      const double jj = (double)jx * jx + (double)jy * jy + (double)jz * jz;
+      v[0] += (double)th;
+      v[1] += 0.5 * jj / (double)rho;
+      v[2] += (double)rho - 1.0;
+      v[3] += 1.0;
+      v[4] = fmax(v[4], (double)th);
+      v[5] = fmax(v[5], sqrt(jj) / (double)rho);
     }
   }
   __shared__ double red[7][4];
@@ -1014,29 +1027,25 @@ __global__ __launch_bounds__(256) void stats_kernel(Grid g, const void *fA, cons
     const int nwv = blockDim.x >> 6;
     double x = red[s][0];
     for (int w = 1; w < nwv; w++) x = (s == 4 || s == 5) ? fmax(x, red[s][w]) : x + red[s][w];
-    part[8 * (size_t)blockIdx.x + s] = x;
+    part[(size_t)s * gridDim.x + blockIdx.x] = x;
   }
 }
 
-// one block: out8[s] = fixed-order reduction of part[8*b + s] over nb blocks
+// block s: out8[s] = fixed-order reduction (sum, or max for s = 4, 5) of the nb partials part[s * nb + b]
 __global__ __launch_bounds__(256) void stats_finish(const double *__restrict__ part, uint32_t nb, double *__restrict__ out8) {
   __shared__ double red[256];
-  for (int s = 0; s < 7; s++) {
-    const bool mx = (s == 4 || s == 5);
-    double x = 0.0;
-    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
-      const double y = part[8 * (size_t)b + s];
-      x = mx ? fmax(x, y) : x + y;
-    }
-    red[threadIdx.x] = x;
-    __syncthreads();
-    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-      if ((int)threadIdx.x < h) red[threadIdx.x] = mx ? fmax(red[threadIdx.x], red[threadIdx.x + h]) : red[threadIdx.x] + red[threadIdx.x + h];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out8[s] = red[0];
+  const int s = blockIdx.x;
+  const bool mx = (s == 4 || s == 5);
+  const double *p = part + (size_t)s * nb;
+  double x = 0.0;
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) x = mx ? fmax(x, p[b]) : x + p[b];
+  red[threadIdx.x] = x;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] = mx ? fmax(red[threadIdx.x], red[threadIdx.x + h]) : red[threadIdx.x] + red[threadIdx.x + h];
     __syncthreads();
   }
+  if (threadIdx.x == 0) out8[s] = red[0];
 }
 
 // one block: *out = fixed-order sum of x[0..n)
@@ -1143,13 +1152,14 @@ static void launch_step_kb(const Grid &g, const Grid *gd, const Streams *sd, con
                            void *gB, const Acc *acc, float *out4, int block, hipStream_t st) {
   const uint32_t bx = (g.NXY + block - 1) / block;
   const uint32_t nk = (uint32_t)(g.nz - g.kmin + KB - 1) / KB;
-  const dim3 grid(bx * nk);
+  const uint32_t sw = g.win == WIN_PLANE_SLABS ? (bx + NUM_XCD - 1) / NUM_XCD : 0;   // plane slabs: blocks per slab
+  const dim3 grid(sw ? NUM_XCD * sw * nk : bx * nk);
   const Acc a = acc ? *acc : Acc{nullptr, nullptr, nullptr};
-  const float ib = 1.0f / (float)bx;
-  if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib);
-  else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib);
-  else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib);
-  else hipLaunchKernelGGL((step_kernel<S, false, false, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib);
+  const float ib = 1.0f / (float)bx, isw = sw ? 1.0f / (float)sw : 0.0f;
+  if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw);
+  else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw);
+  else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw);
+  else hipLaunchKernelGGL((step_kernel<S, false, false, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw);
 }
 
 hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const Streams *sd, const void *fA, void *fB,
@@ -1245,13 +1255,13 @@ hipError_t launch_volume(int storage, const Grid &g, const void *gA, int nzv, fl
   return hipGetLastError();
 }
 
-uint32_t stats_partials(const Grid &g) { return (g.N + 255) / 256; }
+uint32_t stats_blocks(const Grid &g) { return std::min<uint32_t>((g.N + 255) / 256, STATS_MAX_BLOCKS); }
 
 hipError_t launch_stats(int storage, const Grid &g, const void *fA, const void *gA, double *part, double *out8,
                         hipStream_t st) {
-  const uint32_t nb = stats_partials(g);
+  const uint32_t nb = stats_blocks(g);
   AHT_DISPATCH(storage, g.lb, hipLaunchKernelGGL(stats_kernel<S>, dim3(nb), dim3(256), 0, st, g, fA, gA, part))
-  hipLaunchKernelGGL(stats_finish, dim3(1), dim3(256), 0, st, part, nb, out8);
+  hipLaunchKernelGGL(stats_finish, dim3(7), dim3(256), 0, st, part, nb, out8);
   return hipGetLastError();
 }
 

@@ -49,13 +49,54 @@ async function createLiveSolver(opts) {
   return s;
 }
 
+// shortest signed turn from direction a to direction b, degrees in [-180, 180)
+const signedTurn = (a, b) => ((b - a + 540) % 360) - 180;
+const wrap360 = d => ((d % 360) + 360) % 360;
+
+/** The inflow the interior flow has adjusted to, and the rule that re-initialises the flow on a large turn
+ *  (js/live/solver.js:271-285, :296-302): the adjusted state relaxes towards the applied wind with time constant
+ *  ADJ_TAU_S of simulated time; a new wind that turns more than dirTol from the adjusted or the applied direction,
+ *  with neither calm, re-initialises the flow and becomes the adjusted state at once. */
+class AdjustedInflow {
+  constructor(tolDeg, enabled) {
+    this.tol = tolDeg; this.enabled = enabled;
+    this.applied = null;    // { dir, u } of the last met
+    this.adjusted = null;   // { dir, u } the interior flow has adjusted to
+  }
+
+  /** a new met; true when the flow must be re-initialised */
+  observe(dir, u) {
+    const ad = this.adjusted, ap = this.applied;
+    let reinit = false;
+    if (this.enabled && ad && ap && u > P.CALM && Math.max(ad.u, ap.u) > P.CALM) {
+      const turn = Math.max(Math.abs(signedTurn(ad.dir, dir)), Math.abs(signedTurn(ap.dir, dir)));
+      reinit = turn > this.tol;
+    }
+    this.applied = { dir, u };
+    if (!ad || reinit) this.adjusted = { dir, u };
+    return reinit;
+  }
+
+  /** seconds of simulated time pass */
+  relax(seconds) {
+    const ad = this.adjusted, ap = this.applied;
+    if (!ad || !ap) return;
+    const a = 1 - Math.exp(-seconds / P.ADJ_TAU_S);
+    ad.dir = wrap360(ad.dir + signedTurn(ad.dir, ap.dir) * a);
+    ad.u += (ap.u - ad.u) * a;
+  }
+
+  /** the flow was reset to equilibrium with the applied wind */
+  reset() { if (this.applied) this.adjusted = Object.assign({}, this.applied); }
+}
+
 class Solver {
   constructor(A, sc, o, prog) {
     this._A = A; this._sc = sc; this._o = o; this._prog = prog;
     this._h = null; this._destroyed = false;
     this._n = 0; this._steps = 0; this._q = Promise.resolve();
     this.flowReinits = 0;
-    this._flowRef = null; this._applied = null;
+    this._inflow = new AdjustedInflow(P.def(o.dirTol, P.DIR_TOL), o.metReinit !== false);
     this.info = null;
     this.campusInfo = [];
   }
@@ -72,7 +113,7 @@ class Solver {
     this.info = Object.assign(sc.info(), { adapter: 'AMD Instinct (HIP, gfx950)', storage: this._o.storage || 'f32' });
     this.nzv = this.info.nzv;
     this._uploadColumns();
-    this._applyMet(this._o.met || P.DEFAULT_MET, true);
+    this._applyMet(this._o.met || P.DEFAULT_MET);
     this._prog('Initialising the flow', 0.85);
     this.resetFlow();
   }
@@ -100,40 +141,20 @@ class Solver {
     this._A.setParams(this._h, this._sc.params, this._sc.profBlock());
   }
 
-  _applyMet(m, first) {
+  _applyMet(m) {
     const met = this._sc.setMet(m);
     this._uploadParams();
-    // flow re-initialisation on a turn of more than dirTol against the lagged adjusted direction
-    // (js/live/solver.js:271-285)
-    const ref = this._flowRef, applied = this._applied;
-    let reinit = false;
-    if (!first && ref && applied && this._o.metReinit !== false) {
-      const ang = (a, b) => Math.abs(((a - b + 540) % 360) - 180);
-      const turn = Math.max(ang(met.dirFrom, ref.dir), ang(met.dirFrom, applied.dir));
-      const tol = P.def(this._o.dirTol, P.DIR_TOL);
-      if (met.u10 > P.CALM && Math.max(ref.u, applied.u) > P.CALM && turn > tol) reinit = true;
-    }
-    this._applied = { dir: met.dirFrom, u: met.u10 };
-    if (!ref || reinit) this._flowRef = { dir: met.dirFrom, u: met.u10 };
-    if (reinit) {
-      this._A.initFlow(this._h);
+    if (this._inflow.observe(met.dirFrom, met.u10)) {
+      this._A.initFlow(this._h);        // theta kept, simTime keeps running
       this.flowReinits += 1;
     }
     return met;
   }
 
-  _advanceFlowRef(n) {
-    const ref = this._flowRef, ap = this._applied;
-    if (!ref || !ap) return;
-    const a = 1 - Math.exp(-n * this._sc.dt / P.ADJ_TAU_S);
-    ref.dir = (((ref.dir + (((ap.dir - ref.dir + 540) % 360) - 180) * a) % 360) + 360) % 360;
-    ref.u += (ap.u - ref.u) * a;
-  }
-
   /** { u10, dirFrom, wstar, blh, ustar }; applies from the next step (js/live/solver.js:245-285) */
   setMet(m) {
     this._check();
-    return this._enqueue(() => this._applyMet(m || {}, false));
+    return this._enqueue(() => this._applyMet(m || {}));
   }
 
   /** W per m^2 of operational data-center roof (js/live/solver.js:305-310) */
@@ -167,7 +188,7 @@ class Solver {
   resetFlow() {
     this._check();
     this._n = 0; this._steps = 0;
-    if (this._applied) this._flowRef = Object.assign({}, this._applied);
+    this._inflow.reset();
     return this._enqueue(() => { this._A.initFlow(this._h); this._A.resetHeat(this._h); });
   }
 
@@ -176,7 +197,7 @@ class Solver {
     try { this._check(); } catch (e) { return Promise.reject(e); }
     n = Math.max(0, Math.floor(n === undefined ? 1 : n));
     this._n += n; this._steps += n;
-    this._advanceFlowRef(n);
+    this._inflow.relax(n * this._sc.dt);
     return this._enqueue(() => (n > 0 ? this._A.step(this._h, n, flags || 0) : undefined));
   }
 
@@ -210,7 +231,7 @@ class Solver {
     const n = nSteps === undefined ? 300 : nSteps;
     return this.step(Math.min(20, n)).then(() => {
       this._n += n; this._steps += n;
-      this._advanceFlowRef(n);
+      this._inflow.relax(n * this._sc.dt);
       return this._enqueue(() => this._A.benchmark(this._h, n)).then(ms => this.info.nCells * n / (ms * 1e3));
     });
   }

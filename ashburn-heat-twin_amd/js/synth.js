@@ -4,7 +4,7 @@
 // shape of an aggregated grid (dem, frac, hmean, dcarea, cd; north-up), so the reference's build_domain rules
 // (js/live/domain.js:166-190, restated in domain.js) turn it into solver columns exactly as for the Ashburn inputs.
 
-const { dragCoefficient } = require('./domain');
+const { dragCoefficient, overlapCells } = require('./domain');
 
 /** mulberry32: 32-bit state, returns uniform [0, 1) doubles */
 function mulberry32(seed) {
@@ -43,25 +43,6 @@ const BATCH_GRIDS = [[100, 100, 32], [150, 150, 32], [200, 200, 48], [300, 300, 
 const BATCH_WORKLOAD = { domains: 64, grids: BATCH_GRIDS, u10: [2, 9], campusMW: 120, campus: [300, 200],
   buildings: 4000, sources: 210, flux: 400, steps: 3000, seed: 4 };
 const SERVE_GRIDS = [[[100, 100, 32], 0.50], [[200, 200, 48], 0.35], [[600, 600, 64], 0.15]];
-
-/** exact overlap of the rectangle [xa,xb) x [za,zb) (metres from the north-west corner, z southward) with the grid;
- *  calls fn(cell, area) per covered north-up cell */
-function overlapCells(xa, xb, za, zb, nx, ny, dx, fn) {
-  const X1 = nx * dx, Z1 = ny * dx;
-  const x0 = Math.max(xa, 0), x1 = Math.min(xb, X1), z0 = Math.max(za, 0), z1 = Math.min(zb, Z1);
-  if (x1 <= x0 || z1 <= z0) return;
-  const c0 = Math.floor(x0 / dx), c1 = Math.min(nx - 1, Math.floor(x1 / dx));
-  const r0 = Math.floor(z0 / dx), r1 = Math.min(ny - 1, Math.floor(z1 / dx));
-  for (let r = r0; r <= r1; r++) {
-    const oz = Math.min(z1, (r + 1) * dx) - Math.max(z0, r * dx);
-    if (oz <= 0) continue;
-    for (let c = c0; c <= c1; c++) {
-      const ox = Math.min(x1, (c + 1) * dx) - Math.max(x0, c * dx);
-      if (ox <= 0) continue;
-      fn(r * nx + c, ox * oz);
-    }
-  }
-}
 
 /**
  * spec: { nx, ny, dx, hills, buildings, sources, seed, baseM = 80, hillAmp = [0, 40], hillWidth = [300, 1500],

@@ -172,15 +172,16 @@ int aht_batch_set_grid(aht_batch *b, int32_t blocks, int32_t use_graph);
  * a chunk's cell-steps move by more than 2x from where the last tuning started; every candidate computes the same
  * bits (no counterpart in the reference: its kernel has one workgroup size, js/live/kernel.js:395) */
 int aht_batch_set_block(aht_batch *b, int32_t block);
-int aht_batch_grid(const aht_batch *b);              /* chosen grid (0 while still tuning)                 */
-int aht_batch_block(const aht_batch *b);             /* chosen block size (0 while still tuning)           */
+int aht_batch_grid(const aht_batch *b);              /* blocks launched for all members by the last chunk (0: none yet) */
+int aht_batch_block(const aht_batch *b);             /* block size of the last chunk (0: none yet)                      */
 
 /* launch shape of one solver's step kernel (no counterpart in the reference: its WGSL kernel has one fixed shape) */
 typedef struct {
   int32_t block;         /* threads per block: 64, 128, 192 or 256                                        */
   int32_t levels;        /* cells per thread along z: 1, 2 or 4                                           */
   int32_t window;        /* tile order: 1 linear, W = 2, 4, ..., 4096 = runs of W consecutive tiles per XCD,
-                            0 = one contiguous slab of tiles per XCD                                      */
+                            0 = one contiguous slab of tiles per XCD, -1 = plane slabs (XCD x steps the
+                            x-th eighth of the columns of every level, all XCDs level by level together)  */
   int32_t reserved;
 } aht_launch;
 

@@ -193,21 +193,27 @@ def test_autotune_restores_state(tmp_path, pkg):
         s.step(5)
         f0, g0 = s.read_state()
         best = s.autotune(3)
-        assert best['block'] in (64, 128, 256) and best['levels'] in (1, 2, 4) and best['window'] in (0, 1, 2, 4, 16)
+        assert best['block'] in (64, 128, 256) and best['levels'] in (1, 2, 4) and best['window'] in (-1, 0, 1, 2, 4, 16)
         assert s.launch == best
         f1, g1 = s.read_state()
         assert np.array_equal(f0, f1) and np.array_equal(g0, g1) and s.steps == 5
 
 
-@pytest.mark.parametrize('block,levels,window', [(256, 1, 1), (128, 2, 4), (64, 4, 0), (192, 1, 16), (256, 4, 2)])
+@pytest.mark.parametrize('block,levels,window', [(256, 1, 1), (128, 2, 4), (64, 4, 0), (192, 1, 16), (256, 4, 2),
+                                                  (256, 1, -1), (64, 2, -1)])
 def test_launch_shapes_give_identical_states(tmp_path, pkg, block, levels, window):
-    """every launch shape updates every cell with the same arithmetic: states are bit-identical to the default"""
+    """every launch shape updates every cell with the same arithmetic: states and FP64 means are bit-identical to the
+    default"""
     dom = domains(tmp_path)[2]
     with pkg.Solver.from_dump(dom) as a, pkg.Solver.from_dump(dom) as b:
         b.set_launch(block, levels, window)
-        a.step(13)
-        b.step(13)
+        for s in (a, b):
+            s.step(6)
+            s.mean_reset()
+            s.step(7, acc=True)
         fa, ga = a.read_state()
         fb, gb = b.read_state()
         assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32))
         assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32))
+        (sa, ba), (sb, bb) = a.read_mean(), b.read_mean()
+        assert np.array_equal(sa, sb) and ba == bb
