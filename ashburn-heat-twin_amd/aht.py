@@ -88,6 +88,7 @@ SIGNATURES = {
     'aht_read_surface': (ctypes.c_int, [_vp, _f32, _vp, _vp, _vp]),
     'aht_read_volume': (ctypes.c_int, [_vp, _i32, _f32, _vp, ctypes.POINTER(_f32)]),
     'aht_stats': (ctypes.c_int, [_vp, ctypes.POINTER(Diag)]),
+    'aht_stats_noalloc': (ctypes.c_int, [_vp, ctypes.POINTER(Diag)]),
     'aht_read_out': (ctypes.c_int, [_vp, _vp]),
     'aht_mean_reset': (ctypes.c_int, [_vp]),
     'aht_read_mean': (ctypes.c_int, [_vp, _vp, ctypes.POINTER(Budget)]),
@@ -272,6 +273,12 @@ class Solver:
     def stats(self):
         d = Diag()
         self._check(lib().aht_stats(self.h, ctypes.byref(d)))
+        return {k: getattr(d, k) for k, _ in Diag._fields_}
+
+    def stats_noalloc(self):
+        """stats() through the allocation-free entry point (library-owned scratch, no device allocation)"""
+        d = Diag()
+        self._check(lib().aht_stats_noalloc(self.h, ctypes.byref(d)))
         return {k: getattr(d, k) for k, _ in Diag._fields_}
 
     def read_out(self):

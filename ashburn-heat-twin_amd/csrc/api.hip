@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -462,6 +463,62 @@ int aht_stats(aht_solver *s, aht_diag *out) {
   HIP_TRY(s, aht::launch_stats(s->storage, s->g, s->f[p], s->gq[p], s->stat_part, s->red_out, s->st));
   double r[8];
   HIP_TRY(s, hipMemcpyAsync(r, s->red_out, 7 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  out->sum_theta = r[0]; out->sum_ke = r[1]; out->sum_drho = r[2]; out->n_fluid = r[3];
+  out->theta_max = r[4]; out->umax = r[5]; out->non_finite = (int64_t)r[6];
+  return AHT_OK;
+}
+
+}  // extern "C"
+
+// Scratch of aht_stats_noalloc: the partials of the diagnostics (7 per stats block, at most STATS_MAX_BLOCKS blocks)
+// and the 7 results after them, 131,128 bytes held by every GPU the library runs on until the process ends. In host
+// code `scratch` names a host-side stand-in, not device memory: the device address comes from hipGetSymbolAddress,
+// once per device with that device current, and only that address is passed to kernels.
+constexpr size_t SCRATCH_DOUBLES = 7 * (size_t)aht::STATS_MAX_BLOCKS + 8;
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+double *scratch_dev[SCRATCH_DEVICES];   // device address of `scratch` per device ordinal (null: not looked up yet)
+std::mutex scratch_mu;
+
+// the device address of this device's scratch (s's device is current)
+int device_scratch(aht_solver *s, double **out) {
+  if (s->device < 0 || s->device >= SCRATCH_DEVICES)
+    return fail(s, AHT_EINVAL, "allocation-free diagnostics: device %d beyond the %d devices served", s->device,
+                SCRATCH_DEVICES);
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[s->device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(s, AHT_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", s->device, hipGetErrorString(e));
+    scratch_dev[s->device] = (double *)p;
+  }
+  *out = scratch_dev[s->device];
+  return AHT_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int aht_stats_noalloc(aht_solver *s, aht_diag *out) {
+  int rc = check_ready(s);
+  if (rc) return rc;
+  if (!out) return fail(s, AHT_EINVAL, "null stats");
+  if ((rc = set_device(s))) return rc;
+  const uint32_t nb = aht::stats_blocks(s->g);
+  if (7 * (size_t)nb + 8 > SCRATCH_DOUBLES)
+    return fail(s, AHT_EINVAL, "allocation-free diagnostics: %u partial blocks exceed the scratch's %u", nb,
+                aht::STATS_MAX_BLOCKS);
+  double *ptr = nullptr;
+  if ((rc = device_scratch(s, &ptr))) return rc;
+  double *res = ptr + 7 * (size_t)aht::STATS_MAX_BLOCKS;   // every partial and result is written before it is read
+  const int p = s->parity;
+  HIP_TRY(s, aht::launch_stats(s->storage, s->g, s->f[p], s->gq[p], ptr, res, s->st));
+  double r[8];
+  HIP_TRY(s, hipMemcpyAsync(r, res, 7 * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(s, hipStreamSynchronize(s->st));
   out->sum_theta = r[0]; out->sum_ke = r[1]; out->sum_drho = r[2]; out->n_fluid = r[3];
   out->theta_max = r[4]; out->umax = r[5]; out->non_finite = (int64_t)r[6];
