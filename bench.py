@@ -13,6 +13,9 @@ f32 populations), 8 TB/s MI355X peak.
 
 The CPU baseline times the oracle (a C restatement of the reference kernel, OpenMP over all host cores) on a bounded
 sample of the same workload: whole steps on the same 600 x 600 x 64 domain for about --cpu-seconds.
+Also measured in the same run: the f16 deviation storage on the same domain (`reduced_precision`) and the serving
+half of the metric (`serving`: --serve-requests requests at --serve req/s through the Node serving loop, continuous
+batching against one-at-a-time execution; about three minutes).
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -63,23 +66,47 @@ def traffic_from_profiles(kernel_tag):
         return None
 
 
-def serving(rate):
-    """the serving half of the metric (scenarios/hour, p50/p99 latency): measured live by the Node serving loop when
-    rate > 0 (16 requests, batch and sequential, a few minutes), else the committed record of profiles/serving.json"""
-    if rate > 0:
-        out = os.path.join(tempfile.mkdtemp(prefix='aht_serve_'), 'serve.json')
-        subprocess.run(['node', os.path.join(PKG, 'js', 'serve.js'), '--rate', str(rate), '--requests', '16',
-                        '--mode', 'batch,sequential', '--out', out], check=True, stdout=subprocess.DEVNULL)
-        d = json.load(open(out))
-        subprocess.run(['rm', '-rf', os.path.dirname(out)])
-        return {'measured_in_this_run': True, 'rate_req_s': rate,
-                'batch': d['batch']['summary'], 'sequential': d['sequential']['summary']}
-    p = os.path.join(ROOT, 'profiles', 'serving.json')
-    if not os.path.exists(p):
+def serving(rate, requests):
+    """the serving half of the metric (scenarios/hour, p50/p99 latency): measured live by the Node serving loop
+    (js/serve.js: Poisson arrivals at `rate`, the serving grid mix, 2,200 s spin-up + 10-minute mean per request,
+    continuous batching against the same requests one at a time); rate 0 skips it"""
+    if rate <= 0:
         return None
-    d = json.load(open(p))['summaries']
-    return {'measured_in_this_run': False, 'source': 'profiles/serving_c5.md', 'rate_req_s': 5,
-            'batch': d.get('batch_r5'), 'sequential': d.get('sequential_r5')}
+    d = tempfile.mkdtemp(prefix='aht_serve_')
+    out = os.path.join(d, 'serve.json')
+    t0 = time.perf_counter()
+    subprocess.run(['node', os.path.join(PKG, 'js', 'serve.js'), '--rate', str(rate), '--requests', str(requests),
+                    '--mode', 'batch,sequential', '--out', out], check=True, stdout=subprocess.DEVNULL)
+    r = json.load(open(out))
+    subprocess.run(['rm', '-rf', d])
+    b, q = r['batch']['summary'], r['sequential']['summary']
+    return {'measured_in_this_run': True, 'rate_req_s': rate, 'requests': requests,
+            'wall_s': round(time.perf_counter() - t0, 1),
+            'scenarios_per_hour': b['scenarios_per_hour'], 'latency_p50_s': b['latency_p50_s'],
+            'latency_p99_s': b['latency_p99_s'],
+            'vs_sequential': round(b['scenarios_per_hour'] / q['scenarios_per_hour'], 4),
+            'batch': b, 'sequential': q}
+
+
+def timed_step(m, dom, storage, steps, warmup, launch, tune_steps):
+    """event-timed steps of one storage format on the domain: (ms per step, launch shape, stats)"""
+    s = m.Solver.from_dump(dom, storage=storage)
+    try:
+        if launch:
+            b, l, w = (int(x) for x in launch.split(','))
+            s.set_launch(b, l, w)
+        else:
+            s.step(2)
+            log(f'{STORAGE[storage]}: autotuned launch shape', s.autotune(tune_steps))
+        s.step(warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        ms = s.benchmark(steps)          # HIP events on the solver's stream around exactly `steps` step launches
+        s.sync()
+        wall = time.perf_counter() - t0
+        return ms / steps, wall * 1e3 / steps, s.launch, s.stats()
+    finally:
+        s.close()
 
 
 def cpu_baseline(dom, seconds):
@@ -111,11 +138,13 @@ def main():
     ap.add_argument('--storage', type=int, default=0, help='0 f32, 1 f16 deviations, 2 bf16 deviations, 3 f64 twin')
     ap.add_argument('--launch', default=None, help='fixed launch shape block,levels,window (no tuning)')
     ap.add_argument('--tune-steps', type=int, default=12, help='steps per launch-shape candidate of the autotuner')
-    ap.add_argument('--cpu-seconds', type=float, default=12.0)
+    ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--dump', default=None, help='existing domain dump directory')
-    ap.add_argument('--serve', type=float, default=0.0,
-                    help='also measure the serving loop at this arrival rate (req/s); 0 = report the committed record')
+    ap.add_argument('--serve', type=float, default=5.0,
+                    help='also measure the serving loop at this arrival rate (req/s), batch and sequential; 0 = skip')
+    ap.add_argument('--serve-requests', type=int, default=16)
+    ap.add_argument('--f16', type=int, default=1, help='also time the f16 deviation storage on the same domain')
     a = ap.parse_args()
     if a.gpus != 1:
         print(f'bench.py: one GPU only (asked for --gpus {a.gpus}): scenarios are independent per GPU and a single '
@@ -136,28 +165,23 @@ def main():
     N = nx * ny * nz
     nf = int(sum(nz - int(k) for k in dom['ktop']))
     log(f'grid {nx}x{ny}x{nz} cells {N} fluid {nf}')
-    s = m.Solver.from_dump(dom, storage=a.storage)
-    if a.launch:
-        b, l, w = (int(x) for x in a.launch.split(','))
-        s.set_launch(b, l, w)
-    else:
-        s.step(2)
-        log('autotuned launch shape', s.autotune(a.tune_steps))
-    s.step(a.warmup)
-    s.sync()
-    t0 = time.perf_counter()
-    ms = s.benchmark(a.steps)          # HIP events on the solver's stream around exactly K step launches
-    s.sync()
-    wall = time.perf_counter() - t0
-    st = s.stats()
-    launch = s.launch
-    s.close()
-    ms_step = ms / a.steps
+    ms_step, wall_step, launch, st = timed_step(m, dom, a.storage, a.steps, a.warmup, a.launch, a.tune_steps)
+    ms = ms_step * a.steps
     mlups = N * a.steps / (ms * 1e3)
     bpc = BYTES_PER_CELL[a.storage]
     achieved = bpc * nf / (ms_step * 1e-3) / 1e9
-    log(f'{ms_step:.4f} ms/step (events), wall {wall * 1e3 / a.steps:.4f} ms/step, {mlups:.1f} MLUPS, '
+    log(f'{ms_step:.4f} ms/step (events), wall {wall_step:.4f} ms/step, {mlups:.1f} MLUPS, '
         f'{achieved:.1f} GB/s, nonfinite {st["non_finite"]}')
+    reduced = None
+    if a.f16 and a.storage == 0:
+        ms16, _, launch16, st16 = timed_step(m, dom, 1, a.steps, a.warmup, None, a.tune_steps)
+        ach16 = BYTES_PER_CELL[1] * nf / (ms16 * 1e-3) / 1e9
+        reduced = {'storage': 'f16 deviations, f32 arithmetic', 'ms_per_step': round(ms16, 5),
+                   'value': round(N / (ms16 * 1e3), 2), 'unit': 'MLUPS', 'speedup_vs_f32': round(ms_step / ms16, 3),
+                   'roofline': {'bound': 'hbm', 'achieved': round(ach16, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                                'frac': round(ach16 / HBM_PEAK_GBS, 4), 'bytes_per_fluid_cell': BYTES_PER_CELL[1]},
+                   'launch': launch16, 'non_finite': st16['non_finite']}
+        log(f'f16: {ms16:.4f} ms/step, {ms_step / ms16:.3f}x f32')
     kernel_tag = f'step_{STORAGE[a.storage]}_{a.workload}'
     traffic = traffic_from_profiles(kernel_tag)
     cpu = None
@@ -167,6 +191,11 @@ def main():
         log('cpu baseline', cpu)
     if tmp:
         subprocess.run(['rm', '-rf', tmp])
+    serve = None
+    if a.serve > 0:
+        log(f'serving sample: {a.serve_requests} requests at {a.serve} req/s, batch and sequential ...')
+        serve = serving(a.serve, a.serve_requests)
+        log('serving', {k: serve[k] for k in ('scenarios_per_hour', 'latency_p50_s', 'latency_p99_s', 'vs_sequential')})
     out = {
         'metric': METRIC, 'value': round(mlups, 2), 'unit': 'MLUPS', 'n_gpus': 1, 'steps': a.steps,
         'warmup': a.warmup, 'ms_per_step': round(ms_step, 5), 'higher_is_better': True, 'scaling': 'weak',
@@ -178,11 +207,14 @@ def main():
                    'cells': N, 'fluid_cells': nf, 'bytes_per_fluid_cell': bpc, 'dt_s': dom['dt'],
                    'storage': STORAGE[a.storage], 'launch': launch, 'parallelism': 'single GPU'},
         'fluid_mlups': round(nf * a.steps / (ms * 1e3), 2),
-        'wall_ms_per_step': round(wall * 1e3 / a.steps, 5),
+        'wall_ms_per_step': round(wall_step, 5),
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic},
+                     'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
+                     'traffic_source': 'profile: rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (copy-calibrated) per '
+                                       'launch, profiles/traffic.json; not measured in this run'},
         'cpu_baseline': cpu,
-        'serving': serving(a.serve),
+        'reduced_precision': reduced,
+        'serving': serve,
     }
     print(json.dumps(out), flush=True)
 
