@@ -21,8 +21,8 @@ LIB_PATH = os.environ.get('AHT_LIB') or os.path.join(LIB_DIR, 'libaht.so')
 ROOT = os.path.dirname(HERE)
 
 AHT_OK, AHT_EINVAL, AHT_ENOMEM, AHT_EDEVICE, AHT_ESTATE = 0, -1, -2, -3, -4
-STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
-STORAGE_NAMES = {'f32': 0, 'f16': 1, 'bf16': 2, 'f64': 3}
+STORE_F32, STORE_F16, STORE_BF16, STORE_F64, STORE_F16S = 0, 1, 2, 3, 4
+STORAGE_NAMES = {'f32': 0, 'f16': 1, 'bf16': 2, 'f64': 3, 'f16s': 4}
 STEP_ACC, STEP_OUT = 1, 2
 FACE_W, FACE_E, FACE_S, FACE_N = 0, 1, 2, 3
 RHO_CP = 1.16 * 1005.0

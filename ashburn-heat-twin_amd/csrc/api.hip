@@ -168,7 +168,7 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   const aht_dims d = *dims;
   if (d.nx < 2 || d.ny < 2 || d.nz < 2 || d.nz > 1022) return AHT_EINVAL;
   const uint64_t N = (uint64_t)d.nx * d.ny * d.nz;
-  if (d.storage < AHT_STORE_F32 || d.storage > AHT_STORE_F64) return AHT_EINVAL;
+  if (d.storage < AHT_STORE_F32 || d.storage > AHT_STORE_F16S) return AHT_EINVAL;
   if (d.layout != AHT_LAYOUT_AUTO && d.layout != AHT_LAYOUT_QCELL) return AHT_EINVAL;
   // byte offsets within one population plane and within a D3Q7 lattice (theta plane) stay 32-bit
   if (N * (uint64_t)aht::storage_bytes(d.storage) >= (1ull << 32)) return AHT_EINVAL;
@@ -389,7 +389,7 @@ int aht_step(aht_solver *s, int32_t n, uint32_t flags) {
     float *o4 = (out && it == n - 1) ? s->out4 : nullptr;
     if (o4) HIP_TRY(s, hipMemsetAsync(o4, 0, 4 * (size_t)s->g.N * sizeof(float), s->st));
     HIP_TRY(s, aht::launch_step(s->storage, s->g, &s->gdev->g, &s->gdev->s[p], s->f[p], s->f[1 - p], s->gq[p],
-                                s->gq[1 - p], acc ? &a : nullptr, o4, s->block, s->levels, s->st));
+                                s->gq[1 - p], acc ? &a : nullptr, o4, s->block, s->levels, (uint32_t)s->steps, s->st));
     s->parity ^= 1;
     s->steps++;
     if (acc) s->n_acc++;
@@ -766,7 +766,7 @@ void member_tiles(const aht_solver *s, int block, aht::Member &M) {
 int aht_batch_create(int32_t device, int32_t storage, aht_batch **out) {
   if (!out) return AHT_EINVAL;
   *out = nullptr;
-  if (storage < AHT_STORE_F32 || storage > AHT_STORE_F64) return AHT_EINVAL;
+  if (storage < AHT_STORE_F32 || storage > AHT_STORE_F16S) return AHT_EINVAL;
   aht_batch *b = new aht_batch();
   *out = b;
   b->device = device;
@@ -901,6 +901,7 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
     M.parity0 = s->parity;
     M.nsteps = nsteps ? nsteps[d] : n;
     M.acc_from = acc_from ? acc_from[d] : n;
+    M.step0 = (uint32_t)s->steps;
     // the batch stream waits for work the member enqueued on its own stream
     HIP_TRYB(b, hipEventRecord(s->own, s->st));
     HIP_TRYB(b, hipStreamWaitEvent(b->st, s->own, 0));

@@ -64,6 +64,7 @@ struct Member {
   int parity0;         // lattice holding the state at launch index 0 of the chunk
   int nsteps;          // launches of the chunk in which this member steps (the table is ordered by it, descending)
   int acc_from;        // first launch index that accumulates the FP64 means (>= nsteps: none)
+  uint32_t step0;      // the member's step index at launch index 0 of the chunk
 };
 
 size_t face_offset(int face, int nx, int ny, int nz);   // offset of face 0..3 in Acc::face
@@ -86,8 +87,10 @@ hipError_t launch_zero_g_blocked(int storage, const Grid &g, void *gq, hipStream
 // gd = device copy of g: the kernel reads the grid through a pointer, so the compiler loads fields where they are used
 // (as a by-value argument the grid was pinned in SGPRs, which spilled into VGPR lanes: ~120 v_readlane/v_writelane
 // per cell of the f32 fast path); sd = device copy of make_streams(storage, g, fA, fB, gA, gB)
+// step = the solver's step index of this launch (random bits of the stochastic-rounding formats)
 hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const Streams *sd, const void *fA, void *fB,
-                       const void *gA, void *gB, const Acc *acc, float *out4, int block, int levels, hipStream_t st);
+                       const void *gA, void *gB, const Acc *acc, float *out4, int block, int levels, uint32_t step,
+                       hipStream_t st);
 Streams make_streams(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB);
 // one step of the first nmem batch members (launch index `it` of a chunk; their tiles are the first total_tiles) on
 // a grid of min(grid, total_tiles) blocks of `block` (64, 128 or 256) threads

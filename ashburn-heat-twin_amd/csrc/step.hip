@@ -134,10 +134,29 @@ __device__ __forceinline__ void *plane_base(const Streams *sp, int slot, void *l
   else return plane<S, Q>(lat, N, q);
 }
 
+// Random bits of the stochastic-rounding formats: two 32-bit words per cell and step from an integer mix of the cell
+// index and the step index (the cell's own bits in any launch shape or batch); value v (0..18 the D3Q19 populations,
+// 19..25 the D3Q7 ones) takes a rotation of one word, the words alternating between odd and even v so that the two
+// populations of an opposite pair get independent bits.
+struct Dither {
+  uint32_t h[2];
+  Dither() = default;
+  __device__ __forceinline__ Dither(uint32_t n, uint32_t step) {
+    uint32_t x = n * 0x9e3779b1u ^ step * 0x85ebca77u;
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    h[0] = x;
+    x ^= 0x68e31da4u; x *= 0xb5297a4du; x ^= x >> 15; x *= 0x1b56c4e9u; x ^= x >> 16;
+    h[1] = x;
+  }
+  __device__ __forceinline__ uint32_t bits(int v) const {
+    return __builtin_amdgcn_alignbit(h[v & 1], h[v & 1], (uint32_t)(5 * (v >> 1)) & 31u);
+  }
+};
+
 // the seven post-collision theta populations p of cell n: raw, or theta = sum_q p_q in f32 plus the deviations
 template <class S, bool TAB>
 __device__ __forceinline__ void store_g(const Grid &g, const Streams *sp, void *gB, uint32_t n,
-                                        const typename S::real (&p)[7]) {
+                                        const typename S::real (&p)[7], const Dither &dz) {
   using R = typename S::real;
   const uint32_t boff = cell_off<S, 7>(n);
   R th = R(0);
@@ -147,8 +166,12 @@ __device__ __forceinline__ void store_g(const Grid &g, const Streams *sp, void *
     else st_theta(gB, g.th_off + n * 4u, th);
   }
 #pragma unroll
-  for (int q = 0; q < 7; q++)
-    S::st_raw(plane_base<S, TAB, 7>(sp, 19 + q, gB, g.N, q), boff, S::enc_g(p[q], w7q(q), th));
+  for (int q = 0; q < 7; q++) {
+    typename S::elem e;
+    if constexpr (S::SR) e = S::enc_g_sr(p[q], w7q(q), th, dz.bits(19 + q));
+    else e = S::enc_g(p[q], w7q(q), th);
+    S::st_raw(plane_base<S, TAB, 7>(sp, 19 + q, gB, g.N, q), boff, e);
+  }
 }
 
 template <int q>
@@ -613,17 +636,23 @@ struct FaceFix<19> {
                                              const Outflow<typename S::real> &) {}
 };
 
+template <class S, int q>
+__device__ __forceinline__ typename S::elem enc_f_step(typename S::real v, const Dither &dz) {
+  if constexpr (S::SR) return S::enc_f_sr(v, w19<q>(), dz.bits(q));
+  else return S::enc_f(v, w19<q>());
+}
+
 template <int q>
 struct StorePairs {
   template <class S, bool TAB>
   __device__ __forceinline__ static void run(const Streams *sp, void *fB, size_t N, uint32_t boff,
                                              const Moments<typename S::real> &m, const Forces<typename S::real> &F,
-                                             const Relax<typename S::real> &R) {
+                                             const Relax<typename S::real> &R, const Dither &dz) {
     typename S::real a, b;
     collide_pair<q>(m, F, R, a, b);
-    S::st_raw(plane_base<S, TAB, 19>(sp, q, fB, N, q), boff, S::enc_f(a, w19<q>()));
-    S::st_raw(plane_base<S, TAB, 19>(sp, q + 1, fB, N, q + 1), boff, S::enc_f(b, w19<q + 1>()));
-    StorePairs<q + 2>::template run<S, TAB>(sp, fB, N, boff, m, F, R);
+    S::st_raw(plane_base<S, TAB, 19>(sp, q, fB, N, q), boff, enc_f_step<S, q>(a, dz));
+    S::st_raw(plane_base<S, TAB, 19>(sp, q + 1, fB, N, q + 1), boff, enc_f_step<S, q + 1>(b, dz));
+    StorePairs<q + 2>::template run<S, TAB>(sp, fB, N, boff, m, F, R, dz);
   }
 };
 template <>
@@ -631,7 +660,7 @@ struct StorePairs<19> {
   template <class S, bool TAB>
   __device__ __forceinline__ static void run(const Streams *, void *, size_t, uint32_t,
                                              const Moments<typename S::real> &, const Forces<typename S::real> &,
-                                             const Relax<typename S::real> &) {}
+                                             const Relax<typename S::real> &, const Dither &) {}
 };
 
 __device__ __forceinline__ void load_neighbours(const Grid &g, int i, int j, int kg, int (&kn)[3][3]) {
@@ -674,7 +703,7 @@ __device__ __forceinline__ void stream_cell(const Grid &g, const Streams *sp, co
 template <class S, bool ACC, bool OUT, bool TAB>
 __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *sp, const void *fA, void *fB,
                                                  const void *gA, void *gB, const Acc &acc, float *__restrict__ out4,
-                                                 int c, int k, uint32_t ck) {
+                                                 int c, int k, uint32_t ck, uint32_t step) {
   using R = typename S::real;
   const int ktc = col_kt(ck), kg = col_ktop(ck);
   const bool fluid = c < (int)g.NXY && k >= kg && k < g.nz;
@@ -703,8 +732,10 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *s
     const Forces<R> F = forces<true, true>(g.P, g.rp, g.nz, k, kg, ktc, R(cq.x), R(pr.x), R(pr.y), m);
     const Relax<R> Rx = relax(g.P, g.rp, g.nx, g.ny, i, j, R(pr.z), m, F);
     const uint32_t boff = cell_off<S, 19>(n);
-    S::st_raw(fB, boff, S::enc_f(collide_rest(Rx), w19<0>()));
-    StorePairs<1>::template run<S, TAB>(sp, fB, N, boff, m, F, Rx);
+    Dither dz;                                  // read by the stochastic-rounding formats only
+    if constexpr (S::SR) dz = Dither(n, step);
+    S::st_raw(fB, boff, enc_f_step<S, 0>(collide_rest(Rx), dz));
+    StorePairs<1>::template run<S, TAB>(sp, fB, N, boff, m, F, Rx, dz);
     // D3Q7: g^post = a g + b (1 + 4 c.u) + w S (1 - sg)
     const R Sq = (k == kg) ? R(cq.y) * R(g.P.srcscale) : R(0);
     const Scalar<R> sc = scalar_coeffs(m.th, Rx.omg, Sq, F.sg);
@@ -712,7 +743,7 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *s
     const R gp[7] = {sc.a * gg[0] + sc.cs0,          sc.a * gg[1] + (b1 + b4 * F.ux), sc.a * gg[2] + (b1 - b4 * F.ux),
                      sc.a * gg[3] + (b1 + b4 * F.uy), sc.a * gg[4] + (b1 - b4 * F.uy), sc.a * gg[5] + (b1 + b4 * F.uz),
                      sc.a * gg[6] + (b1 - b4 * F.uz)};
-    store_g<S, TAB>(g, sp, gB, n, gp);
+    store_g<S, TAB>(g, sp, gB, n, gp, dz);
     if (OUT) {
       out4[n] = (float)m.th;
       out4[N + n] = (float)F.ux;
@@ -777,7 +808,7 @@ template <class S, bool ACC, bool OUT, int KB>
 __global__ __launch_bounds__(256) void step_kernel(const Grid *__restrict__ gp, const Streams *__restrict__ sp,
                                                    const void *fA, void *fB, const void *gA, void *gB, Acc acc,
                                                    float *__restrict__ out4, uint32_t bx, float inv_bx, uint32_t sw,
-                                                   float inv_sw) {
+                                                   float inv_sw, uint32_t step) {
   const Grid &g = *gp;
   uint32_t cb, kk;
   if (sw) {
@@ -793,7 +824,7 @@ __global__ __launch_bounds__(256) void step_kernel(const Grid *__restrict__ gp, 
   const int k0 = g.kmin + (int)kk * KB;
   const uint32_t ck = ld_tab(g.colk, (size_t)(c < (int)g.NXY ? c : 0));
   for (int u = 0; u < KB; u++) {
-    const double th = step_cell_body<S, ACC, OUT, true>(g, sp, fA, fB, gA, gB, acc, out4, c, k0 + u, ck);
+    const double th = step_cell_body<S, ACC, OUT, true>(g, sp, fA, fB, gA, gB, acc, out4, c, k0 + u, ck, step);
     if (ACC) content_add(g, acc, k0 + u, c0, th);
   }
 }
@@ -808,13 +839,13 @@ __global__ __launch_bounds__(256) void step_kernel(const Grid *__restrict__ gp, 
 // b. Per-member fields make a member's result independent of the mix: it reads lattice parity0 ^ (it & 1) and
 // accumulates means from launch acc_from on. BS (64, 128, 256) and the grid are the batch tuner's choice.
 template <class S, bool ACC>
-__device__ __forceinline__ void batch_tile(const Member &M, int p, int c, int k0) {
+__device__ __forceinline__ void batch_tile(const Member &M, int p, int c, int k0, uint32_t step) {
   const Grid &g = M.g;
   const uint32_t ck = ld_tab(g.colk, (size_t)(c < (int)g.NXY ? c : 0));
   const int c0 = c - (int)(threadIdx.x & 63);
   for (int u = 0; u < M.levels; u++) {
     const double th = step_cell_body<S, ACC, false, false>(g, nullptr, M.f[p], M.f[1 - p], M.gq[p], M.gq[1 - p],
-                                                           M.acc, nullptr, c, k0 + u, ck);
+                                                           M.acc, nullptr, c, k0 + u, ck, step);
     if (ACC) content_add(g, M.acc, k0 + u, c0, th);
   }
 }
@@ -844,8 +875,9 @@ __global__ __launch_bounds__(BS) void batch_step_kernel(const Member *__restrict
     const int c = (int)(cb * BS + threadIdx.x);
     const int k0 = M.g.kmin + (int)kk * M.levels;
     const int p = M.parity0 ^ (it & 1);
-    if (it >= M.acc_from) batch_tile<S, true>(M, p, c, k0);
-    else batch_tile<S, false>(M, p, c, k0);
+    const uint32_t step = M.step0 + (uint32_t)it;                 // the member's own step index
+    if (it >= M.acc_from) batch_tile<S, true>(M, p, c, k0, step);
+    else batch_tile<S, false>(M, p, c, k0, step);
   }
 }
 
@@ -1097,6 +1129,7 @@ uint32_t content_slots(uint32_t nxy, int nz) { return ((nxy + 63u) >> 6) * (uint
 
 int storage_bytes(int storage) { return storage == 0 ? 4 : (storage == 3 ? 8 : 2); }
 bool storage_theta_ref(int storage) { return storage == 1 || storage == 2; }
+static bool storage_16bit(int storage) { return storage == 1 || storage == 2 || storage == 4; }
 size_t theta_offset(int storage, int lb, size_t N) {
   if (!storage_theta_ref(storage)) return 0;
   if (lb)   // after the blocked pair's array, from the D3Q7 base (19 populations into it)
@@ -1117,7 +1150,7 @@ size_t g_lattice_bytes(int storage, int lb, size_t N) {
 }
 int population_layout(int storage, size_t N) {
   // blocked when every byte offset of the pair's array (and theta plane) fits 32 bits (cell_off), else [q][cell]
-  const int lb = storage == 0 ? StoreF32B::LB : (storage == 1 || storage == 2 ? StoreF16B::LB : 0);
+  const int lb = storage == 0 ? StoreF32B::LB : (storage_16bit(storage) ? StoreF16B::LB : 0);
   return lb && f_lattice_bytes(storage, lb, N) < ((size_t)1 << 32) ? lb : 0;
 }
 size_t g_lattice_offset(int storage, int lb) {
@@ -1144,31 +1177,35 @@ __global__ __launch_bounds__(256) void zero_g_blocked_kernel(typename S::elem *g
       if (lb) { using S = StoreBF16B; CALL; } else { using S = StoreBF16; CALL; }      \
       break;                                                                           \
     case 3: { using S = StoreF64; CALL; } break;                                        \
+    case 4:                                                                            \
+      if (lb) { using S = StoreF16SB; CALL; } else { using S = StoreF16S; CALL; }      \
+      break;                                                                           \
     default: return hipErrorInvalidValue;                                              \
   }
 
 template <class S, int KB>
 static void launch_step_kb(const Grid &g, const Grid *gd, const Streams *sd, const void *fA, void *fB, const void *gA,
-                           void *gB, const Acc *acc, float *out4, int block, hipStream_t st) {
+                           void *gB, const Acc *acc, float *out4, int block, uint32_t step, hipStream_t st) {
   const uint32_t bx = (g.NXY + block - 1) / block;
   const uint32_t nk = (uint32_t)(g.nz - g.kmin + KB - 1) / KB;
   const uint32_t sw = g.win == WIN_PLANE_SLABS ? (bx + NUM_XCD - 1) / NUM_XCD : 0;   // plane slabs: blocks per slab
   const dim3 grid(sw ? NUM_XCD * sw * nk : bx * nk);
   const Acc a = acc ? *acc : Acc{nullptr, nullptr, nullptr};
   const float ib = 1.0f / (float)bx, isw = sw ? 1.0f / (float)sw : 0.0f;
-  if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw);
-  else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw);
-  else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw);
-  else hipLaunchKernelGGL((step_kernel<S, false, false, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw);
+  if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw, step);
+  else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw, step);
+  else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw, step);
+  else hipLaunchKernelGGL((step_kernel<S, false, false, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw, step);
 }
 
 hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const Streams *sd, const void *fA, void *fB,
-                       const void *gA, void *gB, const Acc *acc, float *out4, int block, int levels, hipStream_t st) {
+                       const void *gA, void *gB, const Acc *acc, float *out4, int block, int levels, uint32_t step,
+                       hipStream_t st) {
   if (g.kmin >= g.nz) return hipSuccess;
   AHT_DISPATCH(storage, g.lb, {
-    if (levels == 4) (launch_step_kb<S, 4>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, st);
-    else if (levels == 2) (launch_step_kb<S, 2>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, st);
-    else (launch_step_kb<S, 1>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, st);
+    if (levels == 4) (launch_step_kb<S, 4>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, step, st);
+    else if (levels == 2) (launch_step_kb<S, 2>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, step, st);
+    else (launch_step_kb<S, 1>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, step, st);
   })
   return hipGetLastError();
 }

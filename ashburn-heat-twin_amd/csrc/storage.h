@@ -7,8 +7,17 @@
 //          (stored as g_q * 2^10, the rounding of g alone biased the 10-minute mean warming by 1.5e-3 at 1200 x 1200
 //          x 96, while the f deviations contributed 8e-5). f32 arithmetic. The power-of-two scale (exact) keeps
 //          deviations down to ~6e-8 out of the subnormal range and leaves |deviation| <= 64 representable.
-//   BF16 : the same layout with bfloat16 deviations (round to nearest even), f32 arithmetic.
+//   BF16 : the same layout with bfloat16 deviations, f32 arithmetic, stochastic rounding in the step (below).
+//   F16S : f as F16, the D3Q7 lattice as g_q * 2^10 in binary16 with no theta plane (104 B per update instead of
+//          112), both with stochastic rounding in the step.
 //   F64  : f and g raw in f64 with f64 arithmetic: the FP64 twin used as the reference of reduced-precision errors.
+//
+// Stochastic rounding (SR = true): the step's stores round each value up or down to a neighbouring 16-bit value with
+// probabilities given by its position between them, from random bits that depend only on the cell, the population
+// and the step index (so a cell gives the same bits in any launch shape and in a batch). Unbiased rounding keeps the
+// slowly growing heat content and the small per-step increments of the populations from stalling at a fixed
+// 16-bit value, which round-to-nearest does: it is what limits bf16 (8-bit mantissa) and g stored without its theta
+// reference. Initial states, re-initialised cells and written states round to nearest.
 //
 // D3Q7 lattice of a theta-referenced format (theta_ref): [7][N] deviations, then (at byte offset theta_offset(N),
 // 256-aligned) [N] f32 theta = sum_q g_q of the stored post-collision populations.
@@ -44,6 +53,19 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float x) {
   return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);   // round to nearest even (finite inputs)
 }
 
+// stochastic rounding of f32 x to bfloat16: r's low 16 bits are added to the magnitude bits, which are then truncated
+// (the bit patterns of one sign are ordered like their values, so the carry rounds up with probability equal to the
+// dropped fraction)
+__device__ __forceinline__ uint16_t f32_to_bf16_sr(float x, uint32_t r) {
+  return (uint16_t)((__float_as_uint(x) + (r & 0xffffu)) >> 16);
+}
+// stochastic rounding of f32 x to binary16: 13 random bits below the binary16 mantissa, then conversion toward zero
+// (exact for the normal binary16 range; values in its subnormal range, below 6.1e-5, round toward zero more often)
+__device__ __forceinline__ _Float16 f32_to_f16_sr(float x, uint32_t r) {
+  const float y = __uint_as_float(__float_as_uint(x) + (r & 0x1fffu));
+  return (_Float16)__builtin_amdgcn_cvt_pkrtz(y, 0.0f)[0];
+}
+
 struct StoreF32 {
   using elem = float;
   using real = float;
@@ -51,6 +73,9 @@ struct StoreF32 {
   static constexpr int id = 0;
   static constexpr int LB = 0;              // population layout: 0 = [q][cell]; see lat_index
   static constexpr bool theta_ref = false;
+  static constexpr bool SR = false;         // stochastic rounding in the step's stores
+  __device__ static elem enc_f_sr(real v, float w, uint32_t) { return v; }
+  __device__ static elem enc_g_sr(real v, float, real, uint32_t) { return v; }
   __device__ static real dec_f(elem e, float) { return e; }
   __device__ static elem enc_f(real v, float) { return v; }
   __device__ static real dec_g(elem e, float, real) { return e; }
@@ -70,10 +95,13 @@ struct StoreF16 {
   static constexpr int id = 1;
   static constexpr int LB = 0;
   static constexpr bool theta_ref = true;
+  static constexpr bool SR = false;
   __device__ static real dec_f(elem e, float w) { return (float)e * DEV_INV + w; }
   __device__ static elem enc_f(real v, float w) { return (elem)((v - w) * DEV_SCALE); }
   __device__ static real dec_g(elem e, float w, real th) { return (float)e * DEV_INV + w * th; }
   __device__ static elem enc_g(real v, float w, real th) { return (elem)((v - w * th) * DEV_SCALE); }
+  __device__ static elem enc_f_sr(real v, float w, uint32_t) { return enc_f(v, w); }
+  __device__ static elem enc_g_sr(real v, float w, real th, uint32_t) { return enc_g(v, w, th); }
   __device__ static elem ld_raw(const void *base, uint32_t boff) {
     return __builtin_bit_cast(_Float16, (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc(base), boff, 0, AHT_LD_CPOL));
   }
@@ -93,16 +121,32 @@ struct StoreBF16 {
   static constexpr int id = 2;
   static constexpr int LB = 0;
   static constexpr bool theta_ref = true;
+  static constexpr bool SR = true;
   __device__ static real dec_f(elem e, float w) { return bf16_to_f32(e.bits) * DEV_INV + w; }
   __device__ static elem enc_f(real v, float w) { return bf16_t{f32_to_bf16((v - w) * DEV_SCALE)}; }
   __device__ static real dec_g(elem e, float w, real th) { return bf16_to_f32(e.bits) * DEV_INV + w * th; }
   __device__ static elem enc_g(real v, float w, real th) { return bf16_t{f32_to_bf16((v - w * th) * DEV_SCALE)}; }
+  __device__ static elem enc_f_sr(real v, float w, uint32_t r) { return bf16_t{f32_to_bf16_sr((v - w) * DEV_SCALE, r)}; }
+  __device__ static elem enc_g_sr(real v, float w, real th, uint32_t r) {
+    return bf16_t{f32_to_bf16_sr((v - w * th) * DEV_SCALE, r)};
+  }
   __device__ static elem ld_raw(const void *base, uint32_t boff) {
     return bf16_t{(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc(base), boff, 0, AHT_LD_CPOL)};
   }
   __device__ static void st_raw(void *base, uint32_t boff, elem v) {
     __builtin_amdgcn_raw_buffer_store_b16(v.bits, rsrc(base), boff, 0, AHT_ST_CPOL);
   }
+};
+
+// f deviations as StoreF16, the D3Q7 populations as g_q * 2^10 in binary16 (no theta plane), stochastic rounding
+struct StoreF16S : StoreF16 {
+  static constexpr int id = 4;
+  static constexpr bool theta_ref = false;
+  static constexpr bool SR = true;
+  __device__ static real dec_g(elem e, float, real) { return (float)e * DEV_INV; }
+  __device__ static elem enc_g(real v, float, real) { return (elem)(v * DEV_SCALE); }
+  __device__ static elem enc_f_sr(real v, float w, uint32_t r) { return f32_to_f16_sr((v - w) * DEV_SCALE, r); }
+  __device__ static elem enc_g_sr(real v, float, real, uint32_t r) { return f32_to_f16_sr(v * DEV_SCALE, r); }
 };
 
 struct StoreF64 {
@@ -112,6 +156,9 @@ struct StoreF64 {
   static constexpr int id = 3;
   static constexpr int LB = 0;
   static constexpr bool theta_ref = false;
+  static constexpr bool SR = false;
+  __device__ static elem enc_f_sr(real v, float, uint32_t) { return v; }
+  __device__ static elem enc_g_sr(real v, float, real, uint32_t) { return v; }
   __device__ static real dec_f(elem e, float) { return e; }
   __device__ static elem enc_f(real v, float) { return v; }
   __device__ static real dec_g(elem e, float, real) { return e; }
@@ -144,6 +191,10 @@ struct StoreF16B : StoreF16 {
   static constexpr int QB = 26;
 };
 struct StoreBF16B : StoreBF16 {
+  static constexpr int LB = 8;
+  static constexpr int QB = 26;
+};
+struct StoreF16SB : StoreF16S {
   static constexpr int LB = 8;
   static constexpr int QB = 26;
 };

@@ -25,11 +25,11 @@ function loadAddon() {
 }
 
 const STEP_ACC = 1, STEP_OUT = 2;
-const STORAGE = { f32: 0, f16: 1, bf16: 2 };
+const STORAGE = { f32: 0, f16: 1, bf16: 2, f16s: 4 };
 
 /** createLiveSolver (js/live/solver.js:44-101). opts: dx, u10Max, heightM, base (static10 directory),
  *  synthetic (generator spec), nz, alpha, nsp, nsw, outflowClip, existingFlux, met, sideInflowDeg, metReinit, dirTol,
- *  allowAnyDx, onProgress, device (HIP ordinal), storage ('f32' | 'f16' | 'bf16'), block, autotune (tuning steps
+ *  allowAnyDx, onProgress, device (HIP ordinal), storage ('f32' | 'f16' | 'bf16' | 'f16s'), block, autotune (tuning steps
  *  per launch-shape candidate, run once after the flow is initialised; 0 or absent = default shape) */
 async function createLiveSolver(opts) {
   const o = Object.assign({ dx: 30, u10Max: 8, heightM: 720, onProgress: () => {}, device: 0 }, opts || {});

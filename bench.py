@@ -13,7 +13,8 @@ f32 populations), 8 TB/s MI355X peak.
 
 The CPU baseline times the oracle (a C restatement of the reference kernel, OpenMP over all host cores) on a bounded
 sample of the same workload: whole steps on the same 600 x 600 x 64 domain for about --cpu-seconds.
-Also measured in the same run: the f16 deviation storage on the same domain (`reduced_precision`) and the serving
+Also measured in the same run: the 16-bit storage formats on the same domain (`reduced_precision`: f16 without theta
+plane and f16 with its f32 theta plane) and the serving
 half of the metric (`serving`: --serve-requests requests at --serve req/s through the Node serving loop, continuous
 batching against one-at-a-time execution; about three minutes).
 Prints one JSON line on stdout; progress goes to stderr.
@@ -31,8 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, 'ashburn-heat-twin_amd')
 METRIC = 'MLUPS & % HBM roofline at 600×600×64; scenarios/hour + p50/p99 latency'
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL = {0: 208, 1: 112, 2: 112, 3: 416}
-STORAGE = ['f32', 'f16', 'bf16', 'f64']
+BYTES_PER_CELL = {0: 208, 1: 112, 2: 112, 3: 416, 4: 104}
+STORAGE = ['f32', 'f16', 'bf16', 'f64', 'f16s']
 # published reference number for this metric and grid: numba.cuda prototype step at 600x600x64, 10 m, RTX 4090
 # (README.md:37 of the reference, data/manifest.json:9468)
 REFERENCE_MLUPS_600 = 950.0
@@ -135,7 +136,8 @@ def main():
     ap.add_argument('--steps', type=int, default=200)
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--workload', default='c2')
-    ap.add_argument('--storage', type=int, default=0, help='0 f32, 1 f16 deviations, 2 bf16 deviations, 3 f64 twin')
+    ap.add_argument('--storage', type=int, default=0,
+                    help='0 f32, 1 f16 deviations, 2 bf16 deviations, 3 f64 twin, 4 f16 without theta plane (SR)')
     ap.add_argument('--launch', default=None, help='fixed launch shape block,levels,window (no tuning)')
     ap.add_argument('--tune-steps', type=int, default=12, help='steps per launch-shape candidate of the autotuner')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
@@ -144,7 +146,8 @@ def main():
     ap.add_argument('--serve', type=float, default=5.0,
                     help='also measure the serving loop at this arrival rate (req/s), batch and sequential; 0 = skip')
     ap.add_argument('--serve-requests', type=int, default=16)
-    ap.add_argument('--f16', type=int, default=1, help='also time the f16 deviation storage on the same domain')
+    ap.add_argument('--f16', type=int, default=1,
+                    help='also time the 16-bit storage formats (f16s, f16) on the same domain')
     a = ap.parse_args()
     if a.gpus != 1:
         print(f'bench.py: one GPU only (asked for --gpus {a.gpus}): scenarios are independent per GPU and a single '
@@ -174,14 +177,19 @@ def main():
         f'{achieved:.1f} GB/s, nonfinite {st["non_finite"]}')
     reduced = None
     if a.f16 and a.storage == 0:
-        ms16, _, launch16, st16 = timed_step(m, dom, 1, a.steps, a.warmup, None, a.tune_steps)
-        ach16 = BYTES_PER_CELL[1] * nf / (ms16 * 1e-3) / 1e9
-        reduced = {'storage': 'f16 deviations, f32 arithmetic', 'ms_per_step': round(ms16, 5),
-                   'value': round(N / (ms16 * 1e3), 2), 'unit': 'MLUPS', 'speedup_vs_f32': round(ms_step / ms16, 3),
-                   'roofline': {'bound': 'hbm', 'achieved': round(ach16, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                                'frac': round(ach16 / HBM_PEAK_GBS, 4), 'bytes_per_fluid_cell': BYTES_PER_CELL[1]},
-                   'launch': launch16, 'non_finite': st16['non_finite']}
-        log(f'f16: {ms16:.4f} ms/step, {ms_step / ms16:.3f}x f32')
+        reduced = {}
+        for st16, name in ((4, 'f16 deviations, no theta plane, stochastic rounding; f32 arithmetic'),
+                           (1, 'f16 deviations with f32 theta plane; f32 arithmetic')):
+            ms16, _, launch16, s16 = timed_step(m, dom, st16, a.steps, a.warmup, None, a.tune_steps)
+            bpc16 = BYTES_PER_CELL[st16]
+            ach16 = bpc16 * nf / (ms16 * 1e-3) / 1e9
+            reduced[STORAGE[st16]] = {
+                'storage': name, 'ms_per_step': round(ms16, 5), 'value': round(N / (ms16 * 1e3), 2), 'unit': 'MLUPS',
+                'speedup_vs_f32': round(ms_step / ms16, 3),
+                'roofline': {'bound': 'hbm', 'achieved': round(ach16, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                             'frac': round(ach16 / HBM_PEAK_GBS, 4), 'bytes_per_fluid_cell': bpc16},
+                'launch': launch16, 'non_finite': s16['non_finite']}
+            log(f'{STORAGE[st16]}: {ms16:.4f} ms/step, {ms_step / ms16:.3f}x f32')
     kernel_tag = f'step_{STORAGE[a.storage]}_{a.workload}'
     traffic = traffic_from_profiles(kernel_tag)
     cpu = None
@@ -201,7 +209,8 @@ def main():
         'warmup': a.warmup, 'ms_per_step': round(ms_step, 5), 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': round(mlups / REFERENCE_MLUPS_600, 3) if (nx, ny, nz) == (600, 600, 64) else None,
         'baseline_note': 'vs the reference numba.cuda prototype, 950 MLUPS at 600x600x64 on an RTX 4090',
-        'dtype': ['f32', 'f16 storage / f32 arithmetic', 'bf16 storage / f32 arithmetic', 'f64'][a.storage],
+        'dtype': ['f32', 'f16 storage / f32 arithmetic', 'bf16 storage / f32 arithmetic', 'f64',
+                  'f16 storage (no theta plane) / f32 arithmetic'][a.storage],
         'data': 'synthetic (seeded generator: hills, box buildings, source roofs)',
         'config': {'workload': f'{a.workload}: {nx}x{ny}x{nz} at {dom["dx"]} m, fused D3Q19+D3Q7 step',
                    'cells': N, 'fluid_cells': nf, 'bytes_per_fluid_cell': bpc, 'dt_s': dom['dt'],

@@ -36,8 +36,10 @@ extern "C" {
 #define AHT_STORE_F32 0   /* f32 populations (208 B per fluid-cell update)                            */
 #define AHT_STORE_F16 1   /* f16 deviations (f_q - w_q)*2^10; theta in f32 + f16 (g_q - w_q theta)*2^10;
                              112 B per update, f32 math                                                */
-#define AHT_STORE_BF16 2  /* bf16 deviations, same layout as F16                                      */
+#define AHT_STORE_BF16 2  /* bf16 deviations, same layout as F16, stochastic rounding in the step      */
 #define AHT_STORE_F64 3   /* f64 populations and f64 arithmetic: the FP64 twin (416 B per update)     */
+#define AHT_STORE_F16S 4  /* f16 (f_q - w_q)*2^10 and g_q*2^10 (no theta plane), stochastic rounding in the
+                             step; 104 B per update, f32 math                                          */
 
 /* aht_step flags */
 #define AHT_STEP_ACC 1u   /* accumulate the FP64 time means (surface theta/u/v, face fluxes, heat content)  */

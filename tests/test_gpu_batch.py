@@ -33,10 +33,11 @@ def alone(pkg, dom, steps, acc_from, storage=0):
     return f, g, s3, b
 
 
-@pytest.mark.parametrize('storage', [0, 1, 2])
+@pytest.mark.parametrize('storage', [0, 1, 2, 4])
 def test_batch_matches_alone_bitwise(tmp_path, pkg, storage):
     """batched members (stream bases formed on the device, blocked layout of every format) give the bits of the
-    plain step (bases from the host-built table)"""
+    plain step (bases from the host-built table); for the stochastic-rounding formats (bf16, f16 without theta plane)
+    this also checks that a member's random bits follow its own step count"""
     doms = domains(tmp_path)
     steps = [37, 52, 45]
     acc_from = [20, 30, 0]
@@ -78,7 +79,7 @@ def test_batch_matches_alone_bitwise(tmp_path, pkg, storage):
             s.close()
 
 
-@pytest.mark.parametrize('storage', [0, 1, 2])
+@pytest.mark.parametrize('storage', [0, 1, 2, 4])
 def test_qcell_layout_matches_blocked_layout(tmp_path, pkg, storage):
     """Lattice pairs below 4 GiB are blocked on the device (f32 in 2048-cell blocks, f16 / bf16 in 256-cell blocks with
     the theta plane after the array) unless the reference's [q][cell] layout is asked for (the layout of larger
@@ -105,7 +106,7 @@ def test_qcell_layout_matches_blocked_layout(tmp_path, pkg, storage):
     assert a[3] == q[3] and a[7] == q[7]
 
 
-@pytest.mark.parametrize('storage', [0, 1, 2])
+@pytest.mark.parametrize('storage', [0, 1, 2, 4])
 def test_reset_heat_zeroes_theta_populations_only(tmp_path, pkg, storage):
     """aht_reset_heat sets every D3Q7 population (and the theta plane of the 16-bit formats) to zero and leaves the
     D3Q19 populations as they were, in the blocked layout (one array per lattice pair, the D3Q7 populations of each

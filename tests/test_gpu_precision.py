@@ -2,10 +2,11 @@
 
 * The FP64 twin (f64 storage and arithmetic) against the oracle's f64 build (`_f64`, the generator's expressions in
   double): the initial state is bit-identical, steps agree to relative L2 <= 1e-12 (reassociation only).
-* f16 / bf16 deviation storage (f32 arithmetic) against the FP64 twin on the same seeded domain: the state round
-  trip is exact to the format's resolution, and after 200 steps theta stays within the format's accuracy class
-  (relative L2 <= 2e-3 for f16, <= 2e-2 for bf16). The full-size error statement (1200 x 1200 x 96) is made by
-  ashburn-heat-twin_amd/precision.py, not here.
+* f16 / bf16 deviation storage and f16 without theta plane (f32 arithmetic) against the FP64 twin on the same seeded
+  domain: the state round trip is exact to the format's resolution, and after 200 steps theta stays within the
+  format's accuracy class (relative L2 <= 2e-3 for f16, <= 5e-3 for bf16, <= 2e-3 for f16 without theta plane; the
+  last two round stochastically in the step). The full-size statement (1200 x 1200 x 96) is made by
+  ashburn-heat-twin_amd/precision.py and test_gpu_configs.py.
 """
 import numpy as np
 import pytest
@@ -41,7 +42,7 @@ def test_fp64_twin_matches_f64_oracle(tmp_path, pkg, O, wdir, oclip):
         assert rel_l2(f[:, fl], o.f[:, fl]) <= 1e-11 and rel_l2(g[:, fl], o.g[:, fl]) <= 1e-11
 
 
-@pytest.mark.parametrize('storage,quant', [('f16', 2.0 ** -11), ('bf16', 2.0 ** -8)])
+@pytest.mark.parametrize('storage,quant', [('f16', 2.0 ** -11), ('bf16', 2.0 ** -8), ('f16s', 2.0 ** -11)])
 def test_deviation_storage_round_trip(tmp_path, pkg, O, storage, quant):
     dom = tiny(tmp_path)
     o = O.Oracle(dom, 'f32')
@@ -57,10 +58,13 @@ def test_deviation_storage_round_trip(tmp_path, pkg, O, storage, quant):
     assert np.all(np.abs(f - o.f) <= quant * np.abs(o.f - w) + 1e-7 * np.abs(o.f) + 1e-9)
     w7 = np.array([0.25] + [0.125] * 6, np.float32)[:, None]
     th = o.g.astype(np.float64).sum(0).astype(np.float32)
-    assert np.all(np.abs(g - o.g) <= quant * np.abs(o.g - w7 * th) + 2e-7 * np.abs(o.g) + 1e-9)
+    if storage == 'f16s':    # no theta plane: g_q * 2^10 itself in binary16
+        assert np.all(np.abs(g - o.g) <= quant * np.abs(o.g) + 1e-9)
+    else:
+        assert np.all(np.abs(g - o.g) <= quant * np.abs(o.g - w7 * th) + 2e-7 * np.abs(o.g) + 1e-9)
 
 
-@pytest.mark.parametrize('storage,tol', [('f16', 2e-3), ('bf16', 2e-2), ('f32', 1e-5)])
+@pytest.mark.parametrize('storage,tol', [('f16', 2e-3), ('bf16', 5e-3), ('f16s', 2e-3), ('f32', 1e-5)])
 def test_reduced_precision_against_fp64_twin(tmp_path, pkg, storage, tol):
     dom = tiny(tmp_path)
     with pkg.Solver.from_dump(dom, storage='f64') as ref, pkg.Solver.from_dump(dom, storage=storage) as s:
