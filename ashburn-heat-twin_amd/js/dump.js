@@ -86,6 +86,7 @@ function parseArgs(argv) {
 /** options from CLI flags:
  *  --workload c1|c2|c3                     synthetic benchmark workload
  *  --batch-member d[,seed]                  member d of the batched uneven-domain workload (handled in main)
+ *  --serve-request k[,seed]                 request k of the serving loop's request list (handled in main)
  *  --tiny nx,ny,nz,seed[,buildings,sources] small synthetic grid (dx from --dx, default 30)
  *  (default)                                the Ashburn static10 grid at --dx
  *  --u10 --dir --wstar --blh --ustar --flux --nsw --nsp --oclip --campus ABC --u10max --alpha --side */
@@ -132,7 +133,15 @@ if (require.main === module) {
     process.exit(2);
   }
   let sc, source;
-  if (a['batch-member'] !== undefined) {
+  if (a['serve-request'] !== undefined) {
+    // request k of the serving loop's seeded request list (js/serve.js makeRequests at 5 req/s), as it builds it
+    const t = String(a['serve-request']).split(',').map(Number);
+    const S5 = require('./serve');
+    const reqs = S5.makeRequests({ rate: 5, requests: t[0] + 1, seed: t[1] === undefined ? 1 : t[1] });
+    const job = S5.buildScenario(reqs[t[0]], { spinupS: 2200, meanS: 600 });
+    sc = job.sc;
+    source = `serving request ${t[0]} (seed ${t[1] === undefined ? 1 : t[1]}): ${reqs[t[0]].grid.join('x')}, spin-up ${job.spin} + mean ${job.mean} steps`;
+  } else if (a['batch-member'] !== undefined) {
     const t = String(a['batch-member']).split(',').map(Number);
     const m = batchMember(t[0], t[1]);
     sc = m.sc;

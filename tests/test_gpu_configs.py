@@ -17,8 +17,10 @@
   different levels per thread) and alone with the plain step, states bit-identical; a 100 x 100 x 32 member
   stepped in a batch against the oracle (50 steps from rest within 1e-5, then one step from the batch's own state:
   theta within 1e-5).
-* Serving: requests of the serving grid mix (100 x 100 x 32, 200 x 200 x 48, 600 x 600 x 64) served by continuous
-  batching, every result bit-identical to the same request run alone.
+* Serving: requests of the serving grid mix (100 x 100 x 32, 200 x 200 x 48, two 600 x 600 x 64) in one batch with the
+  serving loop's chunk pattern (large members step in the first launches of a chunk only while small ones are
+  active, averaging windows starting inside chunks, members sitting out whole chunks), states and FP64 means
+  bit-identical to each request run alone.
 """
 import json
 import os
@@ -199,6 +201,45 @@ def test_c4_member_in_batch_against_oracle(tmp_path, pkg, O):
     e1 = rel_l2(g1[:, fl].astype(np.float64).sum(0), o.g[:, fl].astype(np.float64).sum(0))
     print(f'c4 member 0 in a batch: 50 steps f {e50[0]:.2e} g {e50[1]:.2e}; one step theta {e1:.2e}')
     assert e50[0] <= 1e-5 and e50[1] <= 1e-5 and e1 <= 1e-5
+
+
+def test_c5_grid_mix_batched_equal_alone_c_abi(tmp_path, pkg):
+    reqs = (0, 4, 5, 7)                     # seed 3: 100x100x32, 200x200x48, 600x600x64, 600x600x64
+    doms = [dump_domain(tmp_path / f'r{k}', '--serve-request', f'{k},3') for k in reqs]
+    assert [(d['nx'], d['nz']) for d in doms] == [(100, 32), (200, 48), (600, 64), (600, 64)]
+    # chunks of 50 launches: (steps per member, first averaging launch per member; >= steps: none)
+    plan = [([50, 50, 7, 7], [50, 20, 50, 3]),
+            ([50, 30, 7, 7], [10, 0, 2, 0]),
+            ([13, 0, 50, 50], [0, 0, 0, 0])]
+    total = [sum(p[0][d] for p in plan) for d in range(4)]
+    acc_start = [60, 20, 9, 3]             # the step at which each member's averaging begins under this plan
+    sol = [pkg.Solver.from_dump(d) for d in doms]
+    with pkg.Batch() as b:
+        for s in sol:
+            s.mean_reset()
+            b.add(s)
+        b.set_grid(1 << 30)
+        for ns, af in plan:
+            b.step(50, ns, af)
+        b.sync()
+        got = []
+        for s in sol:
+            f, g = s.read_state()
+            got.append((f, g) + s.read_mean())
+            b.remove(s)
+    for s in sol:
+        s.close()
+    for d, dom in enumerate(doms):
+        with pkg.Solver.from_dump(dom) as s:
+            s.step(acc_start[d])
+            s.mean_reset()
+            s.step(total[d] - acc_start[d], acc=True)
+            f, g = s.read_state()
+            s3, bud = s.read_mean()
+        assert np.array_equal(f.view(np.uint32), got[d][0].view(np.uint32)), d
+        assert np.array_equal(g.view(np.uint32), got[d][1].view(np.uint32)), d
+        assert np.array_equal(s3, got[d][2]), d
+        assert bud == got[d][3] and bud['n_acc'] == total[d] - acc_start[d], d
 
 
 @pytest.mark.skip(reason='the node process of this case ended with SIGSEGV at exit after printing a correct result '
