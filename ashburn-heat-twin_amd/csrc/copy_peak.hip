@@ -17,10 +17,15 @@
 // 19-population and a 7-population lattice, each [cell / 1024][q][cell % 1024]; `lattice_pair_pull_nt` is the f32
 // step's own layout, both in one array [cell / 2048][19 + 7][cell % 2048]; `lattice16_pair_pull_nt` the same for
 // 2-byte elements in blocks of 256 cells, with the f32 theta array of the 16-bit formats pulled beside it.
+// `lattice16s_pair_pull_nt` is the f16s step's pattern (the 16-bit pair without theta array);
+// `lattice_colmajor_pull_nt` the f32 pair with column blocks outermost and levels inside them; `lattice_pair_aa_*` and
+// `lattice_pair_esotwist*` the in-place patterns in the pair layout.
 // Usage: aht_copy_peak [bytes] [reps] [variant]
 //   (variant: all | grid | flat | unroll4 | unroll4_xcd | unroll4_nt | lattice | lattice_pull | lattice_aa_even
 //    | lattice_aa_odd | lattice_pull_nt | lattice16_pull_nt | lattice_aosoa_pull_nt | lattice_pull_nt_x2
-//    | lattice_pull_nt_x4 | lattice_blocked_pull_nt | lattice_pair_pull_nt | lattice16_pair_pull_nt)
+//    | lattice_pull_nt_x4 | lattice_blocked_pull_nt | lattice_pair_pull_nt | lattice16_pair_pull_nt
+//    | lattice16s_pair_pull_nt | lattice_colmajor_pull_nt | lattice_pair_aa_even | lattice_pair_aa_odd
+//    | lattice_pair_esotwist | lattice_pair_aa_even_nt | lattice_pair_aa_odd_nt | lattice_pair_esotwist_nt)
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -137,6 +142,39 @@ __global__ __launch_bounds__(256) void copy_lattice_aa(float *__restrict__ a, ui
   }
 }
 
+// in-place single-lattice patterns in the f32 step's interleaved pair layout [cell >> LB][26][cell & (2^LB - 1)]:
+// AA (even step: own cell, read slot q, write slot opp(q); odd step: read slot opp(q) at n - c_q, write slot q at
+// n + c_q) and the esoteric twist (every step: population q read from and written back to the cell n + e_q, with
+// e_q = max(-c_q, 0) per component, i.e. the eight cells of the cube above n, the slot switching to opp(q)); interior
+// cells only take the offsets, so every access stays inside the lattice
+template <int LB, int KIND, bool NT>   // KIND 0: AA even, 1: AA odd, 2: esoteric twist
+__global__ __launch_bounds__(256) void copy_lattice_inplace_pair(float *__restrict__ a, uint32_t N, int nx, int ny,
+                                                                 int nz) {
+  const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  constexpr uint32_t M = (1u << LB) - 1;
+  const int nxy = nx * ny, k = (int)(n / nxy), c = (int)(n - (uint32_t)k * nxy), j = c / nx, i = c - j * nx;
+  const bool in = KIND != 0 && i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && k > 0 && k < nz - 1;
+  auto at = [&](int q, int64_t cell) { return ((((size_t)cell >> LB) * 26 + q) << LB) | ((size_t)cell & M); };
+  auto off = [&](int q) {
+    if (!in) return 0;
+    if (KIND == 1) return LCX[q] + LCY[q] * nx + LCZ[q] * nxy;
+    return (LCX[q] < 0 ? 1 : 0) + (LCY[q] < 0 ? nx : 0) + (LCZ[q] < 0 ? nxy : 0);
+  };
+  float v[26];
+#pragma unroll
+  for (int q = 0; q < 26; q++) {
+    const int d = off(q);
+    v[q] = a[at(KIND == 1 ? LOPP[q] : q, KIND == 1 ? (int64_t)n - d : (int64_t)n + d)];
+  }
+#pragma unroll
+  for (int q = 0; q < 26; q++) {
+    float *p = &a[at(KIND == 1 ? q : LOPP[q], (int64_t)n + off(q))];
+    if (NT) __builtin_nontemporal_store(v[q], p);
+    else *p = v[q];
+  }
+}
+
 // the f32 step's layout: D3Q19 lattice then D3Q7 lattice, each [cell >> LB][q][cell & (2^LB - 1)] (N a multiple of
 // 2^LB), pulled, non-temporal stores
 template <int LB>
@@ -181,9 +219,9 @@ __global__ __launch_bounds__(256) void copy_lattice_pair(const float *__restrict
     __builtin_nontemporal_store(v[q], &b[((((size_t)n >> LB) * 26 + q) << LB) | ((size_t)n & M)]);
 }
 
-// 16-bit populations interleaved as the f32 pair, [cell >> LB][26][cell & (2^LB - 1)], and an f32 theta array [cell]
-// pulled at the 7 D3Q7 offsets, non-temporal stores
-template <int LB>
+// 16-bit populations interleaved as the f32 pair, [cell >> LB][26][cell & (2^LB - 1)], and (THETA) an f32 theta
+// array [cell] pulled at the 7 D3Q7 offsets, non-temporal stores; without THETA the pattern of the f16s step
+template <int LB, bool THETA>
 __global__ __launch_bounds__(256) void copy_lattice16_pair(const uint16_t *__restrict__ a, uint16_t *__restrict__ b,
                                                            uint32_t N, int nx, int nxy) {
   const uint32_t n = blockIdx.x * 256 + threadIdx.x;
@@ -198,12 +236,35 @@ __global__ __launch_bounds__(256) void copy_lattice16_pair(const uint16_t *__res
     int64_t src = (int64_t)n - (LCX[q] + LCY[q] * nx + LCZ[q] * nxy);
     src = src < 0 ? 0 : (src >= (int64_t)N ? (int64_t)N - 1 : src);
     v[q] = a[((((size_t)src >> LB) * 26 + q) << LB) | ((size_t)src & M)];
-    if (q >= 19) th += ta[src];
+    if (THETA && q >= 19) th += ta[src];
   }
 #pragma unroll
   for (int q = 0; q < 26; q++)
     __builtin_nontemporal_store(v[q], &b[((((size_t)n >> LB) * 26 + q) << LB) | ((size_t)n & M)]);
-  __builtin_nontemporal_store(th, &tb[n]);
+  if (THETA) __builtin_nontemporal_store(th, &tb[n]);
+}
+
+// the f32 pair with column blocks outermost: element (q, level k, column c) at
+// ((((c >> LB) * nz + k) * 26 + q) << LB) | (c & (2^LB - 1)), so the levels of one column block are adjacent in memory;
+// thread blocks walk a column block through its levels (2^LB / 256 thread blocks per level) before the next one
+template <int LB>
+__global__ __launch_bounds__(256) void copy_lattice_colmajor(const float *__restrict__ a, float *__restrict__ b, int nx,
+                                                             int nxy, int nz) {
+  constexpr uint32_t M = (1u << LB) - 1, SUB = (1u << LB) / 256;
+  const uint32_t t = blockIdx.x, sub = t % SUB, kk = (t / SUB) % nz, cb = t / (SUB * nz);
+  const int c = (int)((cb << LB) + sub * 256 + threadIdx.x), k = (int)kk;
+  if (c >= nxy) return;
+  float v[26];
+#pragma unroll
+  for (int q = 0; q < 26; q++) {
+    int sc = c - (LCX[q] + LCY[q] * nx), sk = k - LCZ[q];
+    sc = sc < 0 ? 0 : (sc >= nxy ? nxy - 1 : sc);
+    sk = sk < 0 ? 0 : (sk >= nz ? nz - 1 : sk);
+    v[q] = a[(((((size_t)sc >> LB) * nz + sk) * 26 + q) << LB) | ((size_t)sc & M)];
+  }
+#pragma unroll
+  for (int q = 0; q < 26; q++)
+    __builtin_nontemporal_store(v[q], &b[(((((size_t)c >> LB) * nz + k) * 26 + q) << LB) | ((size_t)c & M)]);
 }
 
 // blocked layout: element (q, n) at ((n / 64) * 26 + q) * 64 + n % 64
@@ -265,17 +326,21 @@ int main(int argc, char **argv) {
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
-  const char *names[17] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
+  const char *names[25] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
                            "lattice_aa_even", "lattice_aa_odd", "lattice_pull_nt", "lattice16_pull_nt",
                            "lattice_aosoa_pull_nt", "lattice_pull_nt_x2", "lattice_pull_nt_x4",
-                           "lattice_blocked_pull_nt", "lattice_pair_pull_nt", "lattice16_pair_pull_nt"};
+                           "lattice_blocked_pull_nt", "lattice_pair_pull_nt", "lattice16_pair_pull_nt",
+                           "lattice16s_pair_pull_nt", "lattice_colmajor_pull_nt", "lattice_pair_aa_even",
+                           "lattice_pair_aa_odd", "lattice_pair_esotwist", "lattice_pair_aa_even_nt",
+                           "lattice_pair_aa_odd_nt", "lattice_pair_esotwist_nt"};
   const int lnx = 600, lnxy = 600 * 600;
   const uint32_t lN = 600u * 600u * 64u;   // lattice variants: 26 * lN floats each way (2.4 GB)
   double best = 0;
   const char *best_name = "";
-  for (int v = 0; v < 17; v++) {
+  const uint32_t ncb = (lnxy + 2047) / 2048;     // column blocks of the column-major variant (2.44 GB per buffer)
+  for (int v = 0; v < 25; v++) {
     if (strcmp(which, "all") && strcmp(which, names[v])) continue;
-    if (v >= 5 && n * 16 < (size_t)26 * lN * 4) continue;   // the lattice needs 2.4 GB per buffer
+    if (v >= 5 && n * 16 < (size_t)26 * ncb * 2048 * 64 * 4) continue;   // the lattices need 2.44 GB per buffer
     auto launch = [&]() {
       switch (v) {
         case 0: hipLaunchKernelGGL(copy_grid, dim3(cus * 32), dim3(256), 0, 0, a, b, n); break;
@@ -294,7 +359,15 @@ int main(int argc, char **argv) {
         case 13: hipLaunchKernelGGL(copy_lattice_wide<4>, dim3((lN / 4 + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 14: hipLaunchKernelGGL(copy_lattice_blocked<10>, dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 15: hipLaunchKernelGGL(copy_lattice_pair<11>, dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
-        case 16: hipLaunchKernelGGL(copy_lattice16_pair<8>, dim3((lN + 255) / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy); break;
+        case 16: hipLaunchKernelGGL((copy_lattice16_pair<8, true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy); break;
+        case 17: hipLaunchKernelGGL((copy_lattice16_pair<8, false>), dim3((lN + 255) / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy); break;
+        case 18: hipLaunchKernelGGL(copy_lattice_colmajor<11>, dim3(ncb * 64 * 8), dim3(256), 0, 0, (const float *)a, (float *)b, lnx, lnxy, 64); break;
+        case 19: hipLaunchKernelGGL((copy_lattice_inplace_pair<11, 0, false>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
+        case 20: hipLaunchKernelGGL((copy_lattice_inplace_pair<11, 1, false>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
+        case 21: hipLaunchKernelGGL((copy_lattice_inplace_pair<11, 2, false>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
+        case 22: hipLaunchKernelGGL((copy_lattice_inplace_pair<11, 0, true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
+        case 23: hipLaunchKernelGGL((copy_lattice_inplace_pair<11, 1, true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
+        case 24: hipLaunchKernelGGL((copy_lattice_inplace_pair<11, 2, true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
       }
     };
     launch();
@@ -305,7 +378,7 @@ int main(int argc, char **argv) {
     CK(hipEventSynchronize(e1));
     float ms = 0;
     CK(hipEventElapsedTime(&ms, e0, e1));
-    const size_t moved = (v == 10 || v == 16) ? (size_t)28 * lN * 2 : (v >= 5 ? (size_t)26 * lN * 4 : n * 16);
+    const size_t moved = (v == 10 || v == 16) ? (size_t)28 * lN * 2 : (v == 17 ? (size_t)26 * lN * 2 : (v >= 5 ? (size_t)26 * lN * 4 : n * 16));
     const double per = ms / reps, gbs = 2.0 * moved / (per * 1e-3) / 1e9;
     printf("{\"kernel\": \"copy_%s\", \"bytes_copied\": %zu, \"reps\": %d, \"ms_per_launch\": %.5f, "
            "\"GBps_read_plus_write\": %.1f, \"cus\": %d}\n", names[v], moved, reps, per, gbs, cus);
