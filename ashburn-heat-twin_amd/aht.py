@@ -41,12 +41,13 @@ class Dims(ctypes.Structure):
 
 
 class Launch(ctypes.Structure):
-    """launch shape of the step kernel: threads per block, cells per thread along z, tile window (1 = linear)"""
+    """launch shape of the step kernel: threads per block, cells per thread along z, tile window (1 = linear), and the
+    two-step sweep (0 = off; +kc: 32 x 8 column regions, -kc: 16 x 16, level chunks of kc)"""
     _fields_ = [('block', ctypes.c_int32), ('levels', ctypes.c_int32), ('window', ctypes.c_int32),
-                ('reserved', ctypes.c_int32)]
+                ('sweep', ctypes.c_int32)]
 
     def as_dict(self):
-        return {'block': self.block, 'levels': self.levels, 'window': self.window}
+        return {'block': self.block, 'levels': self.levels, 'window': self.window, 'sweep': self.sweep}
 
 
 PARAM_F = ('tau0', 'cs2smag', 'gbeta', 'prt', 'kfloor', 'momfac', 'srcscale', 'sigmax')
@@ -240,13 +241,14 @@ class Solver:
         return int(lib().aht_steps_done(self.h))
 
     def autotune(self, steps=20):
-        """the launch shape {block, levels, window} the autotuner keeps (now set); the state is left unchanged"""
+        """the launch shape {block, levels, window, sweep} the autotuner keeps (now set); the state is left unchanged"""
         best = Launch()
         self._check(lib().aht_autotune(self.h, int(steps), ctypes.byref(best)))
         return best.as_dict()
 
-    def set_launch(self, block=256, levels=1, window=1):
-        self._check(lib().aht_set_launch(self.h, ctypes.byref(Launch(int(block), int(levels), int(window), 0))))
+    def set_launch(self, block=256, levels=1, window=1, sweep=0):
+        self._check(lib().aht_set_launch(self.h, ctypes.byref(Launch(int(block), int(levels), int(window),
+                                                                     int(sweep)))))
 
     @property
     def launch(self):

@@ -502,10 +502,11 @@ static napi_value launch_obj(napi_env env, const aht_launch *l) {
   set_num(env, o, "block", l->block);
   set_num(env, o, "levels", l->levels);
   set_num(env, o, "window", l->window);
+  set_num(env, o, "sweep", l->sweep);
   return o;
 }
 
-/* autotune(h, steps) -> {block, levels, window}: times every launch-shape candidate for `steps` steps, keeps the
+/* autotune(h, steps) -> {block, levels, window, sweep}: times every launch-shape candidate for `steps` steps, keeps the
  * fastest, leaves the state and step counters as they were (blocks the calling thread for the tuning steps) */
 static napi_value js_autotune(napi_env env, napi_callback_info info) {
   napi_value argv[2];
@@ -519,7 +520,7 @@ static napi_value js_autotune(napi_env env, napi_callback_info info) {
   return launch_obj(env, &best);
 }
 
-/* setLaunch(h, {block, levels, window}); getLaunch(h) -> {block, levels, window} */
+/* setLaunch(h, {block, levels, window, sweep}); getLaunch(h) -> {block, levels, window, sweep} */
 static napi_value js_set_launch(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return NULL;
@@ -528,7 +529,7 @@ static napi_value js_set_launch(napi_env env, napi_callback_info info) {
   aht_launch l;
   memset(&l, 0, sizeof l);
   if (!get_prop_i32(env, argv[1], "block", &l.block, 256) || !get_prop_i32(env, argv[1], "levels", &l.levels, 1) ||
-      !get_prop_i32(env, argv[1], "window", &l.window, 1))
+      !get_prop_i32(env, argv[1], "window", &l.window, 1) || !get_prop_i32(env, argv[1], "sweep", &l.sweep, 0))
     return NULL;
   const int rc = aht_set_launch(h->s, &l);
   if (rc) return throw_rc(env, h->s, rc);

@@ -56,6 +56,7 @@ struct aht_solver {
   aht::Grid g{};
   int block = aht::STEP_BLOCK;
   int levels = 1;             // cells per thread along z
+  int sweep = 0;              // aht_launch::sweep: 0 = one step per launch, else the two-step sweep's chunk / region
   int storage = AHT_STORE_F32;
   int eb = 4;                 // bytes per stored population element
   size_t fbytes = 0;          // bytes of one D3Q19 lattice (a blocked f32 pair: the whole array, D3Q7 included)
@@ -384,8 +385,18 @@ int aht_step(aht_solver *s, int32_t n, uint32_t flags) {
   const bool acc = flags & AHT_STEP_ACC, out = flags & AHT_STEP_OUT;
   if (out && !s->out4) HIP_TRY(s, hipMalloc(&s->out4, 4 * (size_t)s->g.N * sizeof(float)));
   aht::Acc a{s->acc_surf, s->acc_face, s->acc_content};
+  const bool sweep = s->sweep != 0 && !acc && aht::sweep2_supported(s->storage, s->g.lb);
   for (int32_t it = 0; it < n; it++) {
     const int p = s->parity;
+    if (sweep && it + 1 < n && !(out && it + 1 == n - 1)) {   // steps it and it + 1 in one launch
+      HIP_TRY(s, aht::launch_sweep2(s->storage, s->g, &s->gdev->g, s->f[p], s->f[1 - p], s->gq[p], s->gq[1 - p],
+                                    s->sweep > 0 ? 32 : 16, s->sweep > 0 ? s->sweep : -s->sweep, s->g.win,
+                                    (uint32_t)s->steps, s->st));
+      s->parity ^= 1;
+      s->steps += 2;
+      it++;
+      continue;
+    }
     float *o4 = (out && it == n - 1) ? s->out4 : nullptr;
     if (o4) HIP_TRY(s, hipMemsetAsync(o4, 0, 4 * (size_t)s->g.N * sizeof(float), s->st));
     HIP_TRY(s, aht::launch_step(s->storage, s->g, &s->gdev->g, &s->gdev->s[p], s->f[p], s->f[1 - p], s->gq[p],
@@ -1004,11 +1015,13 @@ int check_launch(aht_solver *s, const aht_launch &l) {
   if (l.levels != 1 && l.levels != 2 && l.levels != 4) return fail(s, AHT_EINVAL, "levels must be 1, 2 or 4");
   if (l.window != -1 && (l.window < 0 || l.window > 4096 || (l.window & (l.window - 1))))
     return fail(s, AHT_EINVAL, "window must be -1 (plane slabs), 0 (slabs) or a power of two up to 4096");
+  if (l.sweep < -1023 || l.sweep > 1023) return fail(s, AHT_EINVAL, "sweep chunk must be at most 1023 levels");
   return AHT_OK;
 }
 void apply_launch(aht_solver *s, const aht_launch &l) {
   s->block = l.block;
   s->levels = l.levels;
+  s->sweep = l.sweep;
   s->g.win = (uint32_t)l.window;
   s->gdirty = true;
 }
@@ -1039,7 +1052,7 @@ int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best) {
     double ms;
   };
   std::vector<Cand> cand;
-  aht_launch keep{s->block, s->levels, (int32_t)s->g.win, 0};
+  aht_launch keep{s->block, s->levels, (int32_t)s->g.win, s->sweep};
   auto time_shape = [&](const aht_launch &l, int32_t n, double &ms) {
     apply_launch(s, l);
     int r = aht_step(s, 1, 0);
@@ -1090,7 +1103,7 @@ int aht_set_launch(aht_solver *s, const aht_launch *l) {
 
 int aht_get_launch(const aht_solver *s, aht_launch *l) {
   if (!s || !l) return AHT_EINVAL;
-  *l = aht_launch{s->block, s->levels, (int32_t)s->g.win, 0};
+  *l = aht_launch{s->block, s->levels, (int32_t)s->g.win, s->sweep};
   return AHT_OK;
 }
 

@@ -92,6 +92,13 @@ hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const Streams
                        const void *gA, void *gB, const Acc *acc, float *out4, int block, int levels, uint32_t step,
                        hipStream_t st);
 Streams make_streams(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB);
+// two steps per launch (the z sweep with LDS-staged halo planes; f32 pair layout only, no means or debug outputs):
+// regions of tx x (256 / tx) columns (tx = 16 or 32), chunks of kc levels, tiles in the XCD-aware order win (plane
+// slabs fall back to linear); step = the step index of the first of the two steps; reads pair A, writes pair B
+bool sweep2_supported(int storage, int lb);
+hipError_t launch_sweep2(int storage, const Grid &g, const Grid *gd, const void *fA, void *fB, const void *gA,
+                         void *gB, int tx, int kc, uint32_t win, uint32_t step, hipStream_t st);
+
 // one step of the first nmem batch members (launch index `it` of a chunk; their tiles are the first total_tiles) on
 // a grid of min(grid, total_tiles) blocks of `block` (64, 128 or 256) threads
 hipError_t launch_batch_step(int storage, int lb, int block, const Member *mem, int nmem, uint32_t total_tiles, int it,

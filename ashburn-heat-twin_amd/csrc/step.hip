@@ -190,20 +190,14 @@ struct OwnPops<19> {
                                              uint32_t) {}
 };
 
-// outflow-face prelude from the own cell's pre-stream populations (js/live/kernel.js:122-148)
-template <class S>
-__device__ __forceinline__ Outflow<typename S::real> outflow_prelude(const Grid &g,
-                                                                     const typename S::elem *__restrict__ fA, size_t N,
-                                                                     uint32_t n, int i, int j) {
-  using R = typename S::real;
+// outflow-face prelude from the own cell's pre-stream populations (js/live/kernel.js:122-148), once o.own holds them
+// (onb: the cell lies on an outlet face; otherwise o.own is not read)
+template <typename R>
+__device__ __forceinline__ void outflow_finish(const Grid &g, Outflow<R> &o, bool onb, int i, int j) {
   const Params &P = g.P;
-  Outflow<R> o;
   o.rn = R(1); o.uxn = R(0); o.uyn = R(0); o.uzn = R(0);
-  const bool onb = (i == 0 && P.inflow_w == 0) || (i == g.nx - 1 && P.inflow_e == 0) ||
-                   (j == 0 && P.inflow_s == 0) || (j == g.ny - 1 && P.inflow_n == 0);
   if (onb) {
-    R(&a)[19] = o.own;
-    OwnPops<0>::template run<S>(a, fA, N, n);
+    const R(&a)[19] = o.own;
     R r = a[0], sx = R(0), sy = R(0), sz = R(0);
 #pragma unroll
     for (int q = 1; q < 19; q++) {
@@ -225,6 +219,20 @@ __device__ __forceinline__ Outflow<typename S::real> outflow_prelude(const Grid 
   }
   o.uxc = uxc; o.uyc = uyc;
   o.usqc = uxc * uxc + uyc * uyc + o.uzn * o.uzn;
+}
+
+// the prelude with the own cell's populations loaded from lattice fA
+template <class S>
+__device__ __forceinline__ Outflow<typename S::real> outflow_prelude(const Grid &g,
+                                                                     const typename S::elem *__restrict__ fA, size_t N,
+                                                                     uint32_t n, int i, int j) {
+  using R = typename S::real;
+  const Params &P = g.P;
+  Outflow<R> o;
+  const bool onb = (i == 0 && P.inflow_w == 0) || (i == g.nx - 1 && P.inflow_e == 0) ||
+                   (j == 0 && P.inflow_s == 0) || (j == g.ny - 1 && P.inflow_n == 0);
+  if (onb) OwnPops<0>::template run<S>(o.own, fA, N, n);
+  outflow_finish(g, o, onb, i, j);
   return o;
 }
 
@@ -698,6 +706,40 @@ __device__ __forceinline__ void stream_cell(const Grid &g, const Streams *sp, co
   }
 }
 
+// D3Q7 collision (js/live/kernel.js:369-386): g^post = a g + b (1 + 4 c.u) + w S (1 - sg); src: the cell takes the
+// column's rooftop source (k == ktop), of strength cqy * srcscale
+template <typename R>
+__device__ __forceinline__ void theta_post(const Params &P, bool src, float cqy, const Moments<R> &m,
+                                           const Forces<R> &F, const Relax<R> &Rx, const R (&gg)[7], R (&gp)[7]) {
+  const R Sq = src ? R(cqy) * R(P.srcscale) : R(0);
+  const Scalar<R> sc = scalar_coeffs(m.th, Rx.omg, Sq, F.sg);
+  const R b1 = sc.b + sc.cs1, b4 = R(4) * sc.b;
+  gp[0] = sc.a * gg[0] + sc.cs0;
+  gp[1] = sc.a * gg[1] + (b1 + b4 * F.ux);
+  gp[2] = sc.a * gg[2] + (b1 - b4 * F.ux);
+  gp[3] = sc.a * gg[3] + (b1 + b4 * F.uy);
+  gp[4] = sc.a * gg[4] + (b1 - b4 * F.uy);
+  gp[5] = sc.a * gg[5] + (b1 + b4 * F.uz);
+  gp[6] = sc.a * gg[6] + (b1 - b4 * F.uz);
+}
+
+// collision of one cell from its moments, with the post-collision populations of both lattices stored to pair B
+template <class S, bool TAB>
+__device__ __forceinline__ void store_post(const Grid &g, const Streams *sp, void *fB, void *gB, uint32_t n, bool src,
+                                           float cqy, const Moments<typename S::real> &m,
+                                           const Forces<typename S::real> &F, const Relax<typename S::real> &Rx,
+                                           const typename S::real (&gg)[7], uint32_t step) {
+  using R = typename S::real;
+  const uint32_t boff = cell_off<S, 19>(n);
+  Dither dz;                                  // read by the stochastic-rounding formats only
+  if constexpr (S::SR) dz = Dither(n, step);
+  S::st_raw(fB, boff, enc_f_step<S, 0>(collide_rest(Rx), dz));
+  StorePairs<1>::template run<S, TAB>(sp, fB, g.N, boff, m, F, Rx, dz);
+  R gp[7];
+  theta_post(g.P, src, cqy, m, F, Rx, gg, gp);
+  store_g<S, TAB>(g, sp, gB, n, gp, dz);
+}
+
 // one cell of the fused step (genStepWGSL, js/live/kernel.js:395-431) with FP64 accumulation (ACC) and debug outputs
 // (OUT). Every thread of the block must call it (the ACC heat-content sum is a wavefront reduction).
 template <class S, bool ACC, bool OUT, bool TAB>
@@ -731,19 +773,7 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *s
     const Moments<R> m = moments(f, gg);
     const Forces<R> F = forces<true, true>(g.P, g.rp, g.nz, k, kg, ktc, R(cq.x), R(pr.x), R(pr.y), m);
     const Relax<R> Rx = relax(g.P, g.rp, g.nx, g.ny, i, j, R(pr.z), m, F);
-    const uint32_t boff = cell_off<S, 19>(n);
-    Dither dz;                                  // read by the stochastic-rounding formats only
-    if constexpr (S::SR) dz = Dither(n, step);
-    S::st_raw(fB, boff, enc_f_step<S, 0>(collide_rest(Rx), dz));
-    StorePairs<1>::template run<S, TAB>(sp, fB, N, boff, m, F, Rx, dz);
-    // D3Q7: g^post = a g + b (1 + 4 c.u) + w S (1 - sg)
-    const R Sq = (k == kg) ? R(cq.y) * R(g.P.srcscale) : R(0);
-    const Scalar<R> sc = scalar_coeffs(m.th, Rx.omg, Sq, F.sg);
-    const R b1 = sc.b + sc.cs1, b4 = R(4) * sc.b;
-    const R gp[7] = {sc.a * gg[0] + sc.cs0,          sc.a * gg[1] + (b1 + b4 * F.ux), sc.a * gg[2] + (b1 - b4 * F.ux),
-                     sc.a * gg[3] + (b1 + b4 * F.uy), sc.a * gg[4] + (b1 - b4 * F.uy), sc.a * gg[5] + (b1 + b4 * F.uz),
-                     sc.a * gg[6] + (b1 - b4 * F.uz)};
-    store_g<S, TAB>(g, sp, gB, n, gp, dz);
+    store_post<S, TAB>(g, sp, fB, gB, n, k == kg, cq.y, m, F, Rx, gg, step);
     if (OUT) {
       out4[n] = (float)m.th;
       out4[N + n] = (float)F.ux;
@@ -826,6 +856,304 @@ __global__ __launch_bounds__(256) void step_kernel(const Grid *__restrict__ gp, 
   for (int u = 0; u < KB; u++) {
     const double th = step_cell_body<S, ACC, OUT, true>(g, sp, fA, fB, gA, gB, acc, out4, c, k0 + u, ck, step);
     if (ACC) content_add(g, acc, k0 + u, c0, th);
+  }
+}
+
+// ---- two steps per launch: a z sweep with LDS-staged halo planes -----------------------------------------------
+// A workgroup of 256 threads owns a region of TX x TY columns, one column per thread, and walks it up through the
+// levels of a chunk. For each level it runs two phases:
+//   A: the first step (t -> t+1) of plane k+1 for every fluid cell of the region, streamed from lattice pair A exactly
+//      as the single step streams it. The post-collision populations stay on chip: the 20 that a lateral neighbour
+//      pulls go to LDS, grouped by the role the plane plays for the second step (12 pulled within the plane, 4 pulled
+//      upward from it, 4 downward), and the thread keeps the other 10 of its own cell in registers;
+//   B: the second step (t+1 -> t+2) of plane k for the (TX - 2) x (TY - 2) interior columns. Every pull reads LDS
+//      (lateral sources, and the own cell's values that live there) or the thread's registers, under the boundary
+//      path's rules for every cell (lid mirror, lateral faces, half-way bounce-back); collision and stores are the
+//      single step's, into lattice pair B.
+// The pulls of the next plane's first step are issued before phase B, so that their latency overlaps its arithmetic.
+// The intermediate state never reaches HBM (208 B per cell for two steps instead of 416, f32). The price is
+// recomputation: the halo ring of a region is also stepped by the neighbouring workgroup, and a chunk also steps the
+// plane below and the plane above it. The results are bit-identical to two single steps (same expressions, same
+// contraction, same pulled values). f32 pair layout only; neither FP64 means nor debug outputs (such steps take
+// the single-step kernel).
+constexpr int SWEEP_THREADS = 256;
+// slots of the lateral populations in the LDS planes (-1: not lateral)
+__device__ constexpr int SW_FLAT[19] = {-1, 0, 1, 2, 3, -1, -1, 4, 5, 6, 7, -1, -1, -1, -1, -1, -1, -1, -1};
+__device__ constexpr int SW_UP[19] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 0, -1, -1, 1, 2, -1, -1, 3};
+__device__ constexpr int SW_DOWN[19] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 0, 1, -1, -1, 2, 3, -1};
+// the own cell's D3Q19 populations that are kept in registers (the others live in the plane's LDS slots)
+__device__ constexpr int SW_REG[19] = {0, -1, -1, -1, -1, 1, 2, -1, -1, -1, -1, -1, 3, 4, -1, -1, 5, 6, -1};
+
+// plane rings: flat[p & 1] holds plane p's in-plane populations (8 D3Q19, then D3Q7 1..4), up[p % 3] the four D3Q19
+// populations with c_z = +1 and a lateral component, down the four with c_z = -1 of the newest plane only
+struct SweepLds {
+  float flat[2][12][SWEEP_THREADS];
+  float up[3][4][SWEEP_THREADS];
+  float down[4][SWEEP_THREADS];
+};
+
+// the own cell's post-collision populations that stay in registers: D3Q19 0, 5, 6, 12, 13, 16, 17 and D3Q7 0, 5, 6
+struct SweepOwn {
+  float f[7], g[3];
+};
+
+// where the own cell's populations of plane p live: its LDS slots (sf = p & 1, su = p % 3) and its registers
+struct OwnAt {
+  const SweepLds &L;
+  int t, sf, su;
+  const SweepOwn &r;
+  template <int q>
+  __device__ __forceinline__ float f() const {
+    if constexpr (SW_REG[q] >= 0) return r.f[SW_REG[q]];
+    else if constexpr (SW_FLAT[q] >= 0) return L.flat[sf][SW_FLAT[q]][t];
+    else return L.up[su][SW_UP[q]][t];
+  }
+  template <int q>
+  __device__ __forceinline__ float g() const {
+    if constexpr (q == 0) return r.g[0];
+    else if constexpr (q >= 5) return r.g[q - 4];
+    else return L.flat[sf][8 + q - 1][t];
+  }
+};
+
+template <int q>
+struct CollidePairs {
+  __device__ __forceinline__ static void run(const Moments<float> &m, const Forces<float> &F, const Relax<float> &Rx,
+                                             float (&fp)[19]) {
+    collide_pair<q>(m, F, Rx, fp[q], fp[q + 1]);
+    CollidePairs<q + 2>::run(m, F, Rx, fp);
+  }
+};
+template <>
+struct CollidePairs<19> {
+  __device__ __forceinline__ static void run(const Moments<float> &, const Forces<float> &, const Relax<float> &,
+                                             float (&)[19]) {}
+};
+
+template <int q>
+struct OwnAll {   // the own cell's 19 D3Q19 populations (outlet faces)
+  __device__ __forceinline__ static void run(const OwnAt &o, float (&a)[19]) {
+    a[q] = o.template f<q>();
+    OwnAll<q + 1>::run(o, a);
+  }
+};
+template <>
+struct OwnAll<19> {
+  __device__ __forceinline__ static void run(const OwnAt &, float (&)[19]) {}
+};
+
+// post-stream population q of the second step at cell (i, j, k): the rules of pull19, the sources taken from the
+// sweep's planes (own = plane k of this column, f6n = population 6 of plane k + 1, f5m = population 5 of plane k - 1)
+template <int TX, int q>
+__device__ __forceinline__ float sweep_pull19(const Grid &g, const OwnAt &own, int i, int j, int k,
+                                              const int (&kn)[3][3], float uI, float vI, const Outflow<float> &ob,
+                                              int su_km1, float f6n, float f5m) {
+  constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
+  const int NX = g.nx, NY = g.ny;
+  const SweepLds &L = own.L;
+  if (cz == -1 && k == g.nz - 1) {
+    constexpr int m = MIRZ[q];
+    const int si = i - cx, sj = j - cy;
+    if (si < 0 || si >= NX || sj < 0 || sj >= NY) return own.f<m>();
+    if (k < kn[1 - cx][1 - cy]) return own.f<m>();
+    if constexpr (cx == 0 && cy == 0) return own.f<m>();
+    else return L.up[own.su][SW_UP[m]][own.t - cx - cy * TX];
+  }
+  const bool outx = (cx == 1 && i == 0) || (cx == -1 && i == NX - 1);
+  const bool outy = (cy == 1 && j == 0) || (cy == -1 && j == NY - 1);
+  if (outx || outy) return lateral19<StoreF32, q>(g, i, j, uI, vI, ob);
+  const int sk = k - cz;
+  if (sk < kn[1 - cx][1 - cy]) return own.f<OPP[q]>();
+  if constexpr (cx == 0 && cy == 0) {
+    if constexpr (cz == 0) return own.f<q>();
+    else if constexpr (cz == 1) return f5m;        // q = 5 from the level below
+    else return f6n;                               // q = 6 from the level above
+  } else {
+    const int tn = own.t - cx - cy * TX;
+    if constexpr (cz == 0) return L.flat[own.sf][SW_FLAT[q]][tn];
+    else if constexpr (cz == 1) return L.up[su_km1][SW_UP[q]][tn];
+    else return L.down[SW_DOWN[q]][tn];
+  }
+}
+
+// post-stream theta population q of the second step (the rules of pull7)
+template <int TX, int q>
+__device__ __forceinline__ float sweep_pull7(const Grid &g, const OwnAt &own, int i, int j, int k,
+                                             const int (&kn)[3][3], float g6n, float g5m) {
+  constexpr int cx = CX7[q], cy = CY7[q], cz = CZ7[q];
+  const Params &P = g.P;
+  if (cz == -1 && k == g.nz - 1) return own.g<q>();
+  const bool outx = (cx == 1 && i == 0) || (cx == -1 && i == g.nx - 1);
+  const bool outy = (cy == 1 && j == 0) || (cy == -1 && j == g.ny - 1);
+  if (outx || outy) {
+    const bool inflow = (cx == 1 && i == 0 && P.inflow_w != 0) || (cx == -1 && i == g.nx - 1 && P.inflow_e != 0) ||
+                        (cy == 1 && j == 0 && P.inflow_s != 0) || (cy == -1 && j == g.ny - 1 && P.inflow_n != 0);
+    return inflow ? 0.0f : own.g<q>();
+  }
+  const int sk = k - cz;
+  if (sk < kn[1 - cx][1 - cy]) return own.g<OPP7[q]>();
+  if constexpr (cx == 0 && cy == 0) {
+    if constexpr (cz == 0) return own.g<q>();
+    else if constexpr (cz == 1) return g5m;
+    else return g6n;
+  } else {
+    return own.L.flat[own.sf][8 + q - 1][own.t - cx - cy * TX];
+  }
+}
+
+template <int TX, int q>
+struct SweepPull {
+  __device__ __forceinline__ static void run(float (&f)[19], float (&gg)[7], const Grid &g, const OwnAt &own, int i,
+                                             int j, int k, const int (&kn)[3][3], float uI, float vI,
+                                             const Outflow<float> &ob, int su_km1, const SweepOwn &nxt, float f5m,
+                                             float g5m) {
+    f[q] = sweep_pull19<TX, q>(g, own, i, j, k, kn, uI, vI, ob, su_km1, nxt.f[2], f5m);
+    if constexpr (q < 7) gg[q] = sweep_pull7<TX, q>(g, own, i, j, k, kn, nxt.g[2], g5m);
+    SweepPull<TX, q + 1>::run(f, gg, g, own, i, j, k, kn, uI, vI, ob, su_km1, nxt, f5m, g5m);
+  }
+};
+template <int TX>
+struct SweepPull<TX, 19> {
+  __device__ __forceinline__ static void run(float (&)[19], float (&)[7], const Grid &, const OwnAt &, int, int, int,
+                                             const int (&)[3][3], float, float, const Outflow<float> &, int,
+                                             const SweepOwn &, float, float) {}
+};
+
+// a wave-uniform pointer the compiler must treat as new at this point: the population bases formed from it are then
+// formed again in every loop iteration (a few scalar adds) instead of being kept live, and spilled, across the loop
+template <class P>
+__device__ __forceinline__ P *fresh(P *p) {
+  uint64_t v = (uint64_t)(uintptr_t)p;
+  asm volatile("" : "+s"(v));
+  return (P *)(uintptr_t)v;
+}
+
+// the first step's streaming split in two (the values of stream_cell): the plain pulls of the fast and lid paths,
+// issued early, and what depends on them or on solid neighbours (bounce-back and face fix-ups, the boundary path)
+template <class S>
+__device__ __forceinline__ void sweep_issue(const Grid &g, const void *fA, const void *gA, uint32_t n, bool fast,
+                                            bool lid, float (&f)[19], float (&gg)[7]) {
+  if (fast) FastPull<0>::template run<S, false>(f, gg, g, nullptr, fA, gA, n);
+  else if (lid) LidPull<0>::template run<S, false>(f, gg, g, nullptr, fA, gA, n);
+}
+template <class S>
+__device__ __forceinline__ void sweep_finish(const Grid &g, const void *fA, const void *gA, uint32_t n, int c, int i,
+                                             int j, int k, int kg, int kfast, bool fast, bool lid, float uI, float vI,
+                                             float (&f)[19], float (&gg)[7]) {
+  using E = typename S::elem;
+  if (fast) {
+    if (k < kfast) bounce_fix<S, false>(f, gg, g, fA, gA, n, k - kg, ld_tab(g.colnb, (size_t)c));
+    if (i == 0 || i == g.nx - 1 || j == 0 || j == g.ny - 1) {
+      const Outflow<float> ob = outflow_prelude<S>(g, (const E *)fA, g.N, n, i, j);
+      FaceFix<1>::template run<S>(f, gg, g, gA, n, i, j, uI, vI, ob);
+    }
+  } else if (!lid) {
+    int kn[3][3];
+    load_neighbours(g, i, j, kg, kn);
+    const Outflow<float> ob = outflow_prelude<S>(g, (const E *)fA, g.N, n, i, j);
+    Pull19<0>::template run<S>(f, g, (const E *)fA, g.N, n, i, j, k, kn, uI, vI, ob);
+    Pull7<0>::template run<S>(gg, g, gA, n, i, j, k, kn);
+  }
+}
+
+// Blocks: (tile, level chunk), tile = tx + tiles_x * ty fastest, taken in the single step's XCD-aware order (win).
+// Region of tile (tx, ty): columns i = tx (TX - 2) - 1 + rx, j = ty (TY - 2) - 1 + ry, thread t = ry TX + rx.
+// Population bases are formed from the lattice pointers (blocked layout: one scalar add each), not read from the
+// stream table, whose 60 bases would stay live across the level loop in scalar registers.
+template <class S, int TX>
+__global__ __launch_bounds__(SWEEP_THREADS) void sweep2_kernel(const Grid *__restrict__ gp, const void *fA, void *fB,
+                                                               const void *gA, void *gB, uint32_t tiles_x,
+                                                               uint32_t tiles_xy, int kc, uint32_t win,
+                                                               uint32_t step) {
+  static_assert(S::bytes == 4 && !S::theta_ref && !S::SR && S::LB != 0, "the sweep runs the f32 pair layout");
+  constexpr int TY = SWEEP_THREADS / TX;
+  __shared__ SweepLds L;
+  const Grid &g = *gp;
+  const int t = (int)threadIdx.x;
+  const uint32_t b = tile_of_block(blockIdx.x, gridDim.x, win);
+  const uint32_t chunk = b / tiles_xy, tile = b - chunk * tiles_xy, ty = tile / tiles_x, tx = tile - ty * tiles_x;
+  const int rx = t % TX, ry = t / TX;
+  const int i = (int)tx * (TX - 2) - 1 + rx, j = (int)ty * (TY - 2) - 1 + ry;
+  const bool indom = i >= 0 && i < g.nx && j >= 0 && j < g.ny;
+  const bool inner = indom && rx >= 1 && rx <= TX - 2 && ry >= 1 && ry <= TY - 2;
+  const int c = indom ? j * g.nx + i : 0;
+  const uint32_t ck = ld_tab(g.colk, (size_t)c);
+  const int ktc = col_kt(ck), kg = indom ? col_ktop(ck) : g.nz, kfast = col_kfast(ck);
+  const bool near = col_near(ck);
+  float2 cq = make_float2(0.0f, 0.0f);
+  if (kg < g.nz) cq = ld_tab(g.colcq, (size_t)c);
+  int kn[3][3];
+  if (inner) load_neighbours(g, i, j, kg, kn);
+  const int kb0 = g.kmin + (int)chunk * kc, kb1 = min(kb0 + kc, g.nz);
+  auto is_fast = [&](int ka) { return (ka >= kfast || near) && ka < g.nz - 1; };
+  auto is_lid = [&](int ka) {
+    return ka == g.nz - 1 && ka >= kfast && i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
+  };
+  SweepOwn cur, nxt;
+  float f5m = 0.0f, g5m = 0.0f;
+  float fa[19], ga[7];   // pulls of the pending first-step plane
+  {
+    const int ka = kb0 - 1;
+    if (ka >= kg && ka < g.nz)
+      sweep_issue<S>(g, fresh(fA), fresh(gA), (uint32_t)ka * g.NXY + (uint32_t)c, is_fast(ka), is_lid(ka), fa, ga);
+  }
+  for (int ka = kb0 - 1; ka <= kb1; ka++) {
+    // phase A: the first step of plane ka, from the pulls issued one iteration earlier
+    if (ka >= kg && ka < g.nz) {
+      const uint32_t n = (uint32_t)ka * g.NXY + (uint32_t)c;
+      const float4 pr = ld_tab(g.prof, (size_t)max(ka - ktc, 0));
+      sweep_finish<S>(g, fresh(fA), fresh(gA), n, c, i, j, ka, kg, kfast, is_fast(ka), is_lid(ka), pr.x, pr.y, fa, ga);
+      const Moments<float> m = moments(fa, ga);
+      const Forces<float> F = forces<true, true>(g.P, g.rp, g.nz, ka, kg, ktc, ka == kg ? cq.x : 0.0f, pr.x, pr.y, m);
+      const Relax<float> Rx = relax(g.P, g.rp, g.nx, g.ny, i, j, pr.z, m, F);
+      float fp[19], gp7[7];
+      fp[0] = collide_rest(Rx);
+      CollidePairs<1>::run(m, F, Rx, fp);
+      theta_post(g.P, ka == kg, cq.y, m, F, Rx, ga, gp7);
+      const int sf = ka & 1, su = ka % 3;
+#pragma unroll
+      for (int q = 0; q < 19; q++) {
+        if (SW_FLAT[q] >= 0) L.flat[sf][SW_FLAT[q]][t] = fp[q];
+        if (SW_UP[q] >= 0) L.up[su][SW_UP[q]][t] = fp[q];
+        if (SW_DOWN[q] >= 0) L.down[SW_DOWN[q]][t] = fp[q];
+        if (SW_REG[q] >= 0) nxt.f[SW_REG[q]] = fp[q];
+      }
+#pragma unroll
+      for (int q = 1; q < 5; q++) L.flat[sf][8 + q - 1][t] = gp7[q];
+      nxt.g[0] = gp7[0];
+      nxt.g[1] = gp7[5];
+      nxt.g[2] = gp7[6];
+    }
+    __syncthreads();
+    {   // the pulls of the next first-step plane, in flight during phase B
+      const int kn1 = ka + 1;
+      if (kn1 <= kb1 && kn1 >= kg && kn1 < g.nz)
+        sweep_issue<S>(g, fresh(fA), fresh(gA), (uint32_t)kn1 * g.NXY + (uint32_t)c, is_fast(kn1), is_lid(kn1), fa,
+                       ga);
+    }
+    // phase B: the second step of plane k = ka - 1
+    const int k = ka - 1;
+    if (inner && k >= kb0 && k < kb1 && k >= kg) {
+      const uint32_t n = (uint32_t)k * g.NXY + (uint32_t)c;
+      const float4 pr = ld_tab(g.prof, (size_t)max(k - ktc, 0));
+      const Params &P = g.P;
+      const OwnAt own{L, t, k & 1, k % 3, cur};
+      Outflow<float> ob;
+      const bool onb = (i == 0 && P.inflow_w == 0) || (i == g.nx - 1 && P.inflow_e == 0) ||
+                       (j == 0 && P.inflow_s == 0) || (j == g.ny - 1 && P.inflow_n == 0);
+      if (onb) OwnAll<0>::run(own, ob.own);
+      outflow_finish(g, ob, onb, i, j);
+      float f[19], gg[7];
+      SweepPull<TX, 0>::run(f, gg, g, own, i, j, k, kn, pr.x, pr.y, ob, (k + 2) % 3, nxt, f5m, g5m);
+      const Moments<float> m = moments(f, gg);
+      const Forces<float> F = forces<true, true>(g.P, g.rp, g.nz, k, kg, ktc, k == kg ? cq.x : 0.0f, pr.x, pr.y, m);
+      const Relax<float> Rx = relax(g.P, g.rp, g.nx, g.ny, i, j, pr.z, m, F);
+      store_post<S, false>(g, nullptr, fresh(fB), fresh(gB), n, k == kg, cq.y, m, F, Rx, gg, step + 1u);
+    }
+    __syncthreads();
+    f5m = cur.f[1];
+    g5m = cur.g[1];
+    cur = nxt;
   }
 }
 
@@ -1207,6 +1535,27 @@ hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const Streams
     else if (levels == 2) (launch_step_kb<S, 2>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, step, st);
     else (launch_step_kb<S, 1>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, step, st);
   })
+  return hipGetLastError();
+}
+
+bool sweep2_supported(int storage, int lb) { return storage == 0 && lb == StoreF32B::LB; }
+
+hipError_t launch_sweep2(int storage, const Grid &g, const Grid *gd, const void *fA, void *fB, const void *gA,
+                         void *gB, int tx, int kc, uint32_t win, uint32_t step, hipStream_t st) {
+  if (!sweep2_supported(storage, g.lb) || (tx != 16 && tx != 32) || kc < 1) return hipErrorInvalidValue;
+  if (g.kmin >= g.nz) return hipSuccess;
+  const int ty = SWEEP_THREADS / tx;
+  const uint32_t tiles_x = (uint32_t)(g.nx + tx - 3) / (uint32_t)(tx - 2);
+  const uint32_t tiles_y = (uint32_t)(g.ny + ty - 3) / (uint32_t)(ty - 2);
+  const uint32_t chunks = (uint32_t)(g.nz - g.kmin + kc - 1) / (uint32_t)kc;
+  const dim3 grid(tiles_x * tiles_y * chunks);
+  if (win == WIN_PLANE_SLABS) win = 1;
+  if (tx == 16)
+    hipLaunchKernelGGL((sweep2_kernel<StoreF32BC, 16>), grid, dim3(SWEEP_THREADS), 0, st, gd, fA, fB, gA, gB,
+                       tiles_x, tiles_x * tiles_y, kc, win, step);
+  else
+    hipLaunchKernelGGL((sweep2_kernel<StoreF32BC, 32>), grid, dim3(SWEEP_THREADS), 0, st, gd, fA, fB, gA, gB,
+                       tiles_x, tiles_x * tiles_y, kc, win, step);
   return hipGetLastError();
 }
 

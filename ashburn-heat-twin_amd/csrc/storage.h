@@ -182,6 +182,14 @@ struct StoreF32B : StoreF32 {
   static constexpr int LB = 11;
   static constexpr int QB = 26;   // populations per block: 19 D3Q19, then 7 D3Q7
 };
+// the same layout written through the cache: the two-step sweep writes row segments of its column tiles, which start
+// and end inside 128-byte lines that the neighbouring tile writes; non-temporal stores send such partial lines to
+// HBM one by one (1.24 ms per step at 600 x 600 x 64), cached stores let the two halves meet in L2 (0.89 ms)
+struct StoreF32BC : StoreF32B {
+  __device__ static void st_raw(void *base, uint32_t boff, elem v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rsrc(base), boff, 0, 0);
+  }
+};
 
 // the 16-bit formats in the same interleaved layout, in blocks of 256 cells: [cell >> 8][26][cell & 255] of 2-byte
 // deviations, with the f32 theta plane [cell] after the array (the f16 step runs 4.9 % faster than in [q][cell] at

@@ -94,8 +94,7 @@ def timed_step(m, dom, storage, steps, warmup, launch, tune_steps):
     s = m.Solver.from_dump(dom, storage=storage)
     try:
         if launch:
-            b, l, w = (int(x) for x in launch.split(','))
-            s.set_launch(b, l, w)
+            s.set_launch(*(int(x) for x in launch.split(',')))
         else:
             s.step(2)
             log(f'{STORAGE[storage]}: autotuned launch shape', s.autotune(tune_steps))
@@ -138,7 +137,7 @@ def main():
     ap.add_argument('--workload', default='c2')
     ap.add_argument('--storage', type=int, default=0,
                     help='0 f32, 1 f16 deviations, 2 bf16 deviations, 3 f64 twin, 4 f16 without theta plane (SR)')
-    ap.add_argument('--launch', default=None, help='fixed launch shape block,levels,window (no tuning)')
+    ap.add_argument('--launch', default=None, help='fixed launch shape block,levels,window[,sweep] (no tuning)')
     ap.add_argument('--tune-steps', type=int, default=12, help='steps per launch-shape candidate of the autotuner')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--no-cpu', action='store_true')
@@ -190,7 +189,8 @@ def main():
                              'frac': round(ach16 / HBM_PEAK_GBS, 4), 'bytes_per_fluid_cell': bpc16},
                 'launch': launch16, 'non_finite': s16['non_finite']}
             log(f'{STORAGE[st16]}: {ms16:.4f} ms/step, {ms_step / ms16:.3f}x f32')
-    kernel_tag = f'step_{STORAGE[a.storage]}_{a.workload}'
+    sweep = bool(launch.get('sweep')) and a.storage == 0      # two steps per launch (opt-in launch shape)
+    kernel_tag = f'{"sweep2" if sweep else "step"}_{STORAGE[a.storage]}_{a.workload}'
     traffic = traffic_from_profiles(kernel_tag)
     cpu = None
     if not a.no_cpu:
@@ -220,7 +220,8 @@ def main():
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
                      'traffic_source': 'profile: rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (copy-calibrated) per '
-                                       'launch, profiles/traffic.json; not measured in this run'},
+                                       'launch, profiles/traffic.json; not measured in this run',
+                     'steps_per_launch': 2 if sweep else 1},
         'cpu_baseline': cpu,
         'reduced_precision': reduced,
         'serving': serve,

@@ -191,7 +191,14 @@ typedef struct {
   int32_t window;        /* tile order: 1 linear, W = 2, 4, ..., 4096 = runs of W consecutive tiles per XCD,
                             0 = one contiguous slab of tiles per XCD, -1 = plane slabs (XCD x steps the
                             x-th eighth of the columns of every level, all XCDs level by level together)  */
-  int32_t reserved;
+  int32_t sweep;         /* 0: one step per launch. Otherwise two steps per launch (f32 storage in the blocked
+                            pair layout; other formats ignore it): a z sweep over level chunks of |sweep|
+                            levels, regions of 32 x 8 columns (sweep > 0) or 16 x 16 (sweep < 0), with the
+                            first step's state kept on chip (LDS-staged halo planes); steps with FP64 means or
+                            debug outputs, and an odd last step, take the one-step kernel (block, levels).
+                            Bit-identical results. Opt-in, the autotuner does not try it: at 600 x 600 x 64 it
+                            matches the one-step kernel at best (sweep 12, window 4), on smaller grids it is
+                            slower                                                                          */
 } aht_launch;
 
 /* launch-shape autotuner of one solver: times every block x levels x window candidate over `steps` steps each,
