@@ -1,19 +1,29 @@
 /*
  * N-API addon: the thin boundary between the Node host (js/solver.js) and the C-ABI of libaht.so (include/aht.h).
  * One JS function per aht_* entry point, typed arrays passed by pointer, no copies beyond what the C-ABI does.
- * step() and benchmark() run aht_step + aht_sync on a libuv worker thread and return a Promise (the reference's
- * step(n) resolves on queue completion, js/live/solver.js:401-406). Errors throw with aht_last_error's message.
+ * step(), benchmark() and batchStep() run aht_step + aht_sync (or the batch's) on the addon's device thread and return
+ * a Promise (the reference's step(n) resolves on queue completion, js/live/solver.js:401-406). Errors throw with
+ * aht_last_error's message.
+ *
+ * The device thread: one thread per process takes the asynchronous jobs in the order they were issued (one queue, as
+ * the reference's device queue), so two jobs never run the C-ABI on one solver at once, and a Promise settles on the
+ * main thread through a thread-safe function. It is not libuv's thread pool on purpose: libuv joins its pool threads
+ * in a library destructor at process exit, after the static destructors of the libraries the addon loaded (the HIP
+ * runtime among them) have run, so a pool thread that made HIP calls would tear its per-thread HIP state down against
+ * a finished runtime. The device thread is joined in the environment's cleanup hook instead, before those
+ * destructors; after process.exit() it is never joined and keeps its state until the process ends.
  *
  * The solver handle is a JS external; the host serialises calls per solver (the C-ABI is one caller per solver).
  *
- * Lifetime: a step, benchmark or batch step runs on a libuv worker while the main thread goes on. destroy() of a
- * solver (or batch) with such work in flight does not free it under the worker: the handle is marked dead at once
+ * Lifetime: a step, benchmark or batch step runs on the device thread while the main thread goes on. destroy() of a
+ * solver (or batch) with such work in flight does not free it under the job: the handle is marked dead at once
  * (every later call on it throws) and the library object is destroyed by the completion callback of the last job
  * that uses it, on the main thread. A solver that is a member of a batch counts the batch's in-flight steps as its
  * own. Batch membership cannot change while a batch step is in flight.
  */
 #define NAPI_VERSION 4
 #include <node_api.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -33,7 +43,7 @@ typedef struct bhandle bhandle_t;
 
 typedef struct handle {
   aht_solver *s;
-  int inflight;        /* step / benchmark jobs queued or running on a worker                     */
+  int inflight;        /* step / benchmark jobs queued or running on the device thread            */
   int dead;            /* destroy() was called; the solver goes when no job uses it any more       */
   int released;        /* the JS external was finalized (the struct is freed once nothing uses it) */
   bhandle_t *batch;    /* the batch this solver belongs to (NULL: none)                            */
@@ -283,63 +293,169 @@ static napi_value js_reinit_cells(napi_env env, napi_callback_info info) {
   return rc ? throw_rc(env, h->s, rc) : undef(env);
 }
 
-/* ---- asynchronous stepping ------------------------------------------------------------------------------------ */
-typedef struct {
-  napi_async_work work;
+/* ---- asynchronous jobs on the device thread ------------------------------------------------------------------ */
+typedef struct job job_t;
+struct job {
+  job_t *next;
+  napi_threadsafe_function done;               /* settles the job on the main thread                  */
   napi_deferred deferred;
-  napi_ref keep;     /* keeps the handle's external alive while the work runs */
-  handle_t *h;
-  int32_t n;
-  uint32_t flags;
-  int bench;
-  double ms;
+  napi_ref keep;                               /* keeps the handle's external alive while the job runs */
+  void (*run)(job_t *);                        /* on the device thread                                 */
+  void (*complete)(napi_env, job_t *);         /* on the main thread; env NULL: environment torn down  */
   int rc;
+  double ms;
   char err[512];
-} step_job;
+};
 
-static void step_execute(napi_env env, void *data) {
-  (void)env;
-  step_job *j = (step_job *)data;
-  if (j->bench) {
-    j->rc = aht_benchmark(j->h->s, j->n, &j->ms);
-  } else {
-    j->rc = aht_step(j->h->s, j->n, j->flags);
-    if (j->rc == AHT_OK) j->rc = aht_sync(j->h->s);
+static struct {
+  pthread_mutex_t mu;
+  pthread_cond_t cv;
+  job_t *head, *tail;
+  int started, stop;
+  pthread_t th;
+} dq = {PTHREAD_MUTEX_INITIALIZER, PTHREAD_COND_INITIALIZER, NULL, NULL, 0, 0};
+
+static void *device_loop(void *arg) {
+  (void)arg;
+  for (;;) {
+    pthread_mutex_lock(&dq.mu);
+    while (!dq.head && !dq.stop) pthread_cond_wait(&dq.cv, &dq.mu);
+    job_t *j = dq.head;
+    if (!j) {                                  /* stop requested and nothing queued */
+      pthread_mutex_unlock(&dq.mu);
+      return NULL;
+    }
+    dq.head = j->next;
+    if (!dq.head) dq.tail = NULL;
+    pthread_mutex_unlock(&dq.mu);
+    j->run(j);
+    const napi_threadsafe_function tf = j->done;   /* j may be freed on the main thread once the call is queued */
+    if (napi_call_threadsafe_function(tf, j, napi_tsfn_blocking) != napi_ok) j->complete(NULL, j);
+    napi_release_threadsafe_function(tf, napi_tsfn_release);
   }
-  if (j->rc) snprintf(j->err, sizeof j->err, "aht: %s (%s)", aht_last_error(j->h->s), aht_error_string(j->rc));
 }
 
-static void step_complete(napi_env env, napi_status status, void *data) {
-  step_job *j = (step_job *)data;
-  if (status != napi_ok || j->rc) {
+/* the environment's cleanup hook: runs before the static destructors of the loaded libraries at a normal exit */
+static void device_thread_stop(void *arg) {
+  (void)arg;
+  pthread_mutex_lock(&dq.mu);
+  const int started = dq.started;
+  dq.stop = 1;
+  pthread_cond_signal(&dq.cv);
+  pthread_mutex_unlock(&dq.mu);
+  if (started) pthread_join(dq.th, NULL);
+  pthread_mutex_lock(&dq.mu);
+  dq.started = 0;
+  dq.stop = 0;
+  pthread_mutex_unlock(&dq.mu);
+}
+
+static void job_settle(napi_env env, napi_value cb, void *ctx, void *data) {
+  (void)cb; (void)ctx;
+  job_t *j = (job_t *)data;
+  j->complete(env, j);
+}
+
+/* resolves (value) or rejects (j->err) the job's Promise and drops its handle reference */
+static void job_finish(napi_env env, job_t *j, int resolve_ms) {
+  if (j->rc) {
     napi_value msg, e;
-    napi_create_string_utf8(env, j->rc ? j->err : "aht: step cancelled", NAPI_AUTO_LENGTH, &msg);
+    napi_create_string_utf8(env, j->err, NAPI_AUTO_LENGTH, &msg);
     napi_create_error(env, NULL, msg, &e);
     napi_reject_deferred(env, j->deferred, e);
   } else {
     napi_value v;
-    if (j->bench) napi_create_double(env, j->ms, &v);
+    if (resolve_ms) napi_create_double(env, j->ms, &v);
     else napi_get_undefined(env, &v);
     napi_resolve_deferred(env, j->deferred, v);
   }
-  handle_t *h = j->h;
-  h->inflight--;
   napi_delete_reference(env, j->keep);
-  napi_delete_async_work(env, j->work);
+}
+
+/* queues j on the device thread (started on first use); returns the job's Promise, NULL with an exception pending */
+static napi_value queue_job(napi_env env, job_t *j, napi_value keep) {
+  napi_value promise, name;
+  CHECK_NAPI(napi_create_string_utf8(env, "aht_device_job", NAPI_AUTO_LENGTH, &name));
+  CHECK_NAPI(napi_create_threadsafe_function(env, NULL, NULL, name, 0, 1, NULL, NULL, NULL, job_settle, &j->done));
+  const char *fail = NULL;
+  if (napi_create_reference(env, keep, 1, &j->keep) != napi_ok) fail = "aht addon: N-API call failed: napi_create_reference";
+  else if (napi_create_promise(env, &j->deferred, &promise) != napi_ok) {
+    napi_delete_reference(env, j->keep);
+    fail = "aht addon: N-API call failed: napi_create_promise";
+  }
+  if (fail) {
+    napi_release_threadsafe_function(j->done, napi_tsfn_abort);
+    napi_throw_error(env, NULL, fail);
+    return NULL;
+  }
+  pthread_mutex_lock(&dq.mu);
+  if (!dq.started) {
+    if (pthread_create(&dq.th, NULL, device_loop, NULL) != 0) {
+      pthread_mutex_unlock(&dq.mu);
+      napi_release_threadsafe_function(j->done, napi_tsfn_abort);
+      napi_delete_reference(env, j->keep);
+      napi_value err, msg;           /* the Promise exists: reject it rather than leave it pending */
+      napi_create_string_utf8(env, "aht addon: cannot start the device thread", NAPI_AUTO_LENGTH, &msg);
+      napi_create_error(env, NULL, msg, &err);
+      napi_reject_deferred(env, j->deferred, err);
+      napi_throw(env, err);
+      return NULL;
+    }
+    dq.started = 1;
+    napi_add_env_cleanup_hook(env, device_thread_stop, NULL);
+  }
+  j->next = NULL;
+  if (dq.tail) dq.tail->next = j;
+  else dq.head = j;
+  dq.tail = j;
+  pthread_cond_signal(&dq.cv);
+  pthread_mutex_unlock(&dq.mu);
+  return promise;
+}
+
+typedef struct {
+  job_t base;
+  handle_t *h;
+  int32_t n;
+  uint32_t flags;
+  int bench;
+} step_job;
+
+static void step_run(job_t *jb) {
+  step_job *j = (step_job *)jb;
+  if (j->bench) {
+    jb->rc = aht_benchmark(j->h->s, j->n, &jb->ms);
+  } else {
+    jb->rc = aht_step(j->h->s, j->n, j->flags);
+    if (jb->rc == AHT_OK) jb->rc = aht_sync(j->h->s);
+  }
+  if (jb->rc) snprintf(jb->err, sizeof jb->err, "aht: %s (%s)", aht_last_error(j->h->s), aht_error_string(jb->rc));
+}
+
+static void step_complete(napi_env env, job_t *jb) {
+  step_job *j = (step_job *)jb;
+  if (env) job_finish(env, jb, j->bench);
+  handle_t *h = j->h;
   free(j);
-  solver_settle(h);
+  h->inflight--;
+  if (env) solver_settle(h);
 }
 
 static napi_value start_job(napi_env env, napi_value hv, handle_t *h, int32_t n, uint32_t flags, int bench) {
   step_job *j = (step_job *)calloc(1, sizeof *j);
+  if (!j) {
+    napi_throw_error(env, NULL, "aht: out of host memory");
+    return NULL;
+  }
+  j->base.run = step_run;
+  j->base.complete = step_complete;
   j->h = h; j->n = n; j->flags = flags; j->bench = bench;
-  napi_value promise, name;
-  CHECK_NAPI(napi_create_promise(env, &j->deferred, &promise));
-  CHECK_NAPI(napi_create_reference(env, hv, 1, &j->keep));
-  CHECK_NAPI(napi_create_string_utf8(env, bench ? "aht_benchmark" : "aht_step", NAPI_AUTO_LENGTH, &name));
-  CHECK_NAPI(napi_create_async_work(env, NULL, name, step_execute, step_complete, j, &j->work));
-  CHECK_NAPI(napi_queue_async_work(env, j->work));
   h->inflight++;
+  napi_value promise = queue_job(env, &j->base, hv);
+  if (!promise) {
+    h->inflight--;
+    free(j);
+  }
   return promise;
 }
 
@@ -698,45 +814,29 @@ static napi_value js_batch_block(napi_env env, napi_callback_info info) {
 }
 
 typedef struct {
-  napi_async_work work;
-  napi_deferred deferred;
-  napi_ref keep;
+  job_t base;
   bhandle_t *b;
   int32_t n;
   int32_t *ns, *af;
-  double ms;
-  int rc;
-  char err[512];
 } bstep_job;
 
-static void bstep_execute(napi_env env, void *data) {
-  (void)env;
-  bstep_job *j = (bstep_job *)data;
-  j->rc = aht_batch_step(j->b->b, j->n, j->ns, j->af, &j->ms);
-  if (j->rc == AHT_OK) j->rc = aht_batch_sync(j->b->b);
-  if (j->rc) snprintf(j->err, sizeof j->err, "aht batch: %s (%s)", aht_batch_last_error(j->b->b), aht_error_string(j->rc));
+static void bstep_run(job_t *jb) {
+  bstep_job *j = (bstep_job *)jb;
+  jb->rc = aht_batch_step(j->b->b, j->n, j->ns, j->af, &jb->ms);
+  if (jb->rc == AHT_OK) jb->rc = aht_batch_sync(j->b->b);
+  if (jb->rc)
+    snprintf(jb->err, sizeof jb->err, "aht batch: %s (%s)", aht_batch_last_error(j->b->b), aht_error_string(jb->rc));
 }
 
-static void bstep_complete(napi_env env, napi_status status, void *data) {
-  bstep_job *j = (bstep_job *)data;
-  if (status != napi_ok || j->rc) {
-    napi_value msg, e;
-    napi_create_string_utf8(env, j->rc ? j->err : "aht batch: step cancelled", NAPI_AUTO_LENGTH, &msg);
-    napi_create_error(env, NULL, msg, &e);
-    napi_reject_deferred(env, j->deferred, e);
-  } else {
-    napi_value v;
-    napi_create_double(env, j->ms, &v);
-    napi_resolve_deferred(env, j->deferred, v);
-  }
+static void bstep_complete(napi_env env, job_t *jb) {
+  bstep_job *j = (bstep_job *)jb;
+  if (env) job_finish(env, jb, 1);
   bhandle_t *b = j->b;
-  b->inflight--;
-  napi_delete_reference(env, j->keep);
-  napi_delete_async_work(env, j->work);
   free(j->ns);
   free(j->af);
   free(j);
-  if (!b->inflight) {
+  b->inflight--;
+  if (env && !b->inflight) {
     /* members destroyed while the step ran go now (solver_release edits the member list: walk it backwards) */
     for (int k = b->nmem - 1; k >= 0; k--) {
       handle_t *h = b->mem[k];
@@ -768,16 +868,25 @@ static napi_value js_batch_step(napi_env env, napi_callback_info info) {
     return NULL;
   }
   bstep_job *j = (bstep_job *)calloc(1, sizeof *j);
+  if (j && pn && !(j->ns = (int32_t *)malloc(m * 4 + 4))) { free(j); j = NULL; }
+  if (j && pa && !(j->af = (int32_t *)malloc(m * 4 + 4))) { free(j->ns); free(j); j = NULL; }
+  if (!j) {
+    napi_throw_error(env, NULL, "aht batch: out of host memory");
+    return NULL;
+  }
+  if (pn) memcpy(j->ns, pn, m * 4);
+  if (pa) memcpy(j->af, pa, m * 4);
+  j->base.run = bstep_run;
+  j->base.complete = bstep_complete;
   j->b = b; j->n = n;
-  if (pn) { j->ns = (int32_t *)malloc(m * 4 + 4); memcpy(j->ns, pn, m * 4); }
-  if (pa) { j->af = (int32_t *)malloc(m * 4 + 4); memcpy(j->af, pa, m * 4); }
-  napi_value promise, name;
-  CHECK_NAPI(napi_create_promise(env, &j->deferred, &promise));
-  CHECK_NAPI(napi_create_reference(env, argv[0], 1, &j->keep));
-  CHECK_NAPI(napi_create_string_utf8(env, "aht_batch_step", NAPI_AUTO_LENGTH, &name));
-  CHECK_NAPI(napi_create_async_work(env, NULL, name, bstep_execute, bstep_complete, j, &j->work));
-  CHECK_NAPI(napi_queue_async_work(env, j->work));
   b->inflight++;
+  napi_value promise = queue_job(env, &j->base, argv[0]);
+  if (!promise) {
+    b->inflight--;
+    free(j->ns);
+    free(j->af);
+    free(j);
+  }
   return promise;
 }
 

@@ -76,12 +76,15 @@ def serving(rate, requests):
     d = tempfile.mkdtemp(prefix='aht_serve_')
     out = os.path.join(d, 'serve.json')
     t0 = time.perf_counter()
-    subprocess.run(['node', os.path.join(PKG, 'js', 'serve.js'), '--rate', str(rate), '--requests', str(requests),
-                    '--mode', 'batch,sequential', '--out', out], check=True, stdout=subprocess.DEVNULL)
+    p = subprocess.run(['node', os.path.join(PKG, 'js', 'serve.js'), '--rate', str(rate), '--requests', str(requests),
+                        '--mode', 'batch,sequential', '--out', out], stdout=subprocess.DEVNULL)
+    if not os.path.exists(out):
+        raise RuntimeError(f'serving loop failed (exit status {p.returncode}) before writing its record')
     r = json.load(open(out))
     subprocess.run(['rm', '-rf', d])
     b, q = r['batch']['summary'], r['sequential']['summary']
     return {'measured_in_this_run': True, 'rate_req_s': rate, 'requests': requests,
+            'node_exit_status': p.returncode,        # the record is written last; a failure after it shows here
             'wall_s': round(time.perf_counter() - t0, 1),
             'scenarios_per_hour': b['scenarios_per_hour'], 'latency_p50_s': b['latency_p50_s'],
             'latency_p99_s': b['latency_p99_s'],

@@ -75,3 +75,26 @@ console.log(JSON.stringify({ r: { n: r.n, ns: Array.from(r.ns), af: Array.from(r
     assert out['r'] == {'n': 50, 'ns': [50, 7, 20], 'af': [10, 19900, 0]}
     assert out['f'] == {'n': 50, 'ns': [50, 50, 20]}
     assert out['big'] == [50]                  # alone, a large member steps every launch
+
+
+def test_addon_device_thread_against_cpu_standin(tmp_path):
+    """the N-API addon built against a CPU stand-in of the C-ABI (tests/js/aht_standin.cpp), which aborts on any
+    threading or lifetime violation: jobs issued without awaiting run one at a time in issue order, destroy() with
+    a step or batch step in flight is deferred, and no thread that made device calls outlives the runtime's static
+    teardown at exit (libuv joins its pool threads after it; the addon's device thread is joined before it)"""
+    lib, addon = tmp_path / 'libaht.so', tmp_path / 'aht_node.node'
+    subprocess.run(['g++', '-O1', '-std=c++17', '-shared', '-fPIC', '-I', os.path.join(ROOT, 'include'), '-o', str(lib),
+                    os.path.join(ROOT, 'tests', 'js', 'aht_standin.cpp'), '-lpthread'], check=True)
+    subprocess.run(['gcc', '-O2', '-std=c11', '-shared', '-fPIC', '-Wall', '-I/usr/include/node',
+                    '-I', os.path.join(ROOT, 'include'), '-o', str(addon),
+                    os.path.join(ROOT, 'ashburn-heat-twin_amd', 'csrc', 'aht_napi.c'), '-L', str(tmp_path), '-laht',
+                    '-lpthread', '-Wl,-rpath,$ORIGIN'], check=True)
+    for mode in ('natural', 'exit'):
+        r = subprocess.run(['node', os.path.join(ROOT, 'tests', 'js', 'exit_test.js'), str(addon), mode],
+                           capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0 and 'STANDIN FAILURE' not in r.stderr, (mode, r.returncode, r.stderr)
+        out = json.loads(r.stdout)
+        assert out['stepBeforeDestroy'] == 'resolved' and out['stepAfterDestroy'] == 'threw'
+        assert out['batchStep'] == 10 and out['bench'] == 3
+        assert out['order'] == list(range(6))
+        assert 'last step settled' in r.stderr
