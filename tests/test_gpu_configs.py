@@ -129,12 +129,11 @@ def test_c3_reduced_precision_against_fp64_twin(tmp_path, pkg):
     assert out['f32']['warming_mean_rel'] <= 1e-5
 
 
-@pytest.mark.skip(reason='the node process of this case (js/batch_run.js) ended with SIGSEGV at exit after printing a '
-                         'correct result (all ten members identical) in one of three runs; no cause found by reading '
-                         'the code or by running the addon against a CPU stand-in of the C-ABI under AddressSanitizer, '
-                         'so the case stays off the GPU; test_c4_batched_members_equal_alone_c_abi checks the same '
-                         'property through the C-ABI from this process')
 def test_c4_batched_members_equal_alone():
+    """the batched workload's Node driver (js/batch_run.js): ten members, batched and one at a time, bit for bit.
+    Its process once ended with SIGSEGV at exit after a correct result: the addon then ran device jobs on libuv's
+    pool threads, which libuv joins at exit after the HIP runtime's static teardown; jobs now run on the addon's own
+    thread, joined before it (tests/test_js_host.py checks that order against a CPU stand-in)"""
     out = node('batch_run.js', '--domains', 10, '--steps', 300, '--chunk', 50, '--check', 'all', '--tune-steps', 2)
     print(json.dumps(out))
     assert set(out['workload']['mix']) == {'100x100x32', '150x150x32', '200x200x48', '300x300x48'}
@@ -242,11 +241,9 @@ def test_c5_grid_mix_batched_equal_alone_c_abi(tmp_path, pkg):
         assert bud == got[d][3] and bud['n_acc'] == total[d] - acc_start[d], d
 
 
-@pytest.mark.skip(reason='the node process of this case ended with SIGSEGV at exit after printing a correct result '
-                         '(every served result identical to alone) in one run, and passed in an earlier one; no cause '
-                         'found by reading the code or by running the addon against a CPU stand-in of the C-ABI under '
-                         'AddressSanitizer, so the case stays off the GPU; the small-grid serving test covers the path')
 def test_c5_grid_mix_served_equal_alone():
+    """eight requests of the serving grid mix through the Node serving loop, each served result bit-identical to the
+    same request alone (the process-exit fix of the C4 driver's test applies here too)"""
     out = node('../../tests/js/serve_test.js', '--c5')
     print(json.dumps(out))
     assert out['summary']['requests'] == 8
