@@ -155,7 +155,8 @@ int64_t aht_footprint_bytes(const aht_dims *d) {
   const int64_t A = (int64_t)2 << 20;
   const int64_t fb = ((int64_t)aht::f_lattice_bytes(d->storage, lb, (size_t)N) + A - 1) / A * A;
   const int64_t gb = ((int64_t)aht::g_lattice_bytes(d->storage, lb, (size_t)N) + A - 1) / A * A;
-  const int64_t pop = 2 * (fb + gb);
+  const int64_t guard = ((int64_t)aht::lattice_guard_bytes(d->storage, lb, d->nx) + A - 1) / A * A;
+  const int64_t pop = 2 * (fb + gb) + 2 * guard;
   (void)eb;
   const int64_t cols = NXY * (4 + 8 + 4 + 4) + (int64_t)d->nz * 16;
   const int64_t acc = 3 * NXY * 8 + (int64_t)face_total(d->nx, d->ny, d->nz) * 8 + (int64_t)max_content_slots(*d) * 8;
@@ -211,15 +212,18 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
     // inside few translation entries (the f16 step's L1 TLB misses went from 4 M per launch to none at 600x600x64)
     const size_t A = (size_t)2 << 20;
     const size_t fb = (s->fbytes + A - 1) / A * A, gb = (s->gbytes + A - 1) / A * A;
-    char *base = nullptr;
-    HIP_TRY(s, hipMalloc((void **)&base, 2 * (fb + gb)));
-    s->lattices = base;
-    // a blocked f32 pair is one array (gb = 0) whose D3Q7 populations start 19 populations into the first block
+    // blocked pairs: guard parts before the first and after the second pair (pull_guard_bytes; read, never used)
+    const size_t guard = (aht::lattice_guard_bytes(d.storage, s->g.lb, d.nx) + A - 1) / A * A;
+    char *alloc = nullptr;
+    HIP_TRY(s, hipMalloc((void **)&alloc, 2 * (fb + gb) + 2 * guard));
+    s->lattices = alloc;
+    char *base = alloc + guard;
+    // a blocked pair is one array (gb = 0) whose D3Q7 populations start 19 populations into the first block
     const size_t go = aht::g_lattice_offset(d.storage, s->g.lb);
     s->f[0] = base; s->f[1] = base + fb + gb;
     s->gq[0] = gb ? base + fb : base + go;
     s->gq[1] = gb ? base + 2 * fb + gb : base + fb + go;
-    HIP_TRY(s, hipMemsetAsync(base, 0, 2 * (fb + gb), s->st));
+    HIP_TRY(s, hipMemsetAsync(alloc, 0, 2 * (fb + gb) + 2 * guard, s->st));
   }
   HIP_TRY(s, hipMalloc(&s->colk, NXY * sizeof(uint32_t)));
   HIP_TRY(s, hipMalloc(&s->colnb, NXY * sizeof(uint32_t)));

@@ -37,11 +37,13 @@ struct Acc {
 
 // Stream bases of the single-domain fast path, host-built once per lattice parity (the step that reads lattice pair A
 // and writes pair B): rd = the base of every read stream with its pull offset folded in (population plane q shifted
-// back by c_q . (1, nx, nx ny) cells), wr = every written plane. A wavefront loads these 64-bit bases through the
-// scalar cache instead of forming 60 64-bit addresses with scalar multiplies and adds.
+// back by c_q . (1, nx, nx ny) cells; in a blocked layout by the uniform part of the pull, pull_shift), wr = every
+// written plane, lo = the blocked pull's remainder within a block (pull_rem; the lane's compare operand). A wavefront
+// loads these through the scalar cache instead of forming 60 64-bit addresses with scalar multiplies and adds.
 struct Streams {
   uint64_t rd[33];   // f_q (0..18), g_q (19..25), the theta plane pulled along D3Q7 velocity q (26..32, theta_ref)
   uint64_t wr[27];   // f_q planes (0..18), g_q planes (19..25), the theta plane (26, theta_ref)
+  uint32_t lo[19];   // blocked layouts: pull remainder of velocity q (D3Q7 q < 7 shares it)
 };
 
 constexpr int STEP_BLOCK = 256;
@@ -76,6 +78,9 @@ bool storage_theta_ref(int storage);                    // D3Q7 lattice = f32 th
 size_t theta_offset(int storage, int lb, size_t N);     // byte offset of the theta plane from the D3Q7 base (or 0)
 int population_layout(int storage, size_t N);           // Grid::lb for a lattice of N cells in this format
 size_t lattice_cells(int lb, size_t N);                 // cells per population, rounded up to whole blocks
+// bytes kept free before the first and after the last blocked lattice pair of an allocation: the fast path's pulls
+// of face cells at levels 1 and nz - 2 reach up to nx + 1 cells outside the lattice (their values are replaced)
+size_t lattice_guard_bytes(int storage, int lb, int nx);
 size_t f_lattice_bytes(int storage, int lb, size_t N);  // bytes of one D3Q19 lattice (a blocked pair: the whole pair)
 size_t g_lattice_bytes(int storage, int lb, size_t N);  // bytes of one D3Q7 lattice (a blocked pair: 0, inside the D3Q19)
 size_t g_lattice_offset(int storage, int lb);           // a blocked pair's D3Q7 base from its D3Q19 base (else 0)

@@ -356,26 +356,33 @@ struct FastPull {
   __device__ __forceinline__ static void one(typename S::real (&f)[19], typename S::real (&gg)[7], const Grid &g,
                                              const Streams *sp, const void *fA, const void *gA, uint32_t n) {
     if constexpr (S::LB != 0) {
-      // blocked layout: unshifted population bases, per-lane offsets of the pulled cell (clamped into the lattice: a
-      // face cell's out-of-domain pulls are replaced afterwards, and the block arithmetic needs a cell in range)
-      const int off = CX[q] + CY[q] * g.nx + CZ[q] * (int)g.NXY;
-      const uint32_t src = (uint32_t)min(max((int)n - off, 0), (int)g.N - 1);
+      // blocked layout: the pull's uniform part is in the stream base, the lane picks one of two offsets of its own
+      // cell (pull_voff); out-of-domain sources of face cells at levels 1 and nz - 2 read the allocation's guard
+      // bytes and are replaced afterwards
       const void *bf;
-      if constexpr (TAB) bf = (const void *)sp->rd[q];
-      else bf = plane<S, 19>((void *)fA, g.N, q);
-      f[q] = S::dec_f(S::ld_raw(bf, cell_off<S, 19>(src)), w19<q>());
-      if constexpr (q < 7) {                                    // D3Q7 velocity q = D3Q19 velocity q
+      uint32_t o;
+      if constexpr (TAB) {
+        bf = (const void *)sp->rd[q];
+        o = sp->lo[q];
+      } else {
+        const int64_t off = CX[q] + CY[q] * (int64_t)g.nx + CZ[q] * (int64_t)g.NXY;
+        bf = (const void *)((uintptr_t)plane<S, 19>((void *)fA, g.N, q) - (intptr_t)pull_shift(S::LB, S::bytes, off));
+        o = pull_rem(S::LB, off);
+      }
+      const uint32_t voff = pull_voff<S, 19>(n, o);
+      f[q] = S::dec_f(S::ld_raw(bf, voff), w19<q>());
+      if constexpr (q < 7) {                                    // D3Q7 velocity q = D3Q19 velocity q: same offset
         const void *bg;
         if constexpr (TAB) bg = (const void *)sp->rd[19 + q];
-        else bg = plane<S, 7>((void *)gA, g.N, q);
+        else bg = (const void *)((uintptr_t)bf + ((uintptr_t)gA - (uintptr_t)fA));
         if constexpr (S::theta_ref) {                          // theta plane [cell], pulled as in [q][cell]
           const int o7 = CX7[q] + CY7[q] * g.nx + CZ7[q] * (int)g.NXY;
           const void *thb;
           if constexpr (TAB) thb = (const void *)sp->rd[26 + q];
           else thb = (const void *)((uintptr_t)gA + g.th_off - (intptr_t)o7 * 4);
-          gg[q] = S::dec_g(S::ld_raw(bg, cell_off<S, 7>(src)), w7q(q), ld_theta(thb, n * 4u));
+          gg[q] = S::dec_g(S::ld_raw(bg, voff), w7q(q), ld_theta(thb, n * 4u));
         } else {
-          gg[q] = S::dec_g(S::ld_raw(bg, cell_off<S, 7>(src)), 0.0f, typename S::real(0));
+          gg[q] = S::dec_g(S::ld_raw(bg, voff), 0.0f, typename S::real(0));
         }
       }
     } else {   // [q][cell]: the pull offset is folded into each population's base
@@ -431,12 +438,9 @@ struct LidPull {
     } else {
       constexpr int m = MIRZ[q];              // c_m = (c_x, c_y, +1)
       const int off = CX[q] + CY[q] * g.nx;   // the lateral neighbour at the same level
-      if constexpr (S::LB != 0) {
+      if constexpr (S::LB != 0) {   // (the table's blocked bases fold in m's own pull: formed here instead)
         const uint32_t src = (uint32_t)min(max((int)n - off, 0), (int)g.N - 1);
-        const void *bf;
-        if constexpr (TAB) bf = (const void *)sp->rd[m];
-        else bf = plane<S, 19>((void *)fA, g.N, m);
-        f[q] = S::dec_f(S::ld_raw(bf, cell_off<S, 19>(src)), w19<q>());
+        f[q] = S::dec_f(S::ld_raw(plane<S, 19>((void *)fA, g.N, m), cell_off<S, 19>(src)), w19<q>());
       } else {
         const void *bf;   // the table's base of m folds in c_z(m) = +1, one plane below; undo that
         if constexpr (TAB) bf = (const void *)(sp->rd[m] + (uint64_t)g.NXY * S::bytes);
@@ -445,9 +449,7 @@ struct LidPull {
       }
       if constexpr (q < 7) {                  // q = 6: the own cell's D3Q7 population
         if constexpr (S::LB != 0) {
-          const void *bg;
-          if constexpr (TAB) bg = (const void *)sp->rd[19 + q];
-          else bg = plane<S, 7>((void *)gA, g.N, q);
+          const void *bg = plane<S, 7>((void *)gA, g.N, q);
           if constexpr (S::theta_ref) {
             const void *thb;   // the table's theta base folds in c_z = -1, one plane above; undo that
             if constexpr (TAB) thb = (const void *)(sp->rd[26 + q] - (uint64_t)g.NXY * 4u);
@@ -951,13 +953,15 @@ __device__ __forceinline__ float sweep_pull19(const Grid &g, const OwnAt &own, i
   constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
   const int NX = g.nx, NY = g.ny;
   const SweepLds &L = own.L;
-  if (cz == -1 && k == g.nz - 1) {
-    constexpr int m = MIRZ[q];
-    const int si = i - cx, sj = j - cy;
-    if (si < 0 || si >= NX || sj < 0 || sj >= NY) return own.f<m>();
-    if (k < kn[1 - cx][1 - cy]) return own.f<m>();
-    if constexpr (cx == 0 && cy == 0) return own.f<m>();
-    else return L.up[own.su][SW_UP[m]][own.t - cx - cy * TX];
+  if constexpr (cz == -1) {
+    if (k == g.nz - 1) {
+      constexpr int m = MIRZ[q];
+      const int si = i - cx, sj = j - cy;
+      if (si < 0 || si >= NX || sj < 0 || sj >= NY) return own.f<m>();
+      if (k < kn[1 - cx][1 - cy]) return own.f<m>();
+      if constexpr (cx == 0 && cy == 0) return own.f<m>();
+      else return L.up[own.su][SW_UP[m]][own.t - cx - cy * TX];
+    }
   }
   const bool outx = (cx == 1 && i == 0) || (cx == -1 && i == NX - 1);
   const bool outy = (cy == 1 && j == 0) || (cy == -1 && j == NY - 1);
@@ -1466,6 +1470,7 @@ size_t theta_offset(int storage, int lb, size_t N) {
   return (7 * N * (size_t)storage_bytes(storage) + 255) / 256 * 256;
 }
 size_t lattice_cells(int lb, size_t N) { return lb ? (N + ((size_t)1 << lb) - 1) >> lb << lb : N; }
+size_t lattice_guard_bytes(int storage, int lb, int nx) { return pull_guard_bytes(lb, storage_bytes(storage), nx); }
 // a blocked pair is one array of QB populations (the D3Q19 "lattice" covers it, the D3Q7 one lies inside it)
 size_t f_lattice_bytes(int storage, int lb, size_t N) {
   if (!lb) return 19 * N * (size_t)storage_bytes(storage);
@@ -1479,7 +1484,8 @@ size_t g_lattice_bytes(int storage, int lb, size_t N) {
 int population_layout(int storage, size_t N) {
   // blocked when every byte offset of the pair's array (and theta plane) fits 32 bits (cell_off), else [q][cell]
   const int lb = storage == 0 ? StoreF32B::LB : (storage_16bit(storage) ? StoreF16B::LB : 0);
-  return lb && f_lattice_bytes(storage, lb, N) < ((size_t)1 << 32) ? lb : 0;
+  // (a pulled offset reaches pull_dbytes beyond the lane's own cell offset)
+  return lb && f_lattice_bytes(storage, lb, N) + pull_dbytes(lb, storage_bytes(storage)) < ((size_t)1 << 32) ? lb : 0;
 }
 size_t g_lattice_offset(int storage, int lb) {
   return lb ? ((size_t)19 << lb) * (size_t)storage_bytes(storage) : 0;
@@ -1566,12 +1572,15 @@ Streams make_streams(int storage, const Grid &g, const void *fA, void *fB, const
   static const int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
   const int64_t b = storage_bytes(storage), N = g.N;
   Streams s{};
-  if (g.lb) {   // blocked layout: unshifted population bases, the pull offset goes into the per-lane cell offset
+  if (g.lb) {   // blocked layout: population bases minus the pull's uniform part, the remainder for the lane's compare
     for (int q = 0; q < 19; q++) {
-      s.rd[q] = (uint64_t)((uintptr_t)fA + ((size_t)q << g.lb) * b);
+      const int64_t off = cx[q] + (int64_t)cy[q] * g.nx + (int64_t)cz[q] * g.NXY;
+      const int64_t sh = pull_shift(g.lb, (int)b, off);
+      s.lo[q] = pull_rem(g.lb, off);
+      s.rd[q] = (uint64_t)((intptr_t)fA + (intptr_t)(((size_t)q << g.lb) * b) - (intptr_t)sh);
       s.wr[q] = (uint64_t)((uintptr_t)fB + ((size_t)q << g.lb) * b);
       if (q < 7) {
-        s.rd[19 + q] = (uint64_t)((uintptr_t)gA + ((size_t)q << g.lb) * b);
+        s.rd[19 + q] = (uint64_t)((intptr_t)gA + (intptr_t)(((size_t)q << g.lb) * b) - (intptr_t)sh);
         s.wr[19 + q] = (uint64_t)((uintptr_t)gB + ((size_t)q << g.lb) * b);
         if (storage_theta_ref(storage)) {   // the theta plane stays [cell]: pulled bases as in [q][cell]
           const int64_t o7 = cx[q] + (int64_t)cy[q] * g.nx + (int64_t)cz[q] * g.NXY;

@@ -42,6 +42,7 @@ namespace aht {
 
 constexpr float DEV_SCALE = 1024.0f;
 constexpr float DEV_INV = 1.0f / 1024.0f;
+constexpr int PAIR_QB = 26;   // populations per block of a blocked pair: 19 D3Q19, then 7 D3Q7
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, -1, 0x00020000);
@@ -53,17 +54,21 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float x) {
   return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);   // round to nearest even (finite inputs)
 }
 
-// stochastic rounding of f32 x to bfloat16: r's low 16 bits are added to the magnitude bits, which are then truncated
-// (the bit patterns of one sign are ordered like their values, so the carry rounds up with probability equal to the
-// dropped fraction)
+// Stochastic rounding with gfx950's conversion instructions, which take the random bits as a 32-bit operand:
+// v_cvt_sr_bf16_f32 adds the top 16 bits of r to the magnitude bits of x and truncates to bfloat16; v_cvt_sr_f16_f32
+// adds the top 13 bits of r below the binary16 mantissa and truncates, rounding stochastically into the binary16
+// subnormal range too, and gives infinity beyond the binary16 range (both checked lane by lane against these models
+// on the device). The bit patterns of one sign are ordered like their values, so the carry rounds up with
+// probability equal to the dropped fraction. x itself is passed as the other half of the result vector (unused), so
+// no register is cleared for it.
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint16_t f32_to_bf16_sr(float x, uint32_t r) {
-  return (uint16_t)((__float_as_uint(x) + (r & 0xffffu)) >> 16);
+  const bf16x2_t v = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(bf16x2_t, x), x, r, false);
+  return __builtin_bit_cast(uint16_t, (__bf16)v[0]);
 }
-// stochastic rounding of f32 x to binary16: 13 random bits below the binary16 mantissa, then conversion toward zero
-// (exact for the normal binary16 range; values in its subnormal range, below 6.1e-5, round toward zero more often)
 __device__ __forceinline__ _Float16 f32_to_f16_sr(float x, uint32_t r) {
-  const float y = __uint_as_float(__float_as_uint(x) + (r & 0x1fffu));
-  return (_Float16)__builtin_amdgcn_cvt_pkrtz(y, 0.0f)[0];
+  return __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(f16x2_t, x), x, r, false)[0];
 }
 
 struct StoreF32 {
@@ -180,7 +185,7 @@ struct StoreF64 {
 // stays below 4 GiB (32-bit buffer offsets).
 struct StoreF32B : StoreF32 {
   static constexpr int LB = 11;
-  static constexpr int QB = 26;   // populations per block: 19 D3Q19, then 7 D3Q7
+  static constexpr int QB = PAIR_QB;
 };
 // the same layout written through the cache: the two-step sweep writes row segments of its column tiles, which start
 // and end inside 128-byte lines that the neighbouring tile writes; non-temporal stores send such partial lines to
@@ -196,15 +201,15 @@ struct StoreF32BC : StoreF32B {
 // 600 x 600 x 64)
 struct StoreF16B : StoreF16 {
   static constexpr int LB = 8;
-  static constexpr int QB = 26;
+  static constexpr int QB = PAIR_QB;
 };
 struct StoreBF16B : StoreBF16 {
   static constexpr int LB = 8;
-  static constexpr int QB = 26;
+  static constexpr int QB = PAIR_QB;
 };
 struct StoreF16SB : StoreF16S {
   static constexpr int LB = 8;
-  static constexpr int QB = 26;
+  static constexpr int QB = PAIR_QB;
 };
 
 // element index of population q (of Q) at cell n in a lattice of N cells: [q][cell] when S::LB == 0, else
@@ -222,9 +227,41 @@ __host__ __device__ __forceinline__ size_t plane_off(int Q, size_t N, int q) {
   else return ((size_t)q << S::LB) * S::bytes;
 }
 template <class S, int Q>
-__device__ __forceinline__ uint32_t cell_off(uint32_t n) {
+__host__ __device__ __forceinline__ uint32_t cell_off(uint32_t n) {
   if constexpr (S::LB == 0) return n * (uint32_t)S::bytes;
   else return ((n >> S::LB) * ((uint32_t)S::QB << S::LB) + (n & ((1u << S::LB) - 1))) * (uint32_t)S::bytes;
+}
+
+// Pulled offsets in a blocked layout with one compare per population. Write the pull offset (in cells) as
+// off = B 2^LB + o with 0 <= o < 2^LB (B may be negative). The source cell n - off lies in block (n >> LB) - B at
+// position (n & M) - o when n's position in its block (n & M) >= o, and one block further back otherwise, so its byte
+// offset is cell_off(n) - K(off), minus D = (QB - 1) 2^LB bytes in the second case, with the uniform
+// K(off) = (B QB 2^LB + o) bytes. The stream's base takes -K(off) - D (pull_shift), and a lane picks cell_off(n) + D
+// or cell_off(n) (pull_voff), both non-negative. Sources outside [0, N) (face cells at levels 1 and nz - 2, whose
+// out-of-domain pulls are replaced afterwards) land in the guard bytes the allocation keeps around the lattice pairs
+// (pull_guard_bytes).
+__host__ __device__ __forceinline__ constexpr uint32_t pull_dbytes(int lb, int bytes) {
+  return ((uint32_t)(PAIR_QB - 1) << lb) * (uint32_t)bytes;
+}
+__host__ __device__ __forceinline__ int64_t pull_shift(int lb, int bytes, int64_t off) {
+  const int64_t B = off >> lb, o = off & (((int64_t)1 << lb) - 1);     // floor division and remainder
+  return (B * PAIR_QB * ((int64_t)1 << lb) + o) * bytes + pull_dbytes(lb, bytes);
+}
+__host__ __device__ __forceinline__ uint32_t pull_rem(int lb, int64_t off) {
+  return (uint32_t)(off & (((int64_t)1 << lb) - 1));
+}
+// bytes kept free before the first and after the last blocked lattice pair of an allocation: the fast path's pulls
+// reach at most nx cells outside [0, N) (levels 1 and nz - 2), so whole blocks covering nx + 1 cells
+__host__ __device__ __forceinline__ size_t pull_guard_bytes(int lb, int bytes, int nx) {
+  if (!lb) return 0;
+  const size_t blocks = ((size_t)nx + 1 + ((size_t)1 << lb) - 1) >> lb;
+  return (blocks * PAIR_QB << lb) * (size_t)bytes;
+}
+template <class S, int Q>
+__host__ __device__ __forceinline__ uint32_t pull_voff(uint32_t n, uint32_t o) {
+  static_assert(S::LB != 0 && S::QB == PAIR_QB, "blocked pair layouts only");
+  const uint32_t own = cell_off<S, Q>(n);
+  return (n & ((1u << S::LB) - 1)) < o ? own : own + pull_dbytes(S::LB, S::bytes);
 }
 
 // f32 theta of a theta-referenced D3Q7 lattice at a byte offset from a base

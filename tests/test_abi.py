@@ -98,3 +98,21 @@ def test_size_rules_are_checked_before_any_device_work():
         # accepted shapes get as far as the device lookup: 1200 x 1200 x 96 in f64 (plane 1.1 GB) and f16 (2.5 GB)
         assert _create_rc(m, 1200, 1200, 96, m.STORE_F64) == m.AHT_EDEVICE
         assert _create_rc(m, 1200, 1200, 96, m.STORE_F16) == m.AHT_EDEVICE
+
+
+def test_blocked_pull_addresses_stay_in_the_allocation(tmp_path):
+    """the blocked layouts' pulled addresses (storage.h pull_shift / pull_rem / pull_voff, the header the kernels
+    use), built for the host: for every cell the fast path can step and every population, the lane's address is the
+    source cell's element (also for sources outside the lattice, at levels 1 and nz - 2), its offset fits 32 bits, and
+    it lies inside the guarded allocation (tests/native/blocked_pull_check.hip)"""
+    exe = str(tmp_path / 'blocked_pull_check')
+    subprocess.run(['/opt/rocm/bin/hipcc', '-x', 'hip', '--offload-host-only', '-O2', '-std=c++17',
+                    '-I', os.path.join(PKG, 'csrc'), '-I', os.path.join(ROOT, 'include'), '-o', exe,
+                    os.path.join(ROOT, 'tests', 'native', 'blocked_pull_check.hip')], check=True)
+    grids = [(14, 12, 15, 1), (24, 20, 12, 1), (2, 2, 3, 1), (100, 100, 32, 1), (150, 150, 32, 1), (200, 200, 48, 2),
+             (300, 300, 48, 4), (2047, 3, 5, 1), (2049, 3, 5, 1), (255, 3, 4, 1), (256, 3, 4, 1), (6000, 3, 4, 1),
+             (3, 2000, 5, 1), (600, 600, 64, 16)]
+    args = [str(x) for g in grids for x in g]
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count('pulled addresses checked') == 3 * len(grids), r.stdout
