@@ -19,13 +19,17 @@
 // 2-byte elements in blocks of 256 cells, with the f32 theta array of the 16-bit formats pulled beside it.
 // `lattice16s_pair_pull_nt` is the f16s step's pattern (the 16-bit pair without theta array);
 // `lattice_colmajor_pull_nt` the f32 pair with column blocks outermost and levels inside them; `lattice_pair_aa_*` and
-// `lattice_pair_esotwist*` the in-place patterns in the pair layout.
+// `lattice_pair_esotwist*` the in-place patterns in the pair layout. `lattice_pair_{noshift,zonly,xonly,yonly}_nt`
+// and `lattice16s_pair_{noshift,zonly,xonly,yonly}_nt` take the pair patterns apart: every stream read at the cell
+// itself, or pulled along z, x or y only; `lattice_pair_pull_nt_w{2,4,16,64}` and `lattice16s_pair_pull_nt_w{...}`
+// give the pull pattern the step's XCD-aware tile order (runs of W consecutive 256-cell tiles per XCD).
 // Usage: aht_copy_peak [bytes] [reps] [variant]
 //   (variant: all | grid | flat | unroll4 | unroll4_xcd | unroll4_nt | lattice | lattice_pull | lattice_aa_even
 //    | lattice_aa_odd | lattice_pull_nt | lattice16_pull_nt | lattice_aosoa_pull_nt | lattice_pull_nt_x2
 //    | lattice_pull_nt_x4 | lattice_blocked_pull_nt | lattice_pair_pull_nt | lattice16_pair_pull_nt
 //    | lattice16s_pair_pull_nt | lattice_colmajor_pull_nt | lattice_pair_aa_even | lattice_pair_aa_odd
-//    | lattice_pair_esotwist | lattice_pair_aa_even_nt | lattice_pair_aa_odd_nt | lattice_pair_esotwist_nt)
+//    | lattice_pair_esotwist | lattice_pair_aa_even_nt | lattice_pair_aa_odd_nt | lattice_pair_esotwist_nt
+//    | lattice[16s]_pair_{noshift,zonly,xonly,yonly}_nt | lattice[16s]_pair_pull_nt_w{2,4,16,64})
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -219,6 +223,38 @@ __global__ __launch_bounds__(256) void copy_lattice_pair(const float *__restrict
     __builtin_nontemporal_store(v[q], &b[((((size_t)n >> LB) * 26 + q) << LB) | ((size_t)n & M)]);
 }
 
+// the pair patterns taken apart (T = float, LB = 11: the f32 step's; T = uint16_t, LB = 8: the f16s step's). MODE:
+// 0 pull, 1 every stream at the cell itself, 2 pulled along z only, 3 along x only, 4 along y only. Tile t of 256 cells
+// is taken by block b in the step's XCD-aware order (tile_of_block with window `win`; 1 = linear).
+__device__ __forceinline__ uint32_t tile_of(uint32_t b, uint32_t G, uint32_t win) {
+  if (win <= 1) return b;
+  const uint32_t x = b & 7u, i = b >> 3, lw = (uint32_t)__builtin_ctz(win), span_mask = (win << 3) - 1u;
+  if (b >= (G & ~span_mask)) return b;
+  return ((i >> lw) << (lw + 3)) + (x << lw) + (i & (win - 1u));
+}
+template <class T, int LB, int MODE>
+__global__ __launch_bounds__(256) void copy_pair_parts(const T *__restrict__ a, T *__restrict__ b, uint32_t N, int nx,
+                                                       int nxy, uint32_t win) {
+  const uint32_t n = tile_of(blockIdx.x, gridDim.x, win) * 256 + threadIdx.x;
+  if (n >= N) return;
+  constexpr uint32_t M = (1u << LB) - 1;
+  T v[26];
+#pragma unroll
+  for (int q = 0; q < 26; q++) {
+    int cx = LCX[q], cy = LCY[q], cz = LCZ[q];
+    if (MODE == 1) cx = cy = cz = 0;
+    if (MODE == 2) cx = cy = 0;
+    if (MODE == 3) cy = cz = 0;
+    if (MODE == 4) cx = cz = 0;
+    int64_t src = (int64_t)n - (cx + cy * nx + cz * nxy);
+    src = src < 0 ? 0 : (src >= (int64_t)N ? (int64_t)N - 1 : src);
+    v[q] = a[((((size_t)src >> LB) * 26 + q) << LB) | ((size_t)src & M)];
+  }
+#pragma unroll
+  for (int q = 0; q < 26; q++)
+    __builtin_nontemporal_store(v[q], &b[((((size_t)n >> LB) * 26 + q) << LB) | ((size_t)n & M)]);
+}
+
 // 16-bit populations interleaved as the f32 pair, [cell >> LB][26][cell & (2^LB - 1)], and (THETA) an f32 theta
 // array [cell] pulled at the 7 D3Q7 offsets, non-temporal stores; without THETA the pattern of the f16s step
 template <int LB, bool THETA>
@@ -326,19 +362,27 @@ int main(int argc, char **argv) {
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
-  const char *names[25] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
+  const int NV = 41;
+  const char *names[NV] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
                            "lattice_aa_even", "lattice_aa_odd", "lattice_pull_nt", "lattice16_pull_nt",
                            "lattice_aosoa_pull_nt", "lattice_pull_nt_x2", "lattice_pull_nt_x4",
                            "lattice_blocked_pull_nt", "lattice_pair_pull_nt", "lattice16_pair_pull_nt",
                            "lattice16s_pair_pull_nt", "lattice_colmajor_pull_nt", "lattice_pair_aa_even",
                            "lattice_pair_aa_odd", "lattice_pair_esotwist", "lattice_pair_aa_even_nt",
-                           "lattice_pair_aa_odd_nt", "lattice_pair_esotwist_nt"};
+                           "lattice_pair_aa_odd_nt", "lattice_pair_esotwist_nt",
+                           "lattice_pair_noshift_nt", "lattice_pair_zonly_nt", "lattice_pair_xonly_nt",
+                           "lattice_pair_yonly_nt", "lattice_pair_pull_nt_w2", "lattice_pair_pull_nt_w4",
+                           "lattice_pair_pull_nt_w16", "lattice_pair_pull_nt_w64",
+                           "lattice16s_pair_noshift_nt", "lattice16s_pair_zonly_nt", "lattice16s_pair_xonly_nt",
+                           "lattice16s_pair_yonly_nt", "lattice16s_pair_pull_nt_w2", "lattice16s_pair_pull_nt_w4",
+                           "lattice16s_pair_pull_nt_w16", "lattice16s_pair_pull_nt_w64"};
+  const uint32_t wins[4] = {2, 4, 16, 64};
   const int lnx = 600, lnxy = 600 * 600;
   const uint32_t lN = 600u * 600u * 64u;   // lattice variants: 26 * lN floats each way (2.4 GB)
   double best = 0;
   const char *best_name = "";
   const uint32_t ncb = (lnxy + 2047) / 2048;     // column blocks of the column-major variant (2.44 GB per buffer)
-  for (int v = 0; v < 25; v++) {
+  for (int v = 0; v < NV; v++) {
     if (strcmp(which, "all") && strcmp(which, names[v])) continue;
     if (v >= 5 && n * 16 < (size_t)26 * ncb * 2048 * 64 * 4) continue;   // the lattices need 2.44 GB per buffer
     auto launch = [&]() {
@@ -368,6 +412,18 @@ int main(int argc, char **argv) {
         case 22: hipLaunchKernelGGL((copy_lattice_inplace_pair<11, 0, true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
         case 23: hipLaunchKernelGGL((copy_lattice_inplace_pair<11, 1, true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
         case 24: hipLaunchKernelGGL((copy_lattice_inplace_pair<11, 2, true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
+        case 25: hipLaunchKernelGGL((copy_pair_parts<float, 11, 1>), dim3(lN / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy, 1u); break;
+        case 26: hipLaunchKernelGGL((copy_pair_parts<float, 11, 2>), dim3(lN / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy, 1u); break;
+        case 27: hipLaunchKernelGGL((copy_pair_parts<float, 11, 3>), dim3(lN / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy, 1u); break;
+        case 28: hipLaunchKernelGGL((copy_pair_parts<float, 11, 4>), dim3(lN / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy, 1u); break;
+        case 29: case 30: case 31: case 32:
+          hipLaunchKernelGGL((copy_pair_parts<float, 11, 0>), dim3(lN / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy, wins[v - 29]); break;
+        case 33: hipLaunchKernelGGL((copy_pair_parts<uint16_t, 8, 1>), dim3(lN / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy, 1u); break;
+        case 34: hipLaunchKernelGGL((copy_pair_parts<uint16_t, 8, 2>), dim3(lN / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy, 1u); break;
+        case 35: hipLaunchKernelGGL((copy_pair_parts<uint16_t, 8, 3>), dim3(lN / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy, 1u); break;
+        case 36: hipLaunchKernelGGL((copy_pair_parts<uint16_t, 8, 4>), dim3(lN / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy, 1u); break;
+        case 37: case 38: case 39: case 40:
+          hipLaunchKernelGGL((copy_pair_parts<uint16_t, 8, 0>), dim3(lN / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy, wins[v - 37]); break;
       }
     };
     launch();
@@ -378,7 +434,8 @@ int main(int argc, char **argv) {
     CK(hipEventSynchronize(e1));
     float ms = 0;
     CK(hipEventElapsedTime(&ms, e0, e1));
-    const size_t moved = (v == 10 || v == 16) ? (size_t)28 * lN * 2 : (v == 17 ? (size_t)26 * lN * 2 : (v >= 5 ? (size_t)26 * lN * 4 : n * 16));
+    const size_t moved = (v == 10 || v == 16) ? (size_t)28 * lN * 2
+                         : ((v == 17 || v >= 33) ? (size_t)26 * lN * 2 : (v >= 5 ? (size_t)26 * lN * 4 : n * 16));
     const double per = ms / reps, gbs = 2.0 * moved / (per * 1e-3) / 1e9;
     printf("{\"kernel\": \"copy_%s\", \"bytes_copied\": %zu, \"reps\": %d, \"ms_per_launch\": %.5f, "
            "\"GBps_read_plus_write\": %.1f, \"cus\": %d}\n", names[v], moved, reps, per, gbs, cus);
