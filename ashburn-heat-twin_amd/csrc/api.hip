@@ -487,7 +487,7 @@ int aht_stats(aht_solver *s, aht_diag *out) {
 }  // extern "C"
 
 // Scratch of aht_stats_noalloc: the partials of the diagnostics (7 per stats block, at most STATS_MAX_BLOCKS blocks)
-// and the 7 results after them, 131,128 bytes held by every GPU the library runs on until the process ends. In host
+// and the 7 results after them (8 slots), 114,752 bytes held by every GPU the library runs on until the process ends. In host
 // code `scratch` names a host-side stand-in, not device memory: the device address comes from hipGetSymbolAddress,
 // once per device with that device current, and only that address is passed to kernels.
 constexpr size_t SCRATCH_DOUBLES = 7 * (size_t)aht::STATS_MAX_BLOCKS + 8;

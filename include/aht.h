@@ -127,8 +127,8 @@ int aht_read_volume(aht_solver *s, int32_t nzv, float volmax, uint8_t *vol, floa
 int aht_stats(aht_solver *s, aht_diag *out);
 /* aht_stats without device allocation (no counterpart in the reference): for callers that cannot call hipMalloc
  * (under HIP graph capture, in tight serving loops, harnesses that forbid allocation). Same kernels and the same
- * results bit for bit, but the partials live in a fixed library-owned scratch of 131,128 bytes per GPU (7 x 2,048
- * partial blocks + 7 results; the diagnostics kernel strides over any solver size within 2,048 blocks, and the call
+ * results bit for bit, but the partials live in a fixed library-owned scratch of 114,752 bytes per GPU (7 x 2,048
+ * partials + 8 result slots; the diagnostics kernel strides over any solver size within 2,048 blocks, and the call
  * fails with AHT_EINVAL if it ever needed more). Every call on one GPU shares that GPU's scratch: calls that overlap
  * on one GPU (solvers stepped from different threads) race and must be serialised by the caller. */
 int aht_stats_noalloc(aht_solver *s, aht_diag *out);
