@@ -768,13 +768,13 @@ size_t best_candidate(const aht_batch *b) {
   return best;
 }
 
-// tiles of one step of solver s with blocks of `block` columns and its own levels per thread
+// tiles of one step of solver s with blocks of `block` columns, one level per thread (batched launches step every
+// member so, whatever its own launch shape: a kernel with a per-member level loop held 30-40 more VGPRs)
 void member_tiles(const aht_solver *s, int block, aht::Member &M) {
   const aht::Grid &g = s->g;
-  M.levels = s->levels;
   M.bx = (g.NXY + (uint32_t)block - 1) / (uint32_t)block;
   M.inv_bx = 1.0f / (float)M.bx;
-  M.tiles = g.kmin < g.nz ? M.bx * (uint32_t)((g.nz - g.kmin + M.levels - 1) / M.levels) : 0;
+  M.tiles = g.kmin < g.nz ? M.bx * (uint32_t)(g.nz - g.kmin) : 0;
 }
 }  // namespace
 
@@ -917,6 +917,7 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
     M.nsteps = nsteps ? nsteps[d] : n;
     M.acc_from = acc_from ? acc_from[d] : n;
     M.step0 = (uint32_t)s->steps;
+    for (int q = 0; q < 2; q++) M.s[q] = aht::make_streams(s->storage, s->g, s->f[q], s->f[1 - q], s->gq[q], s->gq[1 - q]);
     // the batch stream waits for work the member enqueued on its own stream
     HIP_TRYB(b, hipEventRecord(s->own, s->st));
     HIP_TRYB(b, hipStreamWaitEvent(b->st, s->own, 0));

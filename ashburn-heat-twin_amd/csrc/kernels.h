@@ -58,15 +58,15 @@ struct Member {
   void *f[2];
   void *gq[2];
   Acc acc;
-  uint32_t tiles;      // (block of BS columns, chunk of `levels` levels) tiles of one step
+  uint32_t tiles;      // (block of BS columns, one level) tiles of one step
   uint32_t tile_base;  // first tile in the concatenated list
   uint32_t bx;         // column blocks per level chunk: ceil(NXY / BS)
   float inv_bx;        // 1 / bx
-  int levels;          // cells per thread along z (the member's own launch shape)
   int parity0;         // lattice holding the state at launch index 0 of the chunk
   int nsteps;          // launches of the chunk in which this member steps (the table is ordered by it, descending)
   int acc_from;        // first launch index that accumulates the FP64 means (>= nsteps: none)
   uint32_t step0;      // the member's step index at launch index 0 of the chunk
+  Streams s[2];        // stream bases of the member's lattice parities (make_streams), read as the single step does
 };
 
 size_t face_offset(int face, int nx, int ny, int nz);   // offset of face 0..3 in Acc::face
@@ -105,7 +105,7 @@ hipError_t launch_sweep2(int storage, const Grid &g, const Grid *gd, const void 
                          void *gB, int tx, int kc, uint32_t win, uint32_t step, hipStream_t st);
 
 // one step of the first nmem batch members (launch index `it` of a chunk; their tiles are the first total_tiles) on
-// a grid of min(grid, total_tiles) blocks of `block` (64, 128 or 256) threads
+// a grid of min(grid, total_tiles) blocks of `block` (64, 128 or 256) threads, one level per thread
 hipError_t launch_batch_step(int storage, int lb, int block, const Member *mem, int nmem, uint32_t total_tiles, int it,
                              int grid, hipStream_t st);
 hipError_t launch_init(int storage, const Grid &g, void *f0, void *f1, hipStream_t st);

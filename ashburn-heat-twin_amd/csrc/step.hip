@@ -126,8 +126,9 @@ __device__ __forceinline__ typename S::real ldg(const Grid &g, const void *gA, i
   else return S::dec_g(e, 0.0f, typename S::real(0));
 }
 
-// Stream bases. TAB = true (the single-domain step) reads them from the host-built Streams table; TAB = false
-// (batched launches, the surface read) forms them from the lattice pointers.
+// Stream bases. TAB = true (the single-domain step, batched launches) reads them from a host-built Streams table;
+// TAB = false (the two-step sweep, the surface read) forms them from the lattice pointers. Forming them on the device
+// keeps ~60 64-bit bases live in scalar registers: in the batched kernel that spilled ~980 SGPRs into VGPR lanes.
 template <class S, bool TAB, int Q>
 __device__ __forceinline__ void *plane_base(const Streams *sp, int slot, void *lat, size_t N, int q) {
   if constexpr (TAB) return (void *)sp->wr[slot];
@@ -742,12 +743,13 @@ __device__ __forceinline__ void store_post(const Grid &g, const Streams *sp, voi
   store_g<S, TAB>(g, sp, gB, n, gp, dz);
 }
 
-// one cell of the fused step (genStepWGSL, js/live/kernel.js:395-431) with FP64 accumulation (ACC) and debug outputs
-// (OUT). Every thread of the block must call it (the ACC heat-content sum is a wavefront reduction).
+// one cell of the fused step (genStepWGSL, js/live/kernel.js:395-431) with FP64 accumulation (ACC, and at run time
+// acc_on, which must be uniform over the block) and debug outputs (OUT). Every thread of the block must call it (the
+// ACC heat-content sum is a wavefront reduction).
 template <class S, bool ACC, bool OUT, bool TAB>
 __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *sp, const void *fA, void *fB,
                                                  const void *gA, void *gB, const Acc &acc, float *__restrict__ out4,
-                                                 int c, int k, uint32_t ck, uint32_t step) {
+                                                 int c, int k, uint32_t ck, uint32_t step, bool acc_on = true) {
   using R = typename S::real;
   const int ktc = col_kt(ck), kg = col_ktop(ck);
   const bool fluid = c < (int)g.NXY && k >= kg && k < g.nz;
@@ -782,7 +784,7 @@ __device__ __forceinline__ double step_cell_body(const Grid &g, const Streams *s
       out4[2 * N + n] = (float)F.uy;
       out4[3 * N + n] = (float)F.uz;
     }
-    if (ACC) {
+    if (ACC && acc_on) {
       th_cell = (double)m.th;
       if (k == kg) {
         acc.surf[c] += (double)m.th;
@@ -1163,53 +1165,65 @@ __global__ __launch_bounds__(SWEEP_THREADS) void sweep2_kernel(const Grid *__res
 
 // ---- many domains in one launch ----------------------------------------------------------------------------------
 // One grid of G blocks of BS threads walks one step of the batch members that step in this launch. Member d
-// contributes tiles (BS-column block, chunk of M.levels levels) for its fluid levels, concatenated in table order. The
+// contributes tiles (BS-column block, one level) for its fluid levels, concatenated in table order. The
 // host orders the table by step count, longest first, so the members stepping in launch `it` of a chunk are a prefix
 // of it and the launch covers only their tiles (nmem, total): a member that sits out a launch costs no workgroups.
 // The grid strides through the list in windows of G tiles, taking the tiles of a window in the XCD-aware order of the
 // single-domain step (runs of 64 tiles per XCD); with one block per tile (G = total, the default) block b takes tile
-// b. Per-member fields make a member's result independent of the mix: it reads lattice parity0 ^ (it & 1) and
-// accumulates means from launch acc_from on. BS (64, 128, 256) and the grid are the batch tuner's choice.
-template <class S, bool ACC>
-__device__ __forceinline__ void batch_tile(const Member &M, int p, int c, int k0, uint32_t step) {
+// b. Per-member fields make a member's result independent of the mix: it reads lattice parity0 ^ (it & 1) through the
+// stream table of that parity (M.s, host-built per chunk) and accumulates means from launch acc_from on. BS (64, 128,
+// 256) and the grid are the batch tuner's choice.
+// One copy of the cell body with the accumulation switched at run time, one level per thread, and (PERSIST = false)
+// no loop at all: a compile-time ACC / non-ACC pair of copies in one kernel, or loops over a persistent grid's tiles and
+// a member's levels, raised the kernel from the single step's 80-84 VGPRs to 90-126 and spilled 240-990 SGPRs into
+// VGPR lanes (f16s at 600 x 600 x 64 batched alone: 0.66-0.81 ms against 0.45 ms now).
+template <class S>
+__device__ __forceinline__ void batch_tile(const Member &M, int p, int c, int k, uint32_t step, bool acc) {
   const Grid &g = M.g;
   const uint32_t ck = ld_tab(g.colk, (size_t)(c < (int)g.NXY ? c : 0));
-  const int c0 = c - (int)(threadIdx.x & 63);
-  for (int u = 0; u < M.levels; u++) {
-    const double th = step_cell_body<S, ACC, false, false>(g, nullptr, M.f[p], M.f[1 - p], M.gq[p], M.gq[1 - p],
-                                                           M.acc, nullptr, c, k0 + u, ck, step);
-    if (ACC) content_add(g, M.acc, k0 + u, c0, th);
-  }
+  const double th = step_cell_body<S, true, false, true>(g, &M.s[p], M.f[p], M.f[1 - p], M.gq[p], M.gq[1 - p], M.acc,
+                                                         nullptr, c, k, ck, step, acc);
+  if (acc) content_add(g, M.acc, k, c - (int)(threadIdx.x & 63), th);
 }
 
-template <class S, int BS>
+// the member of tile t: binary search over tile_base (ascending)
+__device__ __forceinline__ int member_of(const Member *__restrict__ mem, int nmem, uint32_t t) {
+  int lo = 0, hi = nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mem[mid].tile_base <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// PERSIST = false: one block per tile; PERSIST = true: a grid of G < total blocks walking the tiles
+template <class S, int BS, bool PERSIST>
 __global__ __launch_bounds__(BS) void batch_step_kernel(const Member *__restrict__ mem, int nmem, uint32_t total,
                                                         int it) {
-  const uint32_t G = gridDim.x, tb = G >= total ? blockIdx.x : tile_of_block(blockIdx.x, G, 64);
-  // member of the block's first tile by binary search over tile_base (ascending), then advanced as t grows
-  int d = 0;
-  if (tb < total) {
-    int lo = 0, hi = nmem - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (mem[mid].tile_base <= tb) lo = mid; else hi = mid - 1;
-    }
-    d = lo;
-  }
-  for (uint32_t base = 0; base < total; base += G) {
-    const uint32_t t = base + tb;
-    if (t >= total) continue;                                   // partial last window
-    while (d + 1 < nmem && mem[d + 1].tile_base <= t) d++;      // t grows with base: the member index only advances
-    const Member &M = mem[d];
-    if (it >= M.nsteps) continue;                               // not reached: the launch covers stepping members
+  if constexpr (!PERSIST) {
+    const uint32_t t = blockIdx.x;
+    if (t >= total) return;
+    const Member &M = mem[member_of(mem, nmem, t)];
+    if (it >= M.nsteps) return;                                  // not reached: the launch covers stepping members
     uint32_t cb;
     const uint32_t kk = div_by(t - M.tile_base, M.bx, M.inv_bx, cb);
-    const int c = (int)(cb * BS + threadIdx.x);
-    const int k0 = M.g.kmin + (int)kk * M.levels;
-    const int p = M.parity0 ^ (it & 1);
-    const uint32_t step = M.step0 + (uint32_t)it;                 // the member's own step index
-    if (it >= M.acc_from) batch_tile<S, true>(M, p, c, k0, step);
-    else batch_tile<S, false>(M, p, c, k0, step);
+    batch_tile<S>(M, M.parity0 ^ (it & 1), (int)(cb * BS + threadIdx.x), M.g.kmin + (int)kk, M.step0 + (uint32_t)it,
+                  it >= M.acc_from);
+  } else {
+    const uint32_t G = gridDim.x, tb = tile_of_block(blockIdx.x, G, 64);
+    // member of the block's first tile, then advanced as t grows
+    int d = tb < total ? member_of(mem, nmem, tb) : 0;
+    for (uint32_t base = 0; base < total; base += G) {
+      const uint32_t t = base + tb;
+      if (t >= total) continue;                                   // partial last window
+      while (d + 1 < nmem && mem[d + 1].tile_base <= t) d++;      // t grows with base: the member index only advances
+      const Member &M = mem[d];
+      if (it >= M.nsteps) continue;                               // not reached: the launch covers stepping members
+      uint32_t cb;
+      const uint32_t kk = div_by(t - M.tile_base, M.bx, M.inv_bx, cb);
+      batch_tile<S>(M, M.parity0 ^ (it & 1), (int)(cb * BS + threadIdx.x), M.g.kmin + (int)kk,
+                    M.step0 + (uint32_t)it, it >= M.acc_from);
+    }
   }
 }
 
@@ -1565,7 +1579,7 @@ hipError_t launch_sweep2(int storage, const Grid &g, const Grid *gd, const void 
   return hipGetLastError();
 }
 
-// the bases FastPull / StorePairs / store_g form on the device with TAB = false, for one lattice parity
+// the stream bases of one lattice parity, host-built (the same the device forms with TAB = false)
 Streams make_streams(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB) {
   static const int cx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
   static const int cy[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
@@ -1616,13 +1630,18 @@ hipError_t launch_zero_g_blocked(int storage, const Grid &g, void *gq, hipStream
 hipError_t launch_batch_step(int storage, int lb, int block, const Member *mem, int nmem, uint32_t total_tiles, int it,
                              int grid, hipStream_t st) {
   if (total_tiles == 0 || nmem == 0) return hipSuccess;
-  const int G = (int)std::min<uint32_t>((uint32_t)grid, total_tiles);
+  const uint32_t G = std::min<uint32_t>((uint32_t)grid, total_tiles);
+  const bool persist = G < total_tiles;
+#define AHT_BATCH_LAUNCH(BS)                                                                                          \
+  if (persist) hipLaunchKernelGGL((batch_step_kernel<S, BS, true>), dim3(G), dim3(BS), 0, st, mem, nmem, total_tiles, it); \
+  else hipLaunchKernelGGL((batch_step_kernel<S, BS, false>), dim3(G), dim3(BS), 0, st, mem, nmem, total_tiles, it);
   AHT_DISPATCH(storage, lb, {
-    if (block == 64) hipLaunchKernelGGL((batch_step_kernel<S, 64>), dim3(G), dim3(64), 0, st, mem, nmem, total_tiles, it);
-    else if (block == 128) hipLaunchKernelGGL((batch_step_kernel<S, 128>), dim3(G), dim3(128), 0, st, mem, nmem, total_tiles, it);
-    else if (block == 256) hipLaunchKernelGGL((batch_step_kernel<S, 256>), dim3(G), dim3(256), 0, st, mem, nmem, total_tiles, it);
+    if (block == 64) { AHT_BATCH_LAUNCH(64) }
+    else if (block == 128) { AHT_BATCH_LAUNCH(128) }
+    else if (block == 256) { AHT_BATCH_LAUNCH(256) }
     else return hipErrorInvalidValue;
   })
+#undef AHT_BATCH_LAUNCH
   return hipGetLastError();
 }
 

@@ -3,9 +3,9 @@
 // one inserted campus each; dump.js batchMember) advanced together, one launch per step for all of them with hipGraph-
 // captured chunks, against the same scenarios run one at a time with the plain per-solver step.
 //
-// Launch shapes: the levels per thread of each grid shape come from the launch-shape autotuner (aht_autotune, run
-// once per distinct grid on the first domain of that shape and reused for the others); the block size of the batched
-// launch is tuned online by the batch (aht_batch_set_block 0).
+// Launch shapes: the one-at-a-time runs use the launch-shape autotuner's pick per grid shape (aht_autotune, run once per
+// distinct grid on the first domain of that shape and reused for the others); the batched launch steps every member
+// one level per thread, its block size tuned online by the batch (aht_batch_set_block 0).
 //
 // Usage: node batch_run.js [--domains 64] [--steps 3000] [--chunk 50] [--seed 4] [--mode batch,sequential]
 //        [--tune-steps 4 (0: no per-grid tuning)] [--block 0|64|128|256] [--check all|N (members compared bit for

@@ -16,7 +16,8 @@ sample of the same workload: whole steps on the same 600 x 600 x 64 domain for a
 Also measured in the same run: the 16-bit storage formats on the same domain (`reduced_precision`: f16 without theta
 plane and f16 with its f32 theta plane) and the serving
 half of the metric (`serving`: --serve-requests requests at --serve req/s through the Node serving loop, continuous
-batching against one-at-a-time execution; about three minutes).
+batching against one-at-a-time execution with the plain f32 step, and the same requests batched in f16s storage;
+about four minutes).
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -67,29 +68,45 @@ def traffic_from_profiles(kernel_tag):
         return None
 
 
-def serving(rate, requests):
+def run_serve(rate, requests, modes, storage, out):
+    """one Node serving-loop process; returns its record (the record is written last: a failure after it shows in
+    the exit status, which is reported)"""
+    p = subprocess.run(['node', os.path.join(PKG, 'js', 'serve.js'), '--rate', str(rate), '--requests', str(requests),
+                        '--mode', modes, '--storage', str(storage), '--out', out], stdout=subprocess.DEVNULL)
+    if not os.path.exists(out):
+        raise RuntimeError(f'serving loop failed (exit status {p.returncode}) before writing its record')
+    return json.load(open(out)), p.returncode
+
+
+def serving(rate, requests, f16s=True):
     """the serving half of the metric (scenarios/hour, p50/p99 latency): measured live by the Node serving loop
     (js/serve.js: Poisson arrivals at `rate`, the serving grid mix, 2,200 s spin-up + 10-minute mean per request,
-    continuous batching against the same requests one at a time); rate 0 skips it"""
+    continuous batching against the same requests one at a time with the plain f32 step); with f16s, the same requests
+    batched once more in f16s storage (stochastic rounding) against the same f32 sequential run, with the largest
+    relative difference of any request's mean warming from its f32 result; rate 0 skips it"""
     if rate <= 0:
         return None
     d = tempfile.mkdtemp(prefix='aht_serve_')
-    out = os.path.join(d, 'serve.json')
     t0 = time.perf_counter()
-    p = subprocess.run(['node', os.path.join(PKG, 'js', 'serve.js'), '--rate', str(rate), '--requests', str(requests),
-                        '--mode', 'batch,sequential', '--out', out], stdout=subprocess.DEVNULL)
-    if not os.path.exists(out):
-        raise RuntimeError(f'serving loop failed (exit status {p.returncode}) before writing its record')
-    r = json.load(open(out))
-    subprocess.run(['rm', '-rf', d])
+    r, rc = run_serve(rate, requests, 'batch,sequential', 0, os.path.join(d, 'serve.json'))
     b, q = r['batch']['summary'], r['sequential']['summary']
-    return {'measured_in_this_run': True, 'rate_req_s': rate, 'requests': requests,
-            'node_exit_status': p.returncode,        # the record is written last; a failure after it shows here
-            'wall_s': round(time.perf_counter() - t0, 1),
-            'scenarios_per_hour': b['scenarios_per_hour'], 'latency_p50_s': b['latency_p50_s'],
-            'latency_p99_s': b['latency_p99_s'],
-            'vs_sequential': round(b['scenarios_per_hour'] / q['scenarios_per_hour'], 4),
-            'batch': b, 'sequential': q}
+    out = {'measured_in_this_run': True, 'rate_req_s': rate, 'requests': requests, 'node_exit_status': rc,
+           'wall_s': None,
+           'scenarios_per_hour': b['scenarios_per_hour'], 'latency_p50_s': b['latency_p50_s'],
+           'latency_p99_s': b['latency_p99_s'],
+           'vs_sequential': round(b['scenarios_per_hour'] / q['scenarios_per_hour'], 4),
+           'batch': b, 'sequential': q}
+    if f16s:
+        r16, rc16 = run_serve(rate, requests, 'batch', 4, os.path.join(d, 'serve16.json'))
+        b16 = r16['batch']['summary']
+        ref = {x['id']: x['warmingMean'] for x in r['sequential']['results']}
+        diff = max(abs(x['warmingMean'] - ref[x['id']]) / abs(ref[x['id']]) for x in r16['batch']['results'])
+        out['batch_f16s'] = dict(b16, node_exit_status=rc16,
+                                 vs_sequential_f32=round(b16['scenarios_per_hour'] / q['scenarios_per_hour'], 4),
+                                 warming_mean_max_rel_diff_vs_f32=float(f'{diff:.3g}'))
+    out['wall_s'] = round(time.perf_counter() - t0, 1)
+    subprocess.run(['rm', '-rf', d])
+    return out
 
 
 def timed_step(m, dom, storage, steps, warmup, launch, tune_steps):
@@ -207,6 +224,10 @@ def main():
         log(f'serving sample: {a.serve_requests} requests at {a.serve} req/s, batch and sequential ...')
         serve = serving(a.serve, a.serve_requests)
         log('serving', {k: serve[k] for k in ('scenarios_per_hour', 'latency_p50_s', 'latency_p99_s', 'vs_sequential')})
+        if 'batch_f16s' in serve:
+            log('serving, batched in f16s', {k: serve['batch_f16s'][k] for k in (
+                'scenarios_per_hour', 'latency_p50_s', 'latency_p99_s', 'vs_sequential_f32',
+                'warming_mean_max_rel_diff_vs_f32')})
     out = {
         'metric': METRIC, 'value': round(mlups, 2), 'unit': 'MLUPS', 'n_gpus': 1, 'steps': a.steps,
         'warmup': a.warmup, 'ms_per_step': round(ms_step, 5), 'higher_is_better': True, 'scaling': 'weak',

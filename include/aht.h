@@ -159,8 +159,8 @@ void aht_batch_destroy(aht_batch *b);
 const char *aht_batch_last_error(const aht_batch *b);
 /* members: same device and storage and one population layout (lattice pairs below 4 GiB are blocked on the device
  * in every format but f64 — f32 by 2048 cells, f16 / bf16 by 256 with their theta plane — larger pairs and f64 keep
- * [q][cell]; a batch cannot mix the two). A member keeps its own levels per thread (aht_set_launch / aht_autotune);
- * the batch's block size applies to all members of a launch. */
+ * [q][cell]; a batch cannot mix the two). Batched launches step every member one level per thread, with the batch's
+ * block size; a member's own launch shape (aht_set_launch / aht_autotune) applies to its steps outside the batch. */
 int aht_batch_add(aht_batch *b, aht_solver *s);
 int aht_batch_remove(aht_batch *b, aht_solver *s);
 int aht_batch_size(const aht_batch *b);

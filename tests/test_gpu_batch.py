@@ -2,8 +2,8 @@
 
 The bar is exact: a member's state and FP64 means after a batched run must be bit-identical to the same domain run
 alone through aht_step with the same steps and averaging window, whatever the mix, the chunking, the block size and
-grid (online-tuned or fixed), the members' levels per thread, graph capture on or off, and changes of a member's
-columns or launch shape between chunks. Tiny domains of different sizes, winds and outlets.
+grid (online-tuned or fixed), the members' own launch shapes (which the batched launch does not use), graph capture on
+or off, and changes of a member's columns or launch shape between chunks. Tiny domains of different sizes, winds and outlets.
 """
 import numpy as np
 import pytest
@@ -150,8 +150,8 @@ def test_members_join_and_leave_mid_run(tmp_path, pkg):
 
 def test_member_columns_and_levels_change_between_chunks(tmp_path, pkg):
     """A captured chunk is replayed only while every launch keeps its member and tile counts: raising a member's lowest
-    fluid level (fewer tiles) or its levels per thread between chunks re-captures, and the member still gives the
-    bits of the same changes applied alone"""
+    fluid level (fewer tiles) between chunks re-captures, and the member, whose own launch shape also changes, still
+    gives the bits of the same changes applied alone"""
     doms = domains(tmp_path)
     d0 = doms[0]
     ktop2 = np.maximum(d0['ktop'], d0['ktop'].min() + 2).astype(np.uint32)    # lowest fluid level up by two

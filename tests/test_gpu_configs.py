@@ -13,8 +13,8 @@
 * 1200 x 1200 x 96 (reduced-precision storage): f16 and bf16 deviations and f16 without theta plane against the FP64
   twin on the same domain, 1,000 steps plus a 500-step mean (shortened from 5,000): mean near-ground warming within
   1e-3 relative for each 16-bit format, the f32 build within 1e-5.
-* 64 batched uneven domains: one member of each grid size stepped 200 times in one launch per step (tuned block size,
-  different levels per thread) and alone with the plain step, states bit-identical; a 100 x 100 x 32 member
+* 64 batched uneven domains: one member of each grid size stepped 200 times in one launch per step (tuned block size)
+  and alone with the plain step (each grid's own launch shape), states bit-identical; a 100 x 100 x 32 member
   stepped in a batch against the oracle (50 steps from rest within 1e-5, then one step from the batch's own state:
   theta within 1e-5).
 * Serving: requests of the serving grid mix (100 x 100 x 32, 200 x 200 x 48, two 600 x 600 x 64) in one batch with the
