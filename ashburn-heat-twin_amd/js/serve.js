@@ -15,8 +15,16 @@
 // of one big step per launch, and the big one resumes its full rate as soon as they leave. Policy 'fifo' steps every
 // member in every launch.
 //
-// The baseline (`mode: 'sequential'`) runs the same requests one at a time with the plain per-solver path.
+// Storage: one format for every request (--storage 0 f32, 4 f16s, ...), or `auto`: f16s (f16 deviations, stochastic
+// rounding) for requests whose grid spacing is at most --f16s-max-dx metres (default 10) and f32 for coarser ones, each
+// format in its own batch (a batch has one format), the two batches stepped chunk by chunk in turn. Measured on this
+// grid mix against the same requests in f32: the 10 m requests differ by at most 5e-5 in mean warming, while the 30 m
+// and 60 m ones, whose unsteady flow amplifies any perturbation (a 1e-6 relative change of the wind moves f32's result
+// by up to 8e-4), differ by up to 1.5e-3 and 4.7e-3 in 16-bit storage; the coarse grids are a small share of the work.
+// The baseline (`mode: 'sequential'`) runs the same requests one at a time with the plain per-solver path (f32 under
+// `auto`).
 // Usage: node serve.js --rate 5 --requests 24 [--mode batch|sequential] [--spinup 2200] [--mean 600] [--seed 1]
+//        [--storage 0|1|2|4|auto] [--f16s-max-dx 10]
 //        [--chunk 50] [--budget-gb 200] [--slots 64] [--policy short-first|fifo] [--short-work 3e10]
 //        [--long-share 8] [--grid blocks (0 = online tuning; default one block per tile)]
 //        [--block 64|128|256 (threads per block of the batched step; default tuned online)] [--out file.json]
@@ -28,6 +36,7 @@ const { loadAddon } = require('./solver');
 const P = require('./physics');
 
 const STEP_ACC = 1;
+const STORAGE_NAMES = ['f32', 'f16', 'bf16', 'f64', 'f16s'];
 // batch grid: one workgroup per tile (the aht batch clamps the grid to the tile count). Measured at 600 x 600 x 64
 // plus four 200 x 200 x 48 domains, it beat the persistent grids (8 and 32 blocks per CU) for every mix; --grid 0
 // lets the batch's online tuner choose.
@@ -85,9 +94,15 @@ function result(req, job, A, h, now) {
   };
 }
 
-function createSolver(A, job, o) {
+/** the storage format of a request: o.storage, or under 'auto' f16s up to o.f16sMaxDx metres of grid spacing, else f32 */
+function storageOf(req, o) {
+  if (o.storage !== 'auto') return o.storage || 0;
+  return (o.boxM || 6000) / req.grid[0] <= (o.f16sMaxDx || 10) ? 4 : 0;
+}
+
+function createSolver(A, job, o, storage) {
   const sc = job.sc, c = sc.col;
-  const h = A.create({ nx: sc.nx, ny: sc.ny, nz: sc.nz, storage: o.storage || 0, device: o.device || 0 });
+  const h = A.create({ nx: sc.nx, ny: sc.ny, nz: sc.nz, storage, device: o.device || 0 });
   A.setColumns(h, c.kt, c.ktop, c.cd, c.q);
   A.setParams(h, sc.params, sc.profBlock());
   A.initFlow(h);
@@ -98,11 +113,12 @@ function createSolver(A, job, o) {
 
 /** launches of the next chunk: members [{cells, total, spin, done}] -> n launches, per-member step counts ns and the
  *  first averaging launch af (relative to the chunk). Under 'short-first', while any member has at most shortWork
- *  cell-steps left, members with more step only in the first ceil(chunk / longShare) launches. */
-function planChunk(members, chunk, policy, shortWork, longShare) {
+ *  cell-steps left (among these members, or anyShort: in another batch), members with more step only in the first
+ *  ceil(chunk / longShare) launches. */
+function planChunk(members, chunk, policy, shortWork, longShare, anyShort = false) {
   const m = members.length, ns = new Int32Array(m), af = new Int32Array(m);
   const work = members.map(a => a.cells * (a.total - a.done));
-  const prio = policy === 'short-first' && work.some(w => w <= shortWork);
+  const prio = policy === 'short-first' && (anyShort || work.some(w => w <= shortWork));
   let n = 0;
   for (let k = 0; k < m; k++) {
     const a = members[k];
@@ -139,12 +155,19 @@ function summary(mode, o, results, t0, t1, peakBytes, busyMs) {
   };
 }
 
-/** continuous batching over one aht batch */
+/** continuous batching over one aht batch per storage format in use */
 async function serveBatched(reqs, o) {
   const A = loadAddon();
-  const B = A.batchCreate(o.device || 0, o.storage || 0);
-  A.batchSetGrid(B, o.grid === undefined ? ONE_BLOCK_PER_TILE : o.grid, o.useGraph === false ? 0 : 1);
-  A.batchSetBlock(B, o.block || 0);
+  const batches = new Map();   // storage -> batch
+  const batchOf = st => {
+    if (!batches.has(st)) {
+      const B = A.batchCreate(o.device || 0, st);
+      A.batchSetGrid(B, o.grid === undefined ? ONE_BLOCK_PER_TILE : o.grid, o.useGraph === false ? 0 : 1);
+      A.batchSetBlock(B, o.block || 0);
+      batches.set(st, B);
+    }
+    return batches.get(st);
+  };
   const budget = (o.budgetGB || 200) * 1e9, slots = o.slots || 64, chunk = o.chunk || 50;
   const shortWork = o.shortWork || 3e10, longShare = o.longShare || 8;
   const future = reqs.slice().sort((a, b) => a.arrival - b.arrival);
@@ -158,15 +181,15 @@ async function serveBatched(reqs, o) {
     while (future.length && future[0].arrival <= now()) pending.push(future.shift());
     // admission: FIFO, by footprint and slots
     while (pending.length && active.length < slots) {
-      const r = pending[0];
-      const fp = A.footprint({ nx: r.grid[0], ny: r.grid[1], nz: r.grid[2], storage: o.storage || 0 });
+      const r = pending[0], st = storageOf(r, o);
+      const fp = A.footprint({ nx: r.grid[0], ny: r.grid[1], nz: r.grid[2], storage: st });
       if (used + fp > budget && active.length > 0) break;
       pending.shift();
       const job = buildScenario(r, o);
       job.keepSurf = !!o.keepSurf;
-      const h = createSolver(A, job, o);
-      A.batchAdd(B, h);
-      active.push({ r, job, h, done: 0, fp, cells: r.grid[0] * r.grid[1] * r.grid[2] });
+      const h = createSolver(A, job, o, st);
+      A.batchAdd(batchOf(st), h);
+      active.push({ r, job, h, st, done: 0, fp, cells: r.grid[0] * r.grid[1] * r.grid[2] });
       used += fp;
       peak = Math.max(peak, used);
     }
@@ -174,15 +197,21 @@ async function serveBatched(reqs, o) {
       if (future.length) await sleep(Math.max(0, (future[0].arrival - now()) * 1e3));
       continue;
     }
-    const { n, ns, af } = planChunk(active.map(a => ({ cells: a.cells, total: a.job.total, spin: a.job.spin, done: a.done })),
-      chunk, o.policy || 'short-first', shortWork, longShare);
-    busy += await A.batchStep(B, n, ns, af);
-    for (let k = 0; k < active.length; k++) active[k].done += ns[k];
+    // one chunk per batch in use, members in add order; the short-first priority looks at the members of every batch
+    const anyShort = active.some(a => a.cells * (a.job.total - a.done) <= shortWork);
+    for (const [st, B] of batches) {
+      const mem = active.filter(a => a.st === st);
+      if (!mem.length) continue;
+      const { n, ns, af } = planChunk(mem.map(a => ({ cells: a.cells, total: a.job.total, spin: a.job.spin, done: a.done })),
+        chunk, o.policy || 'short-first', shortWork, longShare, anyShort);
+      busy += await A.batchStep(B, n, ns, af);
+      for (let k = 0; k < mem.length; k++) mem[k].done += ns[k];
+    }
     for (let k = active.length - 1; k >= 0; k--) {
       const a = active[k];
       if (a.done >= a.job.total) {
         results.push(result(a.r, a.job, A, a.h, now()));
-        A.batchRemove(B, a.h);
+        A.batchRemove(batchOf(a.st), a.h);
         A.destroy(a.h);
         used -= a.fp;
         active.splice(k, 1);
@@ -190,12 +219,17 @@ async function serveBatched(reqs, o) {
     }
   }
   const t1 = now();
-  const grid = A.batchGrid(B), block = A.batchBlock(B);
-  A.batchDestroy(B);
+  const blocks = {}, grids = {};
+  for (const [st, B] of batches) {
+    blocks[STORAGE_NAMES[st]] = A.batchBlock(B);
+    grids[STORAGE_NAMES[st]] = A.batchGrid(B);
+    A.batchDestroy(B);
+  }
   results.sort((a, b) => a.id - b.id);
   return {
     summary: Object.assign(summary('batch', o, results, 0, t1, peak, busy),
-      { batch_grid: grid, batch_block: block, policy: o.policy || 'short-first' }),
+      { storage: o.storage === 'auto' ? `auto (f16s up to ${o.f16sMaxDx || 10} m, else f32)` : STORAGE_NAMES[o.storage || 0],
+        batch_grid: grids, batch_block: blocks, policy: o.policy || 'short-first' }),
     results,
   };
 }
@@ -215,8 +249,9 @@ async function serveSequential(reqs, o) {
     if (r.arrival > now()) await sleep((r.arrival - now()) * 1e3);
     const job = buildScenario(r, o);
     job.keepSurf = !!o.keepSurf;
-    const h = createSolver(A, job, o);
-    peak = Math.max(peak, A.footprint({ nx: r.grid[0], ny: r.grid[1], nz: r.grid[2], storage: o.storage || 0 }));
+    const st = o.storage === 'auto' ? 0 : (o.storage || 0);   // the plain step: f32 under 'auto'
+    const h = createSolver(A, job, o, st);
+    peak = Math.max(peak, A.footprint({ nx: r.grid[0], ny: r.grid[1], nz: r.grid[2], storage: st }));
     for (let done = 0; done < job.total;) {
       const n = Math.min(chunk, job.total - done), spinLeft = job.spin - done;
       if (spinLeft >= n) busy += await A.benchmark(h, n);
@@ -253,7 +288,7 @@ async function main() {
   const o = {
     rate: +(a.rate || 5), requests: +(a.requests || 24), seed: +(a.seed || 1), chunk: +(a.chunk || 50),
     spinupS: +(a.spinup || 2200), meanS: +(a.mean || 600), budgetGB: +(a['budget-gb'] || 200), slots: +(a.slots || 64),
-    storage: +(a.storage || 0), grid: a.grid === undefined ? ONE_BLOCK_PER_TILE : +a.grid, block: +(a.block || 0),
+    storage: a.storage === 'auto' ? 'auto' : +(a.storage || 0), f16sMaxDx: +(a['f16s-max-dx'] || 10), grid: a.grid === undefined ? ONE_BLOCK_PER_TILE : +a.grid, block: +(a.block || 0),
     useGraph: a['no-graph'] ? false : true,
     policy: a.policy || 'short-first', shortWork: +(a['short-work'] || 3e10), longShare: +(a['long-share'] || 8),
   };
@@ -269,6 +304,6 @@ async function main() {
   if (a.out) fs.writeFileSync(a.out, JSON.stringify(out));
 }
 
-module.exports = { makeRequests, buildScenario, planChunk, serveBatched, serveSequential };
+module.exports = { makeRequests, buildScenario, planChunk, storageOf, serveBatched, serveSequential };
 
 if (require.main === module) main().catch(e => { console.error(e && e.stack || e); process.exit(1); });

@@ -16,8 +16,8 @@ sample of the same workload: whole steps on the same 600 x 600 x 64 domain for a
 Also measured in the same run: the 16-bit storage formats on the same domain (`reduced_precision`: f16 without theta
 plane and f16 with its f32 theta plane) and the serving
 half of the metric (`serving`: --serve-requests requests at --serve req/s through the Node serving loop, continuous
-batching against one-at-a-time execution with the plain f32 step, and the same requests batched in f16s storage;
-about four minutes).
+batching against one-at-a-time execution with the plain f32 step, and the same requests batched again with f16s
+storage for the 10 m grids; about four minutes).
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -82,8 +82,9 @@ def serving(rate, requests, f16s=True):
     """the serving half of the metric (scenarios/hour, p50/p99 latency): measured live by the Node serving loop
     (js/serve.js: Poisson arrivals at `rate`, the serving grid mix, 2,200 s spin-up + 10-minute mean per request,
     continuous batching against the same requests one at a time with the plain f32 step); with f16s, the same requests
-    batched once more in f16s storage (stochastic rounding) against the same f32 sequential run, with the largest
-    relative difference of any request's mean warming from its f32 result; rate 0 skips it"""
+    batched once more with storage per request (serve.js --storage auto: f16s with stochastic rounding for the 10 m
+    grids, which carry most of the work, f32 for the coarser ones) against the same f32 sequential run, with the
+    largest relative difference of any request's mean warming from its f32 result; rate 0 skips it"""
     if rate <= 0:
         return None
     d = tempfile.mkdtemp(prefix='aht_serve_')
@@ -97,7 +98,7 @@ def serving(rate, requests, f16s=True):
            'vs_sequential': round(b['scenarios_per_hour'] / q['scenarios_per_hour'], 4),
            'batch': b, 'sequential': q}
     if f16s:
-        r16, rc16 = run_serve(rate, requests, 'batch', 4, os.path.join(d, 'serve16.json'))
+        r16, rc16 = run_serve(rate, requests, 'batch', 'auto', os.path.join(d, 'serve16.json'))
         b16 = r16['batch']['summary']
         ref = {x['id']: x['warmingMean'] for x in r['sequential']['results']}
         diff = max(abs(x['warmingMean'] - ref[x['id']]) / abs(ref[x['id']]) for x in r16['batch']['results'])
@@ -225,7 +226,7 @@ def main():
         serve = serving(a.serve, a.serve_requests)
         log('serving', {k: serve[k] for k in ('scenarios_per_hour', 'latency_p50_s', 'latency_p99_s', 'vs_sequential')})
         if 'batch_f16s' in serve:
-            log('serving, batched in f16s', {k: serve['batch_f16s'][k] for k in (
+            log('serving, batched with f16s for the 10 m grids', {k: serve['batch_f16s'][k] for k in (
                 'scenarios_per_hour', 'latency_p50_s', 'latency_p99_s', 'vs_sequential_f32',
                 'warming_mean_max_rel_diff_vs_f32')})
     out = {
