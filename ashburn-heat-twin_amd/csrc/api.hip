@@ -1051,7 +1051,7 @@ int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best) {
   // every shape over `steps` timed steps (after one untimed step that also uploads the changed grid), then the four
   // fastest again twice over 3 x `steps`, best of the two: neighbouring shapes differ by ~1 %, about the noise of a
   // single short timing
-  const int cb[3] = {64, 128, 256}, cl[3] = {1, 2, 4}, cw[6] = {1, 2, 4, 16, 0, -1};
+  const int cb[3] = {64, 128, 256}, cl[3] = {1, 2, 4}, cw[7] = {1, 2, 4, 8, 16, 0, -1};
   struct Cand {
     aht_launch l;
     double ms;
