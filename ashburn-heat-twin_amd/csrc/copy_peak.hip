@@ -18,6 +18,7 @@
 // step's own layout, both in one array [cell / 2048][19 + 7][cell % 2048]; `lattice16_pair_pull_nt` the same for
 // 2-byte elements in blocks of 256 cells, with the f32 theta array of the 16-bit formats pulled beside it.
 // `lattice16s_pair_pull_nt` is the f16s step's pattern (the 16-bit pair without theta array);
+// `lattice16_pair_theta_in_block_pull_nt` keeps the f32 theta values inside each 256-cell block (28 two-byte units);
 // `lattice_colmajor_pull_nt` the f32 pair with column blocks outermost and levels inside them; `lattice_pair_aa_*` and
 // `lattice_pair_esotwist*` the in-place patterns in the pair layout. `lattice_pair_{noshift,zonly,xonly,yonly}_nt`
 // and `lattice16s_pair_{noshift,zonly,xonly,yonly}_nt` take the pair patterns apart: every stream read at the cell
@@ -29,7 +30,8 @@
 //    | lattice_pull_nt_x4 | lattice_blocked_pull_nt | lattice_pair_pull_nt | lattice16_pair_pull_nt
 //    | lattice16s_pair_pull_nt | lattice_colmajor_pull_nt | lattice_pair_aa_even | lattice_pair_aa_odd
 //    | lattice_pair_esotwist | lattice_pair_aa_even_nt | lattice_pair_aa_odd_nt | lattice_pair_esotwist_nt
-//    | lattice[16s]_pair_{noshift,zonly,xonly,yonly}_nt | lattice[16s]_pair_pull_nt_w{2,4,16,64})
+//    | lattice[16s]_pair_{noshift,zonly,xonly,yonly}_nt | lattice[16s]_pair_pull_nt_w{2,4,16,64}
+//    | lattice16_pair_theta_in_block_pull_nt)
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -280,6 +282,30 @@ __global__ __launch_bounds__(256) void copy_lattice16_pair(const uint16_t *__res
   if (THETA) __builtin_nontemporal_store(th, &tb[n]);
 }
 
+// the 16-bit pair with the f32 theta values inside each block: [cell >> 8][28][256] 2-byte units, units 0-25 the 26
+// populations, units 26-27 the 256 theta values of the block's cells (as f32), theta pulled at the 7 D3Q7 offsets
+__global__ __launch_bounds__(256) void copy_lattice16_pair_theta_in_block(const uint16_t *__restrict__ a,
+                                                                         uint16_t *__restrict__ b, uint32_t N, int nx,
+                                                                         int nxy) {
+  const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  const float *ta = (const float *)a;
+  float *tb = (float *)b;
+  uint16_t v[26];
+  float th = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 26; q++) {
+    int64_t src = (int64_t)n - (LCX[q] + LCY[q] * nx + LCZ[q] * nxy);
+    src = src < 0 ? 0 : (src >= (int64_t)N ? (int64_t)N - 1 : src);
+    v[q] = a[((((size_t)src >> 8) * 28 + q) << 8) | ((size_t)src & 255)];
+    if (q >= 19) th += ta[((size_t)src >> 8) * 3584 + 3328 + ((size_t)src & 255)];
+  }
+#pragma unroll
+  for (int q = 0; q < 26; q++)
+    __builtin_nontemporal_store(v[q], &b[((((size_t)n >> 8) * 28 + q) << 8) | ((size_t)n & 255)]);
+  __builtin_nontemporal_store(th, &tb[((size_t)n >> 8) * 3584 + 3328 + ((size_t)n & 255)]);
+}
+
 // the f32 pair with column blocks outermost: element (q, level k, column c) at
 // ((((c >> LB) * nz + k) * 26 + q) << LB) | (c & (2^LB - 1)), so the levels of one column block are adjacent in memory;
 // thread blocks walk a column block through its levels (2^LB / 256 thread blocks per level) before the next one
@@ -362,7 +388,7 @@ int main(int argc, char **argv) {
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
-  const int NV = 41;
+  const int NV = 42;
   const char *names[NV] = {"grid", "flat", "unroll4", "unroll4_xcd", "unroll4_nt", "lattice", "lattice_pull",
                            "lattice_aa_even", "lattice_aa_odd", "lattice_pull_nt", "lattice16_pull_nt",
                            "lattice_aosoa_pull_nt", "lattice_pull_nt_x2", "lattice_pull_nt_x4",
@@ -375,7 +401,8 @@ int main(int argc, char **argv) {
                            "lattice_pair_pull_nt_w16", "lattice_pair_pull_nt_w64",
                            "lattice16s_pair_noshift_nt", "lattice16s_pair_zonly_nt", "lattice16s_pair_xonly_nt",
                            "lattice16s_pair_yonly_nt", "lattice16s_pair_pull_nt_w2", "lattice16s_pair_pull_nt_w4",
-                           "lattice16s_pair_pull_nt_w16", "lattice16s_pair_pull_nt_w64"};
+                           "lattice16s_pair_pull_nt_w16", "lattice16s_pair_pull_nt_w64",
+                           "lattice16_pair_theta_in_block_pull_nt"};
   const uint32_t wins[4] = {2, 4, 16, 64};
   const int lnx = 600, lnxy = 600 * 600;
   const uint32_t lN = 600u * 600u * 64u;   // lattice variants: 26 * lN floats each way (2.4 GB)
@@ -424,6 +451,7 @@ int main(int argc, char **argv) {
         case 36: hipLaunchKernelGGL((copy_pair_parts<uint16_t, 8, 4>), dim3(lN / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy, 1u); break;
         case 37: case 38: case 39: case 40:
           hipLaunchKernelGGL((copy_pair_parts<uint16_t, 8, 0>), dim3(lN / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy, wins[v - 37]); break;
+        case 41: hipLaunchKernelGGL(copy_lattice16_pair_theta_in_block, dim3(lN / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy); break;
       }
     };
     launch();
@@ -434,7 +462,7 @@ int main(int argc, char **argv) {
     CK(hipEventSynchronize(e1));
     float ms = 0;
     CK(hipEventElapsedTime(&ms, e0, e1));
-    const size_t moved = (v == 10 || v == 16) ? (size_t)28 * lN * 2
+    const size_t moved = (v == 10 || v == 16 || v == 41) ? (size_t)28 * lN * 2
                          : ((v == 17 || v >= 33) ? (size_t)26 * lN * 2 : (v >= 5 ? (size_t)26 * lN * 4 : n * 16));
     const double per = ms / reps, gbs = 2.0 * moved / (per * 1e-3) / 1e9;
     printf("{\"kernel\": \"copy_%s\", \"bytes_copied\": %zu, \"reps\": %d, \"ms_per_launch\": %.5f, "
