@@ -13,7 +13,8 @@
  * a finished runtime. The device thread is joined in the environment's cleanup hook instead, before those
  * destructors; after process.exit() it is never joined and keeps its state until the process ends.
  *
- * The solver handle is a JS external; the host serialises calls per solver (the C-ABI is one caller per solver).
+ * The solver handle is a JS external. The C-ABI takes one caller per solver at a time: the synchronous calls on a
+ * solver (or batch) throw while an asynchronous job that uses it is queued or running; steps queue behind each other.
  *
  * Lifetime: a step, benchmark or batch step runs on the device thread while the main thread goes on. destroy() of a
  * solver (or batch) with such work in flight does not free it under the job: the handle is marked dead at once
@@ -135,6 +136,17 @@ static handle_t *get_handle(napi_env env, napi_value v) {
   return (handle_t *)p;
 }
 
+/* the handle of a solver that no asynchronous job uses (its own, or its batch's): the synchronous calls run the C-ABI
+ * on the main thread, and the C-ABI takes one caller per solver at a time */
+static handle_t *get_idle_handle(napi_env env, napi_value v) {
+  handle_t *h = get_handle(env, v);
+  if (h && solver_busy(h)) {
+    napi_throw_error(env, NULL, "aht addon: the solver has a step in flight (await it first)");
+    return NULL;
+  }
+  return h;
+}
+
 /* pointer and byte length of a typed array; NULL for null/undefined */
 static int get_buf(napi_env env, napi_value v, void **ptr, size_t *bytes) {
   napi_valuetype t;
@@ -237,7 +249,7 @@ static napi_value js_destroy(napi_env env, napi_callback_info info) {
 static napi_value js_set_columns(napi_env env, napi_callback_info info) {
   napi_value argv[5];
   if (!get_args(env, info, 5, argv)) return NULL;
-  handle_t *h = get_handle(env, argv[0]);
+  handle_t *h = get_idle_handle(env, argv[0]);
   if (!h) return NULL;
   void *p[4];
   size_t b[4];
@@ -250,7 +262,7 @@ static napi_value js_set_columns(napi_env env, napi_callback_info info) {
 static napi_value js_set_params(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return NULL;
-  handle_t *h = get_handle(env, argv[0]);
+  handle_t *h = get_idle_handle(env, argv[0]);
   if (!h) return NULL;
   aht_params P;
   memset(&P, 0, sizeof P);
@@ -271,7 +283,7 @@ static napi_value js_set_params(napi_env env, napi_callback_info info) {
   static napi_value name(napi_env env, napi_callback_info info) {        \
     napi_value argv[1];                                                  \
     if (!get_args(env, info, 1, argv)) return NULL;                      \
-    handle_t *h = get_handle(env, argv[0]);                              \
+    handle_t *h = get_idle_handle(env, argv[0]);                              \
     if (!h) return NULL;                                                 \
     int rc = call(h->s);                                                 \
     return rc ? throw_rc(env, h->s, rc) : undef(env);                    \
@@ -284,7 +296,7 @@ SIMPLE(js_sync, aht_sync)
 static napi_value js_reinit_cells(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return NULL;
-  handle_t *h = get_handle(env, argv[0]);
+  handle_t *h = get_idle_handle(env, argv[0]);
   if (!h) return NULL;
   void *p;
   size_t b;
@@ -493,7 +505,7 @@ static napi_value js_benchmark(napi_env env, napi_callback_info info) {
 static napi_value js_read_surface(napi_env env, napi_callback_info info) {
   napi_value argv[5];
   if (!get_args(env, info, 5, argv)) return NULL;
-  handle_t *h = get_handle(env, argv[0]);
+  handle_t *h = get_idle_handle(env, argv[0]);
   if (!h) return NULL;
   double conv;
   void *p[3];
@@ -508,7 +520,7 @@ static napi_value js_read_surface(napi_env env, napi_callback_info info) {
 static napi_value js_read_volume(napi_env env, napi_callback_info info) {
   napi_value argv[4];
   if (!get_args(env, info, 4, argv)) return NULL;
-  handle_t *h = get_handle(env, argv[0]);
+  handle_t *h = get_idle_handle(env, argv[0]);
   if (!h) return NULL;
   int32_t nzv;
   double vm;
@@ -527,7 +539,7 @@ static napi_value js_read_volume(napi_env env, napi_callback_info info) {
 static napi_value js_stats(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
-  handle_t *h = get_handle(env, argv[0]);
+  handle_t *h = get_idle_handle(env, argv[0]);
   if (!h) return NULL;
   aht_diag d;
   int rc = aht_stats(h->s, &d);
@@ -548,7 +560,7 @@ static napi_value js_stats(napi_env env, napi_callback_info info) {
 static napi_value js_read_mean(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return NULL;
-  handle_t *h = get_handle(env, argv[0]);
+  handle_t *h = get_idle_handle(env, argv[0]);
   if (!h) return NULL;
   void *p;
   size_t b;
@@ -574,7 +586,7 @@ static napi_value js_read_mean(napi_env env, napi_callback_info info) {
 static napi_value js_state(napi_env env, napi_callback_info info, int write) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return NULL;
-  handle_t *h = get_handle(env, argv[0]);
+  handle_t *h = get_idle_handle(env, argv[0]);
   if (!h) return NULL;
   void *f, *g;
   size_t bf, bg;
@@ -588,7 +600,7 @@ static napi_value js_write_state(napi_env env, napi_callback_info info) { return
 static napi_value js_read_out(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return NULL;
-  handle_t *h = get_handle(env, argv[0]);
+  handle_t *h = get_idle_handle(env, argv[0]);
   if (!h) return NULL;
   void *p;
   size_t b;
@@ -627,7 +639,7 @@ static napi_value launch_obj(napi_env env, const aht_launch *l) {
 static napi_value js_autotune(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return NULL;
-  handle_t *h = get_handle(env, argv[0]);
+  handle_t *h = get_idle_handle(env, argv[0]);
   int32_t steps;
   if (!h || !get_i32(env, argv[1], &steps)) return NULL;
   aht_launch best;
@@ -640,7 +652,7 @@ static napi_value js_autotune(napi_env env, napi_callback_info info) {
 static napi_value js_set_launch(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return NULL;
-  handle_t *h = get_handle(env, argv[0]);
+  handle_t *h = get_idle_handle(env, argv[0]);
   if (!h) return NULL;
   aht_launch l;
   memset(&l, 0, sizeof l);
@@ -777,6 +789,10 @@ static napi_value js_batch_set_grid(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 3, argv)) return NULL;
   bhandle_t *b = get_bhandle(env, argv[0]);
   if (!b) return NULL;
+  if (b->inflight) {
+    napi_throw_error(env, NULL, "aht batch: the batch has a step in flight (await it first)");
+    return NULL;
+  }
   int32_t g, ug;
   if (!get_i32(env, argv[1], &g) || !get_i32(env, argv[2], &ug)) return NULL;
   int rc = aht_batch_set_grid(b->b, g, ug);
@@ -798,6 +814,10 @@ static napi_value js_batch_set_block(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 2, argv)) return NULL;
   bhandle_t *b = get_bhandle(env, argv[0]);
   if (!b) return NULL;
+  if (b->inflight) {
+    napi_throw_error(env, NULL, "aht batch: the batch has a step in flight (await it first)");
+    return NULL;
+  }
   int32_t bl;
   if (!get_i32(env, argv[1], &bl)) return NULL;
   int rc = aht_batch_set_block(b->b, bl);

@@ -17,6 +17,24 @@ async function main() {
   A.destroy(h2);
   out.stepBeforeDestroy = await inflight.then(() => 'resolved', e => 'rejected: ' + e.message);
   try { A.step(h2, 1, 0); out.stepAfterDestroy = 'no error'; } catch (e) { out.stepAfterDestroy = 'threw'; }
+  // synchronous calls on a solver (or batch) with a job in flight throw; after it settles they run
+  const h5 = mk();
+  const s5 = A.step(h5, 20, 0);
+  const tryCall = f => { try { f(); return 'ran'; } catch (e) { return 'threw'; } };
+  out.statsDuringStep = tryCall(() => A.stats(h5));
+  out.setParamsDuringStep = tryCall(() => A.resetHeat(h5));
+  await s5;
+  out.statsAfterStep = tryCall(() => A.stats(h5));
+  const B5 = A.batchCreate(0, 0);
+  A.batchAdd(B5, h5);
+  const b5 = A.batchStep(B5, 5, null, null);
+  out.memberStatsDuringBatchStep = tryCall(() => A.stats(h5));
+  out.setBlockDuringBatchStep = tryCall(() => A.batchSetBlock(B5, 128));
+  await b5;
+  out.setBlockAfterBatchStep = tryCall(() => A.batchSetBlock(B5, 128));
+  A.batchRemove(B5, h5);
+  A.batchDestroy(B5);
+  A.destroy(h5);
   // a batch and one of its members destroyed under a batch step
   const B = A.batchCreate(0, 0), h3 = mk();
   A.batchAdd(B, h1);

@@ -80,7 +80,8 @@ console.log(JSON.stringify({ r: { n: r.n, ns: Array.from(r.ns), af: Array.from(r
 def test_addon_device_thread_against_cpu_standin(tmp_path):
     """the N-API addon built against a CPU stand-in of the C-ABI (tests/js/aht_standin.cpp), which aborts on any
     threading or lifetime violation: jobs issued without awaiting run one at a time in issue order, destroy() with
-    a step or batch step in flight is deferred, and no thread that made device calls outlives the runtime's static
+    a step or batch step in flight is deferred, synchronous calls on a solver or batch with a job in flight throw
+    instead of running beside it, and no thread that made device calls outlives the runtime's static
     teardown at exit (libuv joins its pool threads after it; the addon's device thread is joined before it)"""
     lib, addon = tmp_path / 'libaht.so', tmp_path / 'aht_node.node'
     subprocess.run(['g++', '-O1', '-std=c++17', '-shared', '-fPIC', '-I', os.path.join(ROOT, 'include'), '-o', str(lib),
@@ -97,4 +98,8 @@ def test_addon_device_thread_against_cpu_standin(tmp_path):
         assert out['stepBeforeDestroy'] == 'resolved' and out['stepAfterDestroy'] == 'threw'
         assert out['batchStep'] == 10 and out['bench'] == 3
         assert out['order'] == list(range(6))
+        assert out['statsDuringStep'] == 'threw' and out['setParamsDuringStep'] == 'threw'
+        assert out['statsAfterStep'] == 'ran'
+        assert out['memberStatsDuringBatchStep'] == 'threw' and out['setBlockDuringBatchStep'] == 'threw'
+        assert out['setBlockAfterBatchStep'] == 'ran'
         assert 'last step settled' in r.stderr
