@@ -15,7 +15,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, 'lib', 'liblbm_oracle.so')
+# AHT_ORACLE_LIB selects another build of the oracle (the sanitizer build of the host tests)
+LIB_PATH = os.environ.get('AHT_ORACLE_LIB') or os.path.join(HERE, 'lib', 'liblbm_oracle.so')
 RHO_CP = 1.16 * 1005.0
 
 # lattice tables (js/live/kernel.js:19-28)
