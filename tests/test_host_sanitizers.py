@@ -14,6 +14,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SAN = ['-O1', '-g', '-fno-omit-frame-pointer', '-fsanitize=address,undefined', '-fno-sanitize-recover=undefined']
 
 
+def preload(rt):
+    """LD_PRELOAD with the sanitizer runtime first and whatever the environment already preloads after it"""
+    cur = os.environ.get('LD_PRELOAD', '')
+    return rt + (':' + cur if cur else '')
+
+
 def runtime(name):
     p = subprocess.run(['gcc', f'-print-file-name={name}'], capture_output=True, text=True).stdout.strip()
     if not os.path.isabs(p) or not os.path.exists(p):
@@ -28,7 +34,8 @@ def test_oracle_under_asan_ubsan(tmp_path):
     subprocess.run(['gcc'] + SAN + ['-fPIC', '-fopenmp', '-ffp-contract=off', '-fno-fast-math', '-std=c11', '-Wall',
                                     '-Wno-unused-function', '-shared', '-o', str(lib),
                                     os.path.join(ROOT, 'oracle', 'lbm_oracle.c'), '-lm'], check=True)
-    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS='detect_leaks=0', AHT_ORACLE_LIB=str(lib), OMP_NUM_THREADS='2')
+    env = dict(os.environ, LD_PRELOAD=preload(asan), ASAN_OPTIONS='detect_leaks=0', AHT_ORACLE_LIB=str(lib),
+               OMP_NUM_THREADS='2')
     probe = ('import sys; sys.path.insert(0, %r); from oracle import oracle as O; O.lib(); '
              'm = open("/proc/self/maps").read(); print(%r in m, "libasan" in m)' % (ROOT, str(lib)))
     r = subprocess.run([sys.executable, '-c', probe], env=env, capture_output=True, text=True, timeout=120)
@@ -63,7 +70,7 @@ def test_addon_under_sanitizers(tmp_path, san):
         rt, flags, opts = runtime('libtsan.so'), ['-O1', '-g', '-fno-omit-frame-pointer', '-fsanitize=thread'], \
             {'TSAN_OPTIONS': 'halt_on_error=1'}
     addon = build_addon(tmp_path, flags)   # one directory per test: the addon loads libaht.so beside it
-    env = dict(os.environ, LD_PRELOAD=rt, **opts)
+    env = dict(os.environ, LD_PRELOAD=preload(rt), **opts)
     for mode in ('natural', 'exit'):
         r = subprocess.run(['node', os.path.join(ROOT, 'tests', 'js', 'exit_test.js'), addon, mode], env=env,
                            capture_output=True, text=True, timeout=120)
