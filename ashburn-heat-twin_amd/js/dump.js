@@ -84,7 +84,7 @@ function parseArgs(argv) {
 }
 
 /** options from CLI flags:
- *  --workload c1|c2|c3                     synthetic benchmark workload
+ *  --workload c1|c2|c3|c2flat              synthetic benchmark workload (c2flat: c2 without terrain or buildings)
  *  --batch-member d[,seed]                  member d of the batched uneven-domain workload (handled in main)
  *  --serve-request k[,seed]                 request k of the serving loop's request list (handled in main)
  *  --tiny nx,ny,nz,seed[,buildings,sources] small synthetic grid (dx from --dx, default 30)

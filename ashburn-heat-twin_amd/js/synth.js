@@ -30,6 +30,9 @@ const WORKLOADS = {
   // plain drop-in on one GPU: 600 x 600 x 64 at 10 m (23 M cells)
   c2: { nx: 600, ny: 600, nz: 64, dx: 10, hills: 12, buildings: 4000, sources: 210, flux: 400,
     met: { u10: 4, dirFrom: 315 }, alpha: 0.2, steps: 20000, meanSeconds: 600, seed: 2 },
+  // timing reference: the same grid and met without terrain, buildings or source roofs (every cell on the fast path)
+  c2flat: { nx: 600, ny: 600, nz: 64, dx: 10, hills: 0, buildings: 0, sources: 0, flux: 400,
+    met: { u10: 4, dirFrom: 315 }, alpha: 0.2, steps: 20000, meanSeconds: 600, seed: 2 },
   // reduced-precision storage: 1200 x 1200 x 96 at 5 m (138 M cells)
   c3: { nx: 1200, ny: 1200, nz: 96, dx: 5, hills: 12, buildings: 16000, sources: 850, flux: 400,
     met: { u10: 4, dirFrom: 315 }, alpha: 0.2, steps: 5000, meanSeconds: 600, seed: 3 },
