@@ -236,7 +236,8 @@ def main():
         'baseline_note': 'vs the reference numba.cuda prototype, 950 MLUPS at 600x600x64 on an RTX 4090',
         'dtype': ['f32', 'f16 storage / f32 arithmetic', 'bf16 storage / f32 arithmetic', 'f64',
                   'f16 storage (no theta plane) / f32 arithmetic'][a.storage],
-        'data': 'synthetic (seeded generator: hills, box buildings, source roofs)',
+        'data': 'synthetic (seeded generator: ' + ('flat ground, no buildings' if a.workload == 'c2flat' and not a.dump
+                                                   else 'hills, box buildings, source roofs') + ')',
         'config': {'workload': f'{a.workload}: {nx}x{ny}x{nz} at {dom["dx"]} m, fused D3Q19+D3Q7 step',
                    'cells': N, 'fluid_cells': nf, 'bytes_per_fluid_cell': bpc, 'dt_s': dom['dt'],
                    'storage': STORAGE[a.storage], 'launch': launch, 'parallelism': 'single GPU'},
