@@ -1048,9 +1048,10 @@ int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best) {
   }
   HIP_TRY(s, hipMemcpyAsync(bf, s->f[p0], fb, hipMemcpyDeviceToDevice, s->st));
   if (gb) HIP_TRY(s, hipMemcpyAsync(bg, s->gq[p0], gb, hipMemcpyDeviceToDevice, s->st));
-  // every shape over `steps` timed steps (after one untimed step that also uploads the changed grid), then the four
-  // fastest again twice over 3 x `steps`, best of the two: neighbouring shapes differ by ~1 %, about the noise of a
-  // single short timing
+  // every shape over `steps` timed steps (after one untimed step that also uploads the changed grid), then the six
+  // fastest again in three interleaved rounds over 10 x `steps`, best of the three: neighbouring shapes differ by
+  // ~1 %, about the noise of a short timing (with the four fastest twice over 3 x `steps` the 600 x 600 x 64 f32 step
+  // once kept 256,2,1 at 0.829 ms over 256,1,1 at 0.819 ms)
   const int cb[3] = {64, 128, 256}, cl[3] = {1, 2, 4}, cw[7] = {1, 2, 4, 8, 16, 0, -1};
   struct Cand {
     aht_launch l;
@@ -1074,13 +1075,13 @@ int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best) {
       }
   if (!rc) {
     std::sort(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
-    cand.resize(std::min<size_t>(cand.size(), 4));
+    cand.resize(std::min<size_t>(cand.size(), 6));
     for (Cand &c : cand) c.ms = 1e30;
-    for (int round = 0; round < 2 && !rc; round++)
+    for (int round = 0; round < 3 && !rc; round++)
       for (Cand &c : cand) {
         double ms = 0;
-        if ((rc = time_shape(c.l, 3 * steps, ms))) break;
-        c.ms = std::min(c.ms, ms / (3 * steps));
+        if ((rc = time_shape(c.l, 10 * steps, ms))) break;
+        c.ms = std::min(c.ms, ms / (10 * steps));
       }
     if (!rc) keep = std::min_element(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; })->l;
   }

@@ -202,8 +202,8 @@ typedef struct {
 } aht_launch;
 
 /* launch-shape autotuner of one solver: times every block x levels x window candidate over `steps` steps each,
- * re-times the four fastest twice over 3 x `steps`, keeps the fastest (written to *best, may be NULL), and restores
- * the state and counters */
+ * re-times the six fastest in three rounds over 10 x `steps`, keeps the fastest (written to *best, may be NULL), and
+ * restores the state and counters */
 int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best);
 int aht_set_launch(aht_solver *s, const aht_launch *l);
 int aht_get_launch(const aht_solver *s, aht_launch *l);
