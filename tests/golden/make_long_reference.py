@@ -14,8 +14,8 @@ fluid-volume heat content — in float64. From those it records what the GPU tes
 Usage: python tests/golden/make_long_reference.py --case c1                (the committed record)
        python tests/golden/make_long_reference.py --case c2s --every 64 [--threads 6] [--ckpt DIR] [--ckpt-every 100]
        python tests/golden/make_long_reference.py --case c2m --every 64 --threads 6 --ckpt DIR
-The 600 x 600 x 64 cases take about 20 minutes (c2s) and an hour (c2m) on 6-8 cores; --ckpt resumes an interrupted
-run.
+The 600 x 600 x 64 cases take about 20 minutes (c2s) and 2.5 hours (c2m: 2.2-2.3 s per step on 6 threads) on 6-8
+cores; --ckpt resumes an interrupted run.
 """
 import argparse
 import json

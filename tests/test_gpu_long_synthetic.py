@@ -9,7 +9,8 @@ host (checked by the column fingerprint), runs the same protocol through the C-A
   * heat-budget closure (advected out / injected) within 0.5 percentage points, heat content within 1e-4 relative,
     mass imbalance within 5e-5 absolute (the bars of the reference's own validation).
 Cases: c1 = the 200 x 200 x 32 config's full protocol (2,000 steps + the 10-minute mean); c2s = the 600 x 600 x 64
-domain with a shortened protocol (1,000 + 300 steps).
+domain with a shortened protocol (1,000 + 300 steps); c2m = the same domain past the start-up transient (3,000 steps,
+then a 1,000-step mean).
 """
 import glob
 import json
