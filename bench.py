@@ -171,8 +171,8 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f'bench.py: one GPU only (asked for --gpus {a.gpus}): scenarios are independent per GPU and a single '
-              f'domain would be cut into y-slabs with a one-plane halo exchange per step, which this build does not '
-              f'run', file=sys.stderr)
+              f'domain would be cut into y-slabs with a one-plane halo exchange per step, which this library does '
+              f'not run', file=sys.stderr)
         sys.exit(2)
     m = load_pkg()
     tmp = None

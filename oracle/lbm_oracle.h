@@ -2,7 +2,7 @@
  * CPU oracle of the coupled D3Q19 + D3Q7 lattice-Boltzmann step — TEST INFRASTRUCTURE ONLY.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this library, and only as the
- * checker (or as the timed CPU baseline). The product library (libaht.so) never links it and has no CPU path.
+ * reference the GPU results are compared with (or as the timed CPU baseline). The product library (libaht.so) never links it and has no CPU path.
  *
  * It restates the reference kernel generator js/live/kernel.js (step :395-431 with its sections :102-386,
  * near-surface read :435-471, init :524-553, geometry reinit :557-575, diagnostics :580-633) as plain C, one

@@ -1,7 +1,7 @@
 """CPU oracle bindings — TEST INFRASTRUCTURE ONLY.
 
-Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, as the checker or as
-the timed CPU baseline. The product (libaht.so and its hosts) never imports it and has no CPU fallback.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, as the reference the GPU
+results are compared with or as the timed CPU baseline. The product (libaht.so and its hosts) never imports it and has no CPU fallback.
 
 Wraps oracle/lib/liblbm_oracle.so (a C restatement of the reference kernel generator js/live/kernel.js; see
 lbm_oracle.h) and the domain-dump format written by ashburn-heat-twin_amd/js/dump.js. Also restates the

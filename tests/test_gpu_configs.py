@@ -7,7 +7,7 @@
   reports 0.2 % for its 10 m prototype, README.md:45; the oracle's 200 x 200 x 32 protocol gives 0.20 %), a
   positive mean warming below its peak, no non-finite cell. The oracle pins the same domain over 1,300 steps
   (test_gpu_long_synthetic.py) and the single-step bar below.
-* 600 x 600 x 64, north_star's single-step bar at full size: after 1,000 GPU steps the state is read back, the oracle
+* 600 x 600 x 64, the single-step bar at full size: after 1,000 GPU steps the state is read back, the oracle
   takes one step from exactly that state on the host and the GPU one more step; the theta fields (sum of the D3Q7
   populations) and the force-corrected velocities agree to relative L2 <= 1e-5.
 * 1200 x 1200 x 96 (reduced-precision storage): f16 and bf16 deviations and f16 without theta plane against the FP64

@@ -5,8 +5,8 @@ backflow-clipped and prototype outlets, the free-slip lid, bounce-back on terrai
 both sponges and the rooftop sources. Bar: the f32 kernel and the f32 oracle evaluate the same expressions in the
 The HIP kernel evaluates the reference physics with reassociated f32 arithmetic (population pairs, reciprocals,
 FMA); the oracle keeps the generator's expression order. Bars: the initial state is bit-identical; after one step
-states and outputs agree to relative L2 <= 1e-6 (a few f32 ulps); after 50 steps to <= 1e-5 (north_star's
-single-step bar on theta is 1e-5).
+states and outputs agree to relative L2 <= 1e-6 (a few f32 ulps); after 50 steps to <= 1e-5 (the single-step bar
+on theta is 1e-5).
 """
 import numpy as np
 import pytest
@@ -92,7 +92,7 @@ def test_tall_towers_match_oracle(tmp_path, pkg, O):
 
 
 def test_single_step_theta_from_developed_state(tmp_path, pkg, O):
-    """north_star bar: one step from an identical developed state, relative L2 of theta <= 1e-5"""
+    """the single-step bar: one step from an identical developed state, relative L2 of theta <= 1e-5"""
     dom = tiny(tmp_path, 210, 1, 5)
     o, s = pair(pkg, O, dom)
     with s:

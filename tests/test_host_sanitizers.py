@@ -1,7 +1,7 @@
 """Host code under the sanitizers (CPU only): the oracle and the N-API addon built with AddressSanitizer and
 UndefinedBehaviorSanitizer (the addon also with ThreadSanitizer), and the host build of the blocked-pull address
-functions with both. The HIP library's host code is not run here: it needs a GPU, and the GPU boxes run no
-sanitizer builds. Errors abort the sanitized process (-fno-sanitize-recover), so a clean exit status is the check;
+functions with both. The HIP library's host code is not covered: it needs a GPU, and its tests there run the
+uninstrumented build. Errors abort the sanitized process (-fno-sanitize-recover), so a clean exit status is the check;
 leak reports are off (the addon's handles are finalised by the JS garbage collector, which need not run at exit)."""
 import json
 import os

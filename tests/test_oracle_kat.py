@@ -1,7 +1,7 @@
 """The CPU oracle and the JavaScript host pinned against the reference's recorded known answers
 (tests/live/validation_summary.json, kept as tests/golden/reference_kats.json) and against closed forms.
 
-These run on the CPU only. They pin the checker the GPU parity tests rely on:
+These run on the CPU only. They pin the oracle the GPU parity tests rely on:
   * host parameter block and dt at dx = 20 / 30 / 40 (recorded f32 values, js/live/solver.js:225-238);
   * Ashburn domain facts (zbase, kt range, ktop max, building columns, nz, fluid cells; js/live/domain.js:166-190);
   * the initial state, bit-identical to the numpy closed form (tools/live/validate_live.py:132-143);
