@@ -41,13 +41,15 @@ class Dims(ctypes.Structure):
 
 
 class Launch(ctypes.Structure):
-    """launch shape of the step kernel: threads per block, cells per thread along z, tile window (1 = linear), and the
-    two-step sweep (0 = off; +kc: 32 x 8 column regions, -kc: 16 x 16, level chunks of kc)"""
+    """launch shape of the step kernel: threads per block, cells per thread along z, tile window (1 = linear), the
+    two-step sweep (0 = off; +kc: 32 x 8 column regions, -kc: 16 x 16, level chunks of kc) and the cap of wavefronts
+    per SIMD (0 = none)"""
     _fields_ = [('block', ctypes.c_int32), ('levels', ctypes.c_int32), ('window', ctypes.c_int32),
-                ('sweep', ctypes.c_int32)]
+                ('sweep', ctypes.c_int32), ('waves', ctypes.c_int32)]
 
     def as_dict(self):
-        return {'block': self.block, 'levels': self.levels, 'window': self.window, 'sweep': self.sweep}
+        return {'block': self.block, 'levels': self.levels, 'window': self.window, 'sweep': self.sweep,
+                'waves': self.waves}
 
 
 PARAM_F = ('tau0', 'cs2smag', 'gbeta', 'prt', 'kfloor', 'momfac', 'srcscale', 'sigmax')
@@ -241,14 +243,15 @@ class Solver:
         return int(lib().aht_steps_done(self.h))
 
     def autotune(self, steps=20):
-        """the launch shape {block, levels, window, sweep} the autotuner keeps (now set); the state is left unchanged"""
+        """the launch shape {block, levels, window, sweep, waves} the autotuner keeps (now set); the state is left
+        unchanged"""
         best = Launch()
         self._check(lib().aht_autotune(self.h, int(steps), ctypes.byref(best)))
         return best.as_dict()
 
-    def set_launch(self, block=256, levels=1, window=1, sweep=0):
+    def set_launch(self, block=256, levels=1, window=1, sweep=0, waves=0):
         self._check(lib().aht_set_launch(self.h, ctypes.byref(Launch(int(block), int(levels), int(window),
-                                                                     int(sweep)))))
+                                                                     int(sweep), int(waves)))))
 
     @property
     def launch(self):

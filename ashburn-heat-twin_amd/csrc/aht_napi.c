@@ -631,6 +631,7 @@ static napi_value launch_obj(napi_env env, const aht_launch *l) {
   set_num(env, o, "levels", l->levels);
   set_num(env, o, "window", l->window);
   set_num(env, o, "sweep", l->sweep);
+  set_num(env, o, "waves", l->waves);
   return o;
 }
 
@@ -657,7 +658,8 @@ static napi_value js_set_launch(napi_env env, napi_callback_info info) {
   aht_launch l;
   memset(&l, 0, sizeof l);
   if (!get_prop_i32(env, argv[1], "block", &l.block, 256) || !get_prop_i32(env, argv[1], "levels", &l.levels, 1) ||
-      !get_prop_i32(env, argv[1], "window", &l.window, 1) || !get_prop_i32(env, argv[1], "sweep", &l.sweep, 0))
+      !get_prop_i32(env, argv[1], "window", &l.window, 1) || !get_prop_i32(env, argv[1], "sweep", &l.sweep, 0) ||
+      !get_prop_i32(env, argv[1], "waves", &l.waves, 0))
     return NULL;
   const int rc = aht_set_launch(h->s, &l);
   if (rc) return throw_rc(env, h->s, rc);

@@ -57,6 +57,8 @@ struct aht_solver {
   int block = aht::STEP_BLOCK;
   int levels = 1;             // cells per thread along z
   int sweep = 0;              // aht_launch::sweep: 0 = one step per launch, else the two-step sweep's chunk / region
+  int waves = 0;              // aht_launch::waves: wavefronts per SIMD cap of the one-step kernel (0 = none)
+  uint32_t lds_cu = 0, lds_block = 0;   // the device's LDS per CU and per block (bytes)
   int storage = AHT_STORE_F32;
   int eb = 4;                 // bytes per stored population element
   size_t fbytes = 0;          // bytes of one D3Q19 lattice (a blocked f32 pair: the whole array, D3Q7 included)
@@ -196,6 +198,13 @@ int aht_create(const aht_dims *dims, aht_solver **out) {
   *out = s;
   int rc = set_device(s);
   if (rc) return rc;
+  {
+    int cu = 0, blk = 0;
+    HIP_TRY(s, hipDeviceGetAttribute(&cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, s->device));
+    HIP_TRY(s, hipDeviceGetAttribute(&blk, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
+    s->lds_cu = (uint32_t)std::max(cu, 0);
+    s->lds_block = (uint32_t)std::max(blk, 0);
+  }
   HIP_TRY(s, hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
   HIP_TRY(s, hipEventCreate(&s->ev0));
   HIP_TRY(s, hipEventCreate(&s->ev1));
@@ -404,7 +413,8 @@ int aht_step(aht_solver *s, int32_t n, uint32_t flags) {
     float *o4 = (out && it == n - 1) ? s->out4 : nullptr;
     if (o4) HIP_TRY(s, hipMemsetAsync(o4, 0, 4 * (size_t)s->g.N * sizeof(float), s->st));
     HIP_TRY(s, aht::launch_step(s->storage, s->g, &s->gdev->g, &s->gdev->s[p], s->f[p], s->f[1 - p], s->gq[p],
-                                s->gq[1 - p], acc ? &a : nullptr, o4, s->block, s->levels, (uint32_t)s->steps, s->st));
+                                s->gq[1 - p], acc ? &a : nullptr, o4, s->block, s->levels, (uint32_t)s->steps, s->st,
+                                aht::occupancy_lds(s->block, s->waves, s->lds_cu, s->lds_block)));
     s->parity ^= 1;
     s->steps++;
     if (acc) s->n_acc++;
@@ -1021,12 +1031,17 @@ int check_launch(aht_solver *s, const aht_launch &l) {
   if (l.window != -1 && (l.window < 0 || l.window > 4096 || (l.window & (l.window - 1))))
     return fail(s, AHT_EINVAL, "window must be -1 (plane slabs), 0 (slabs) or a power of two up to 4096");
   if (l.sweep < -1023 || l.sweep > 1023) return fail(s, AHT_EINVAL, "sweep chunk must be at most 1023 levels");
+  if (l.waves < 0 || l.waves > 8) return fail(s, AHT_EINVAL, "waves must be 0 (no cap) or 1 to 8");
+  if (l.waves && !aht::occupancy_lds(l.block, l.waves, s->lds_cu, s->lds_block))
+    return fail(s, AHT_EINVAL, "a cap of %d wavefronts per SIMD cannot be set for blocks of %d threads on this device",
+                l.waves, l.block);
   return AHT_OK;
 }
 void apply_launch(aht_solver *s, const aht_launch &l) {
   s->block = l.block;
   s->levels = l.levels;
   s->sweep = l.sweep;
+  s->waves = l.waves;
   s->g.win = (uint32_t)l.window;
   s->gdirty = true;
 }
@@ -1058,7 +1073,7 @@ int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best) {
     double ms;
   };
   std::vector<Cand> cand;
-  aht_launch keep{s->block, s->levels, (int32_t)s->g.win, s->sweep};
+  aht_launch keep{s->block, s->levels, (int32_t)s->g.win, s->sweep, s->waves};
   auto time_shape = [&](const aht_launch &l, int32_t n, double &ms) {
     apply_launch(s, l);
     int r = aht_step(s, 1, 0);
@@ -1070,9 +1085,23 @@ int aht_autotune(aht_solver *s, int32_t steps, aht_launch *best) {
       for (int w : cw) {
         if (rc) break;
         double ms = 0;
-        rc = time_shape(aht_launch{b, l, w, 0}, steps, ms);
-        cand.push_back(Cand{aht_launch{b, l, w, 0}, ms / steps});
+        rc = time_shape(aht_launch{b, l, w, 0, 0}, steps, ms);
+        cand.push_back(Cand{aht_launch{b, l, w, 0, 0}, ms / steps});
       }
+  // the three fastest shapes again with their wavefronts capped at 5 and 4 per SIMD
+  if (!rc) {
+    std::sort(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+    const size_t top = std::min<size_t>(cand.size(), 3);
+    for (size_t t = 0; t < top && !rc; t++)
+      for (int wv : {5, 4}) {
+        aht_launch l = cand[t].l;
+        l.waves = wv;
+        if (!aht::occupancy_lds(l.block, wv, s->lds_cu, s->lds_block)) continue;
+        double ms = 0;
+        if ((rc = time_shape(l, steps, ms))) break;
+        cand.push_back(Cand{l, ms / steps});
+      }
+  }
   if (!rc) {
     std::sort(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
     cand.resize(std::min<size_t>(cand.size(), 6));
@@ -1109,7 +1138,7 @@ int aht_set_launch(aht_solver *s, const aht_launch *l) {
 
 int aht_get_launch(const aht_solver *s, aht_launch *l) {
   if (!s || !l) return AHT_EINVAL;
-  *l = aht_launch{s->block, s->levels, (int32_t)s->g.win, s->sweep};
+  *l = aht_launch{s->block, s->levels, (int32_t)s->g.win, s->sweep, s->waves};
   return AHT_OK;
 }
 

@@ -93,9 +93,12 @@ hipError_t launch_zero_g_blocked(int storage, const Grid &g, void *gq, hipStream
 // (as a by-value argument the grid was pinned in SGPRs, which spilled into VGPR lanes: ~120 v_readlane/v_writelane
 // per cell of the f32 fast path); sd = device copy of make_streams(storage, g, fA, fB, gA, gB)
 // step = the solver's step index of this launch (random bits of the stochastic-rounding formats)
+// dynamic LDS (bytes) that caps a kernel of `block` threads at `waves` wavefronts per SIMD on a CU with `lds_cu`
+// bytes of LDS (0: no cap, or the cap cannot be expressed within `lds_block` bytes per block)
+uint32_t occupancy_lds(int block, int waves, uint32_t lds_cu, uint32_t lds_block);
 hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const Streams *sd, const void *fA, void *fB,
                        const void *gA, void *gB, const Acc *acc, float *out4, int block, int levels, uint32_t step,
-                       hipStream_t st);
+                       hipStream_t st, uint32_t lds = 0);
 Streams make_streams(int storage, const Grid &g, const void *fA, void *fB, const void *gA, void *gB);
 // two steps per launch (the z sweep with LDS-staged halo planes; f32 pair layout only, no means or debug outputs):
 // regions of tx x (256 / tx) columns (tx = 16 or 32), chunks of kc levels, tiles in the XCD-aware order win (plane

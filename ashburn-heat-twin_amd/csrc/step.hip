@@ -1534,29 +1534,42 @@ __global__ __launch_bounds__(256) void zero_g_blocked_kernel(typename S::elem *g
 
 template <class S, int KB>
 static void launch_step_kb(const Grid &g, const Grid *gd, const Streams *sd, const void *fA, void *fB, const void *gA,
-                           void *gB, const Acc *acc, float *out4, int block, uint32_t step, hipStream_t st) {
+                           void *gB, const Acc *acc, float *out4, int block, uint32_t step, hipStream_t st,
+                           uint32_t lds) {
   const uint32_t bx = (g.NXY + block - 1) / block;
   const uint32_t nk = (uint32_t)(g.nz - g.kmin + KB - 1) / KB;
   const uint32_t sw = g.win == WIN_PLANE_SLABS ? (bx + NUM_XCD - 1) / NUM_XCD : 0;   // plane slabs: blocks per slab
   const dim3 grid(sw ? NUM_XCD * sw * nk : bx * nk);
   const Acc a = acc ? *acc : Acc{nullptr, nullptr, nullptr};
   const float ib = 1.0f / (float)bx, isw = sw ? 1.0f / (float)sw : 0.0f;
-  if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw, step);
-  else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw, step);
-  else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw, step);
-  else hipLaunchKernelGGL((step_kernel<S, false, false, KB>), grid, dim3(block), 0, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw, step);
+  if (acc && out4) hipLaunchKernelGGL((step_kernel<S, true, true, KB>), grid, dim3(block), lds, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw, step);
+  else if (acc) hipLaunchKernelGGL((step_kernel<S, true, false, KB>), grid, dim3(block), lds, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw, step);
+  else if (out4) hipLaunchKernelGGL((step_kernel<S, false, true, KB>), grid, dim3(block), lds, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw, step);
+  else hipLaunchKernelGGL((step_kernel<S, false, false, KB>), grid, dim3(block), lds, st, gd, sd, fA, fB, gA, gB, a, out4, bx, ib, sw, isw, step);
 }
 
 hipError_t launch_step(int storage, const Grid &g, const Grid *gd, const Streams *sd, const void *fA, void *fB,
                        const void *gA, void *gB, const Acc *acc, float *out4, int block, int levels, uint32_t step,
-                       hipStream_t st) {
+                       hipStream_t st, uint32_t lds) {
   if (g.kmin >= g.nz) return hipSuccess;
   AHT_DISPATCH(storage, g.lb, {
-    if (levels == 4) (launch_step_kb<S, 4>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, step, st);
-    else if (levels == 2) (launch_step_kb<S, 2>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, step, st);
-    else (launch_step_kb<S, 1>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, step, st);
+    if (levels == 4) (launch_step_kb<S, 4>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, step, st, lds);
+    else if (levels == 2) (launch_step_kb<S, 2>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, step, st, lds);
+    else (launch_step_kb<S, 1>)(g, gd, sd, fA, fB, gA, gB, acc, out4, block, step, st, lds);
   })
   return hipGetLastError();
+}
+
+// A CU's 4 SIMDs take blocks of block / 64 wavefronts each; at most `waves` per SIMD means at most 256 waves / block
+// blocks per CU, which a reservation of lds_cu / blocks bytes (rounded down to 1 KiB, the allocation step assumed
+// here) enforces: one more block would need more LDS than the CU has. The step kernel uses no LDS itself.
+uint32_t occupancy_lds(int block, int waves, uint32_t lds_cu, uint32_t lds_block) {
+  if (waves <= 0 || block <= 0) return 0;
+  const uint32_t blocks = (uint32_t)(256 * waves / block);
+  if (blocks == 0) return 0;
+  const uint32_t lds = lds_cu / blocks / 1024u * 1024u;
+  if (lds == 0 || lds > lds_block || (uint64_t)(blocks + 1) * lds <= lds_cu) return 0;
+  return lds;
 }
 
 bool sweep2_supported(int storage, int lb) { return storage == 0 && lb == StoreF32B::LB; }

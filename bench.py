@@ -158,7 +158,7 @@ def main():
     ap.add_argument('--workload', default='c2')
     ap.add_argument('--storage', type=int, default=0,
                     help='0 f32, 1 f16 deviations, 2 bf16 deviations, 3 f64 twin, 4 f16 without theta plane (SR)')
-    ap.add_argument('--launch', default=None, help='fixed launch shape block,levels,window[,sweep] (no tuning)')
+    ap.add_argument('--launch', default=None, help='fixed launch shape block,levels,window[,sweep[,waves]] (no tuning)')
     ap.add_argument('--tune-steps', type=int, default=12, help='steps per launch-shape candidate of the autotuner')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--no-cpu', action='store_true')

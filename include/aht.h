@@ -199,6 +199,11 @@ typedef struct {
                             Bit-identical results. Opt-in, the autotuner does not try it: at 600 x 600 x 64 it
                             matches the one-step kernel at best (sweep 12, window 4), on smaller grids it is
                             slower                                                                          */
+  int32_t waves;         /* 0: as many wavefronts per SIMD as the kernel's registers allow. 1-8: at most that many
+                            of the one-step kernel's wavefronts share a SIMD (the launch reserves LDS the kernel
+                            does not use, so that fewer blocks fit on a CU). Bit-identical results. At
+                            600 x 600 x 64 the f32 step runs 3 % faster at 4 (0.799 against 0.823 ms at its
+                            register-bound 6); the 16-bit steps run slower with a cap                       */
 } aht_launch;
 
 /* launch-shape autotuner of one solver: times every block x levels x window candidate over `steps` steps each,

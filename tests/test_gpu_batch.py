@@ -194,20 +194,23 @@ def test_autotune_restores_state(tmp_path, pkg):
         s.step(5)
         f0, g0 = s.read_state()
         best = s.autotune(3)
-        assert best['block'] in (64, 128, 256) and best['levels'] in (1, 2, 4) and best['window'] in (-1, 0, 1, 2, 4, 16)
+        assert best['block'] in (64, 128, 256) and best['levels'] in (1, 2, 4) and best['window'] in (-1, 0, 1, 2, 4, 8, 16)
+        assert best['waves'] in (0, 4, 5)
         assert s.launch == best
         f1, g1 = s.read_state()
         assert np.array_equal(f0, f1) and np.array_equal(g0, g1) and s.steps == 5
 
 
-@pytest.mark.parametrize('block,levels,window', [(256, 1, 1), (128, 2, 4), (64, 4, 0), (192, 1, 16), (256, 4, 2),
-                                                  (256, 1, -1), (64, 2, -1)])
-def test_launch_shapes_give_identical_states(tmp_path, pkg, block, levels, window):
-    """every launch shape updates every cell with the same arithmetic: states and FP64 means are bit-identical to the
-    default"""
+@pytest.mark.parametrize('block,levels,window,waves', [(256, 1, 1, 0), (128, 2, 4, 0), (64, 4, 0, 0), (192, 1, 16, 0),
+                                                        (256, 4, 2, 0), (256, 1, -1, 0), (64, 2, -1, 0), (256, 1, 1, 4),
+                                                        (128, 1, 8, 5), (64, 2, 1, 3), (192, 1, 1, 4)])
+def test_launch_shapes_give_identical_states(tmp_path, pkg, block, levels, window, waves):
+    """every launch shape (and cap of wavefronts per SIMD) updates every cell with the same arithmetic: states and FP64
+    means are bit-identical to the default"""
     dom = domains(tmp_path)[2]
     with pkg.Solver.from_dump(dom) as a, pkg.Solver.from_dump(dom) as b:
-        b.set_launch(block, levels, window)
+        b.set_launch(block, levels, window, 0, waves)
+        assert b.launch['waves'] == waves
         for s in (a, b):
             s.step(6)
             s.mean_reset()
@@ -218,3 +221,5 @@ def test_launch_shapes_give_identical_states(tmp_path, pkg, block, levels, windo
         assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32))
         (sa, ba), (sb, bb) = a.read_mean(), b.read_mean()
         assert np.array_equal(sa, sb) and ba == bb
+        with pytest.raises(pkg.AhtError):
+            b.set_launch(256, 1, 1, 0, 9)

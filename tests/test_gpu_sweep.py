@@ -88,7 +88,7 @@ def test_sweep_launch_field(tmp_path, pkg):
     dom = tiny(tmp_path, 2)
     with pkg.Solver.from_dump(dom) as s:
         s.set_launch(128, 2, 4, -16)
-        assert s.launch == {'block': 128, 'levels': 2, 'window': 4, 'sweep': -16}
+        assert s.launch == {'block': 128, 'levels': 2, 'window': 4, 'sweep': -16, 'waves': 0}
         with pytest.raises(pkg.AhtError):
             s.set_launch(256, 1, 1, 5000)
     with pkg.Solver.from_dump(dom, storage='f16s') as a, pkg.Solver.from_dump(dom, storage='f16s') as b:
