@@ -718,15 +718,20 @@ struct aht_batch {
   // the captured chunk: valid while the block size and every launch's (members, tiles, grid) are unchanged (a change
   // of a member's columns or levels changes its tile count, and with it the launches)
   hipGraphExec_t gexec = nullptr;
-  int g_block = -1, g_grid = -1;
+  int g_block = -1, g_grid = -1, g_waves = -1;
   std::vector<uint64_t> g_key;
-  // online tuner: (block, grid) candidates timed on real chunks as device time per cell-step (chunks with different
-  // members compare), tuned again when the chunk's cell count moves by more than 2x from where the last tuning started
-  std::vector<std::pair<int, int>> cand;
+  // online tuner: (block, grid, wavefronts-per-SIMD cap) candidates timed on real chunks as device time per cell-step
+  // (chunks with different members compare), tuned again when the chunk's cell count moves by more than 2x from where
+  // the last tuning started
+  struct Cand {
+    int block, grid, waves;
+  };
+  std::vector<Cand> cand;
   std::vector<double> cand_ms;
   int tune_next = 0;
   double tune_cells = 0.0;
-  int last_block = 0, last_grid = 0;   // used by the most recent chunk
+  int last_block = 0, last_grid = 0, last_waves = 0;   // used by the most recent chunk
+  uint32_t lds_cu = 0, lds_block = 0;                  // the device's LDS per CU and per block (bytes)
   int64_t steps = 0;
   std::string err;
 };
@@ -759,13 +764,17 @@ int cu_count(int device) {
 constexpr int ONE_BLOCK_PER_TILE = 1 << 30;
 
 // candidates of the batch tuner: fixed values or, when tuned, block sizes 256 / 128 / 64 and grids of one block per
-// tile or 32 persistent blocks per CU
+// tile or 32 persistent blocks per CU; one block per tile also with its wavefronts capped at 5 and 4 per SIMD
 void reset_tuner(aht_batch *b) {
   std::vector<int> blocks = b->block ? std::vector<int>{b->block} : std::vector<int>{256, 128, 64};
   std::vector<int> grids = b->grid ? std::vector<int>{b->grid} : std::vector<int>{ONE_BLOCK_PER_TILE, 32 * cu_count(b->device)};
   b->cand.clear();
   for (int g : grids)
-    for (int bl : blocks) b->cand.emplace_back(bl, g);
+    for (int bl : blocks)
+      for (int wv : {0, 5, 4}) {
+        if (wv && (g != ONE_BLOCK_PER_TILE || !aht::occupancy_lds(bl, wv, b->lds_cu, b->lds_block))) continue;
+        b->cand.push_back(aht_batch::Cand{bl, g, wv});
+      }
   b->cand_ms.assign(b->cand.size(), 0.0);
   b->tune_next = 0;
   b->tune_cells = 0.0;
@@ -800,6 +809,11 @@ int aht_batch_create(int32_t device, int32_t storage, aht_batch **out) {
   HIP_TRYB(b, hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
   HIP_TRYB(b, hipEventCreate(&b->ev0));
   HIP_TRYB(b, hipEventCreate(&b->ev1));
+  int cu = 0, blk = 0;
+  HIP_TRYB(b, hipDeviceGetAttribute(&cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
+  HIP_TRYB(b, hipDeviceGetAttribute(&blk, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  b->lds_cu = (uint32_t)std::max(cu, 0);
+  b->lds_block = (uint32_t)std::max(blk, 0);
   reset_tuner(b);
   return AHT_OK;
 }
@@ -902,9 +916,11 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
   } else {
     pick = 0;
   }
-  const int block = b->cand[pick].first, grid = b->cand[pick].second;
+  const int block = b->cand[pick].block, grid = b->cand[pick].grid, waves = b->cand[pick].waves;
+  const uint32_t lds = aht::occupancy_lds(block, waves, b->lds_cu, b->lds_block);
   b->last_block = block;
   b->last_grid = grid;
+  b->last_waves = waves;
   // member table, longest-stepping members first: the members that step in launch `it` are then a prefix of it
   b->order.resize(nm);
   for (size_t d = 0; d < nm; d++) b->order[d] = (int)d;
@@ -958,10 +974,10 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
   const int lb = b->mem[0]->g.lb;
   auto enqueue = [&](int it) {
     const auto &L = launches[(size_t)it];
-    return aht::launch_batch_step(b->storage, lb, block, b->dmem, L.first, L.second, it, grid, b->st);
+    return aht::launch_batch_step(b->storage, lb, block, b->dmem, L.first, L.second, it, grid, b->st, lds);
   };
   if (b->use_graph) {
-    if (!b->gexec || b->g_block != block || b->g_grid != grid || b->g_key != key) {
+    if (!b->gexec || b->g_block != block || b->g_grid != grid || b->g_waves != waves || b->g_key != key) {
       if (b->gexec) HIP_TRYB(b, hipGraphExecDestroy(b->gexec));
       b->gexec = nullptr;
       hipGraph_t graph;
@@ -979,6 +995,7 @@ int aht_batch_step(aht_batch *b, int32_t n, const int32_t *nsteps, const int32_t
       HIP_TRYB(b, hipGraphDestroy(graph));
       b->g_block = block;
       b->g_grid = grid;
+      b->g_waves = waves;
       b->g_key = key;
     }
     HIP_TRYB(b, hipEventRecord(b->ev0, b->st));       // after the (host-side) capture: times the device work only

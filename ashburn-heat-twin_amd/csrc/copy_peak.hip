@@ -24,7 +24,7 @@
 // and `lattice16s_pair_{noshift,zonly,xonly,yonly}_nt` take the pair patterns apart: every stream read at the cell
 // itself, or pulled along z, x or y only; `lattice_pair_pull_nt_w{2,4,16,64}` and `lattice16s_pair_pull_nt_w{...}`
 // give the pull pattern the step's XCD-aware tile order (runs of W consecutive 256-cell tiles per XCD).
-// Usage: aht_copy_peak [bytes] [reps] [variant]
+// Usage: aht_copy_peak [bytes] [reps] [variant] [waves] (waves: cap of wavefronts per SIMD for the two pair patterns)
 //   (variant: all | grid | flat | unroll4 | unroll4_xcd | unroll4_nt | lattice | lattice_pull | lattice_aa_even
 //    | lattice_aa_odd | lattice_pull_nt | lattice16_pull_nt | lattice_aosoa_pull_nt | lattice_pull_nt_x2
 //    | lattice_pull_nt_x4 | lattice_blocked_pull_nt | lattice_pair_pull_nt | lattice16_pair_pull_nt
@@ -377,14 +377,19 @@ int main(int argc, char **argv) {
   const size_t bytes = argc > 1 ? strtoull(argv[1], nullptr, 10) : (size_t)4 << 30;
   const int reps = argc > 2 ? atoi(argv[2]) : 20;
   const char *which = argc > 3 ? argv[3] : "all";
+  // optional cap of wavefronts per SIMD for the pair patterns (15, 17): LDS reserved per 256-thread block so that at
+  // most `waves` blocks fit on a CU (the copies use no LDS themselves); 0 = none
+  const int waves = argc > 4 ? atoi(argv[4]) : 0;
   const size_t n = bytes / 16;
   float4 *a = nullptr, *b = nullptr;
   CK(hipMalloc(&a, n * 16));
   CK(hipMalloc(&b, n * 16));
   CK(hipMemset(a, 0, n * 16));
   CK(hipMemset(b, 0, n * 16));
-  int cus = 0;
+  int cus = 0, lds_cu = 0;
   CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+  CK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, 0));
+  const unsigned cap_lds = waves > 0 ? (unsigned)(lds_cu / waves) / 1024u * 1024u : 0u;   // 256 threads: waves blocks
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
@@ -429,9 +434,9 @@ int main(int argc, char **argv) {
         case 12: hipLaunchKernelGGL(copy_lattice_wide<2>, dim3((lN / 2 + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 13: hipLaunchKernelGGL(copy_lattice_wide<4>, dim3((lN / 4 + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 14: hipLaunchKernelGGL(copy_lattice_blocked<10>, dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
-        case 15: hipLaunchKernelGGL(copy_lattice_pair<11>, dim3((lN + 255) / 256), dim3(256), 0, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
+        case 15: hipLaunchKernelGGL(copy_lattice_pair<11>, dim3((lN + 255) / 256), dim3(256), cap_lds, 0, (const float *)a, (float *)b, lN, lnx, lnxy); break;
         case 16: hipLaunchKernelGGL((copy_lattice16_pair<8, true>), dim3((lN + 255) / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy); break;
-        case 17: hipLaunchKernelGGL((copy_lattice16_pair<8, false>), dim3((lN + 255) / 256), dim3(256), 0, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy); break;
+        case 17: hipLaunchKernelGGL((copy_lattice16_pair<8, false>), dim3((lN + 255) / 256), dim3(256), cap_lds, 0, (const uint16_t *)a, (uint16_t *)b, lN, lnx, lnxy); break;
         case 18: hipLaunchKernelGGL(copy_lattice_colmajor<11>, dim3(ncb * 64 * 8), dim3(256), 0, 0, (const float *)a, (float *)b, lnx, lnxy, 64); break;
         case 19: hipLaunchKernelGGL((copy_lattice_inplace_pair<11, 0, false>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
         case 20: hipLaunchKernelGGL((copy_lattice_inplace_pair<11, 1, false>), dim3((lN + 255) / 256), dim3(256), 0, 0, (float *)a, lN, lnx, lnx, 64); break;
@@ -466,7 +471,8 @@ int main(int argc, char **argv) {
                          : ((v == 17 || v >= 33) ? (size_t)26 * lN * 2 : (v >= 5 ? (size_t)26 * lN * 4 : n * 16));
     const double per = ms / reps, gbs = 2.0 * moved / (per * 1e-3) / 1e9;
     printf("{\"kernel\": \"copy_%s\", \"bytes_copied\": %zu, \"reps\": %d, \"ms_per_launch\": %.5f, "
-           "\"GBps_read_plus_write\": %.1f, \"cus\": %d}\n", names[v], moved, reps, per, gbs, cus);
+           "\"GBps_read_plus_write\": %.1f, \"cus\": %d, \"waves_cap\": %d}\n", names[v], moved, reps, per, gbs, cus,
+           (v == 15 || v == 17) ? waves : 0);
     if (v < 5 && gbs > best) { best = gbs; best_name = names[v]; }
   }
   printf("{\"copy_peak_GBps\": %.1f, \"variant\": \"%s\"}\n", best, best_name);

@@ -109,8 +109,9 @@ hipError_t launch_sweep2(int storage, const Grid &g, const Grid *gd, const void 
 
 // one step of the first nmem batch members (launch index `it` of a chunk; their tiles are the first total_tiles) on
 // a grid of min(grid, total_tiles) blocks of `block` (64, 128 or 256) threads, one level per thread
+// lds: occupancy_lds's reservation for a grid of one block per tile (a persistent grid launches without one)
 hipError_t launch_batch_step(int storage, int lb, int block, const Member *mem, int nmem, uint32_t total_tiles, int it,
-                             int grid, hipStream_t st);
+                             int grid, hipStream_t st, uint32_t lds = 0);
 hipError_t launch_init(int storage, const Grid &g, void *f0, void *f1, hipStream_t st);
 hipError_t launch_reinit(int storage, const Grid &g, const uint32_t *old_ktop, void *f, void *gq, hipStream_t st);
 hipError_t launch_surface(int storage, const Grid &g, const void *fA, const void *gA, float conv, float *out3,

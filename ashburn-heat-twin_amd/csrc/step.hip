@@ -1642,13 +1642,13 @@ hipError_t launch_zero_g_blocked(int storage, const Grid &g, void *gq, hipStream
 }
 
 hipError_t launch_batch_step(int storage, int lb, int block, const Member *mem, int nmem, uint32_t total_tiles, int it,
-                             int grid, hipStream_t st) {
+                             int grid, hipStream_t st, uint32_t lds) {
   if (total_tiles == 0 || nmem == 0) return hipSuccess;
   const uint32_t G = std::min<uint32_t>((uint32_t)grid, total_tiles);
   const bool persist = G < total_tiles;
 #define AHT_BATCH_LAUNCH(BS)                                                                                          \
   if (persist) hipLaunchKernelGGL((batch_step_kernel<S, BS, true>), dim3(G), dim3(BS), 0, st, mem, nmem, total_tiles, it); \
-  else hipLaunchKernelGGL((batch_step_kernel<S, BS, false>), dim3(G), dim3(BS), 0, st, mem, nmem, total_tiles, it);
+  else hipLaunchKernelGGL((batch_step_kernel<S, BS, false>), dim3(G), dim3(BS), lds, st, mem, nmem, total_tiles, it);
   AHT_DISPATCH(storage, lb, {
     if (block == 64) { AHT_BATCH_LAUNCH(64) }
     else if (block == 128) { AHT_BATCH_LAUNCH(128) }

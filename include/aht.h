@@ -177,9 +177,10 @@ int aht_batch_sync(aht_batch *b);
  * without graph capture */
 int aht_batch_set_grid(aht_batch *b, int32_t blocks, int32_t use_graph);
 /* threads per block of the batched step: 64, 128 or 256 (tiles of that many columns), 0 = online tuning over the
- * three. The tuner times one (block, grid) candidate per chunk as device time per cell-step and tunes again whenever
- * a chunk's cell-steps move by more than 2x from where the last tuning started; every candidate computes the same
- * bits (no counterpart in the reference: its kernel has one workgroup size, js/live/kernel.js:395) */
+ * three. The tuner times one (block, grid, cap) candidate per chunk as device time per cell-step, where a grid of one
+ * block per tile is also tried with its wavefronts capped at 5 and 4 per SIMD (aht_launch::waves), and tunes again
+ * whenever a chunk's cell-steps move by more than 2x from where the last tuning started; every candidate computes the
+ * same bits (no counterpart in the reference: its kernel has one workgroup size, js/live/kernel.js:395) */
 int aht_batch_set_block(aht_batch *b, int32_t block);
 int aht_batch_grid(const aht_batch *b);              /* blocks launched for all members by the last chunk (0: none yet) */
 int aht_batch_block(const aht_batch *b);             /* block size of the last chunk (0: none yet)                      */
